@@ -1,0 +1,225 @@
+"""damascene_amd -- MI355X-native gPb (global probability of boundary).
+
+Python host side over the C ABI of libdmsc.so (include/dmsc.h).  Device
+memory is held in torch tensors (plumbing only); every stage runs in the
+library's HIP kernels.  Stage functions mirror the reference's stage API
+(bryancatanzaro/damascene include/*.h): find_textons, kmeans, local_cues,
+combine, non_max_suppression, intervene, generalized_eigensolve,
+spectral_pb, calc_gpb, plus gpb() for the whole pipeline.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import DmscError, GpbOutputs, GpbParams, check, lib
+
+__all__ = [
+    "DmscError", "Context", "generate_image", "boundary_mask", "pitch_in_floats", "gpb",
+    "PREC_FP32", "PREC_MIXED",
+]
+
+PREC_FP32 = 0
+PREC_MIXED = 1
+TEXTON32 = 1
+
+
+def _p(t):
+    """Raw pointer of a tensor / ndarray (None -> NULL)."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def pitch_in_floats(n):
+    return lib().dmsc_pitch_in_floats(int(n))
+
+
+def generate_image(w, h, ncells, seed, with_cells=False):
+    """Synthetic RGB8 test image packed one uint32 per pixel (R in bits 0-7)."""
+    rgb = np.zeros((h, w), np.uint32)
+    cells = np.zeros((h, w), np.int32) if with_cells else None
+    check(lib().dmsc_generate_image(w, h, ncells, ctypes.c_uint64(seed), _p(rgb), _p(cells)))
+    return (rgb, cells) if with_cells else rgb
+
+
+def boundary_mask(cells):
+    h, w = cells.shape
+    m = np.zeros((h, w), np.uint8)
+    check(lib().dmsc_boundary_mask(w, h, _p(np.ascontiguousarray(cells, np.int32)), _p(m)))
+    return m
+
+
+class Context:
+    """A device + stream + scratch arenas (dmsc_ctx)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+
+        self._torch = torch
+        self.device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        check(lib().dmsc_ctx_create(ctypes.byref(h), device, ctypes.c_void_p(stream.cuda_stream)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().dmsc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().dmsc_ctx_synchronize(self.handle))
+
+    # -------------------------------------------------------------- buffers
+    def empty(self, *shape, dtype=None):
+        return self._torch.empty(*shape, dtype=dtype or self._torch.float32, device=self.device)
+
+    def zeros(self, *shape, dtype=None):
+        return self._torch.zeros(*shape, dtype=dtype or self._torch.float32, device=self.device)
+
+    def upload(self, arr):
+        t = self._torch.from_numpy(np.ascontiguousarray(arr))
+        if t.dtype == self._torch.uint32:
+            t = t.view(self._torch.int32)
+        return t.to(self.device)
+
+    # -------------------------------------------------------------- stages
+    def rgb_to_grey(self, rgb):
+        h, w = rgb.shape
+        out = self.empty(h, w)
+        check(lib().dmsc_rgb_to_grey(self.handle, w, h, _p(rgb), _p(out)))
+        return out
+
+    def rgb_to_lab(self, rgb, gamma=2.5, normalize=True):
+        h, w = rgb.shape
+        L, a, b = self.empty(h, w), self.empty(h, w), self.empty(h, w)
+        check(lib().dmsc_rgb_to_lab(self.handle, w, h, gamma, _p(rgb), _p(L), _p(a), _p(b)))
+        if normalize:
+            check(lib().dmsc_normalize_lab(self.handle, w, h, _p(L), _p(a), _p(b)))
+        return L, a, b
+
+    def texton_responses(self, grey):
+        h, w = grey.shape
+        out = self.empty(34, h, w)
+        check(lib().dmsc_texton_responses(self.handle, w, h, _p(grey), _p(out)))
+        return out
+
+    def kmeans(self, resp, precision=PREC_FP32, max_iter=15):
+        _, h, w = resp.shape
+        labels = self.empty(h, w, dtype=self._torch.int32)
+        ch, it = ctypes.c_int(), ctypes.c_int()
+        check(lib().dmsc_kmeans(self.handle, w, h, 32, 34, _p(resp), _p(labels), max_iter, 0, precision,
+                                ctypes.byref(ch), ctypes.byref(it)))
+        return labels, it.value
+
+    def find_textons(self, grey, precision=PREC_FP32):
+        h, w = grey.shape
+        labels = self.empty(h, w, dtype=self._torch.int32)
+        it = ctypes.c_int()
+        check(lib().dmsc_find_textons(self.handle, w, h, _p(grey), _p(labels), TEXTON32, precision,
+                                      ctypes.byref(it)))
+        return labels, it.value
+
+    def local_cues(self, L, a, b, textons, raw=False):
+        """Returns (bg, cga, cgb, tg) as [24, N] views of pitched stacks (and raw [4,3,8,N])."""
+        h, w = L.shape
+        n = w * h
+        pitch = pitch_in_floats(n)
+        cues = [self.zeros(24, pitch) for _ in range(4)]
+        if raw:
+            r = self.empty(4, 3, 8, n)
+            check(lib().dmsc_local_cues_raw(self.handle, w, h, _p(L), _p(a), _p(b), _p(textons),
+                                            *[_p(c) for c in cues], pitch, _p(r)))
+            return [c[:, :n] for c in cues], r
+        check(lib().dmsc_local_cues(self.handle, w, h, _p(L), _p(a), _p(b), _p(textons),
+                                    *[_p(c) for c in cues], pitch, TEXTON32))
+        return [c[:, :n] for c in cues]
+
+    def combine(self, w, h, bg, cga, cgb, tg):
+        """Cue stacks given as pitched [24, pitch] tensors; returns pitched (mpb, combinedG)."""
+        pitch = bg.shape[1]
+        mpb, cg = self.zeros(8, pitch), self.zeros(8, pitch)
+        check(lib().dmsc_combine(self.handle, w, h, pitch, _p(bg), _p(cga), _p(cgb), _p(tg), _p(mpb), _p(cg),
+                                 TEXTON32))
+        return mpb, cg
+
+    def non_max_suppression(self, w, h, pb):
+        out = self.empty(h, w)
+        check(lib().dmsc_nonmax(self.handle, w, h, _p(pb), pb.shape[1], _p(out)))
+        return out
+
+    def intervene(self, mpb, sigma=0.1):
+        h, w = mpb.shape
+        pitch = pitch_in_floats(w * h)
+        m = self.empty(81, pitch)
+        check(lib().dmsc_intervene(self.handle, w, h, _p(mpb), _p(m), pitch, sigma))
+        return m
+
+    def convert_matrix(self, w, h, m):
+        d, rs = self.empty(h, w), self.empty(h, w)
+        check(lib().dmsc_convert_matrix(self.handle, w, h, _p(m), m.shape[1], _p(d), _p(rs)))
+        return d, rs
+
+    def spmv(self, w, h, m, x):
+        y = self.empty(h, w)
+        check(lib().dmsc_stencil_spmv(self.handle, w, h, _p(m), m.shape[1], _p(x), _p(y)))
+        return y
+
+    def generalized_eigensolve(self, w, h, m, nvec=9, tol=1e-3, precision=PREC_FP32):
+        vals = np.zeros(nvec, np.float32)
+        vecs = self.empty(nvec, h, w)
+        it = ctypes.c_int()
+        check(lib().dmsc_generalized_eigensolve(self.handle, w, h, _p(m), m.shape[1], nvec, tol, precision,
+                                                _p(vals), _p(vecs), ctypes.byref(it)))
+        return vals, vecs, it.value
+
+    def spectral_pb(self, eigvals, eigvecs):
+        nvec, h, w = eigvecs.shape
+        pitch = pitch_in_floats(w * h)
+        out = self.zeros(8, pitch)
+        ev = np.ascontiguousarray(eigvals, np.float32)
+        check(lib().dmsc_spectral_pb(self.handle, _p(ev), _p(eigvecs), w, h, nvec, _p(out), pitch))
+        return out
+
+    def calc_gpb(self, w, h, cg, spb, nmax):
+        pitch = cg.shape[1]
+        all_ = self.zeros(8, pitch)
+        g = self.empty(h, w)
+        check(lib().dmsc_calc_gpb(self.handle, w * h, pitch, 8, _p(cg), _p(spb), _p(nmax), _p(all_), _p(g)))
+        return all_, g
+
+    def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False):
+        """Whole pipeline on a device RGB8 image (int32/uint32 [h, w] tensor)."""
+        h, w = rgb.shape
+        n = w * h
+        pitch = pitch_in_floats(n)
+        torch = self._torch
+        out = {
+            "textons": self.empty(h, w, dtype=torch.int32),
+            "mpb": self.zeros(8, pitch), "combined_g": self.zeros(8, pitch),
+            "nmax": self.empty(h, w), "eigvecs": self.empty(n_eig, h, w),
+            "spb": self.zeros(8, pitch), "gpb_all": self.zeros(8, pitch), "gpb": self.empty(h, w),
+        }
+        o = GpbOutputs()
+        o.d_textons, o.d_mpb, o.d_combined_g = out["textons"].data_ptr(), out["mpb"].data_ptr(), out["combined_g"].data_ptr()
+        o.d_nmax, o.d_eigvecs = out["nmax"].data_ptr(), out["eigvecs"].data_ptr()
+        o.d_spb, o.d_gpb_all, o.d_gpb = out["spb"].data_ptr(), out["gpb_all"].data_ptr(), out["gpb"].data_ptr()
+        o.pitch = pitch
+        p = GpbParams(n_eig, tol, TEXTON32, precision)
+        check(lib().dmsc_gpb(self.handle, w, h, _p(rgb), ctypes.byref(p), ctypes.byref(o), 1 if timing else 0))
+        out["eigvals"] = np.array(o.h_eigvals[:n_eig], np.float32)
+        out["kmeans_iterations"] = o.kmeans_iterations
+        out["lanczos_iterations"] = o.lanczos_iterations
+        out["stage_ms"] = list(o.stage_ms[:6])
+        out["pitch"] = pitch
+        return out

@@ -1,0 +1,79 @@
+"""Build libdmsc.so (HIP kernels for gfx950 + host C++) in-tree.
+
+    python -m damascene_amd.build        # or damascene_amd.build.build()
+
+Host table builders (tables.cpp, generator.cpp) are compiled by the system
+C++ compiler with the same floating-point flags as the CPU oracle
+(-ffp-contract=off -fno-builtin); everything else by hipcc for gfx950 with
+-ffp-contract=off and packed FP32 instructions disabled.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libdmsc.so")
+ARCH = os.environ.get("DMSC_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", CSRC, "-I", os.path.join(ROOT, "include")]
+HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                      "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-builtin", "-I", CSRC,
+              "-I", os.path.join(ROOT, "include"), "-Wall"]
+
+HOST_ONLY = ["tables.cpp", "generator.cpp"]
+
+
+def _sources():
+    out = []
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith(".hip") or f.endswith(".cpp"):
+            out.append(f)
+    return out
+
+
+def _compile(src):
+    obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
+    path = os.path.join(CSRC, src)
+    if src in HOST_ONLY:
+        cmd = [CXX] + HOST_FLAGS + ["-c", path, "-o", obj]
+    elif src.endswith(".hip"):
+        cmd = [HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", path, "-o", obj]
+    else:
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return src, obj, r.returncode, r.stdout + r.stderr, cmd
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = _sources()
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(_compile, srcs))
+    failed = [r for r in results if r[2] != 0]
+    for src, _, rc, log, cmd in results:
+        if verbose or rc != 0:
+            if log.strip():
+                print(f"--- {src}\n{log}", file=sys.stderr)
+    if failed:
+        raise RuntimeError("compilation failed: " + ", ".join(r[0] for r in failed))
+    objs = [r[1] for r in results]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

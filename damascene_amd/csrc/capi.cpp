@@ -1,0 +1,436 @@
+// capi.cpp -- extern "C" entry points of libdmsc.so (include/dmsc.h).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "dmsc_internal.hpp"
+#include "kernels.hpp"
+
+namespace dmsc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+Ctx::~Ctx() {
+  for (int i = 0; i < kArenas; i++)
+    if (arena[i]) (void)hipFree(arena[i]);
+  if (d_texton_coeffs) (void)hipFree(d_texton_coeffs);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+int Ctx::scratch(int slot, size_t bytes, void** ptr) {
+  if (slot < 0 || slot >= kArenas) return fail(DMSC_ERR_INTERNAL, "bad scratch slot %d", slot);
+  if (bytes > arena_bytes[slot]) {
+    if (arena[slot]) {
+      DMSC_HIP(hipStreamSynchronize(stream));
+      DMSC_HIP(hipFree(arena[slot]));
+      arena[slot] = nullptr;
+      arena_bytes[slot] = 0;
+    }
+    DMSC_HIP(hipMalloc(&arena[slot], bytes));
+    arena_bytes[slot] = bytes;
+  }
+  *ptr = arena[slot];
+  return DMSC_OK;
+}
+
+int Ctx::adopt(int slot, void* p, size_t bytes) {
+  if (arena[slot]) {
+    DMSC_HIP(hipStreamSynchronize(stream));
+    DMSC_HIP(hipFree(arena[slot]));
+  }
+  arena[slot] = p;
+  arena_bytes[slot] = bytes;
+  return DMSC_OK;
+}
+
+int Ctx::texton_coeffs(const float** out) {
+  if (!d_texton_coeffs) {
+    const TextonBank& b = texton_bank();
+    DMSC_HIP(hipMalloc(&d_texton_coeffs, sizeof(float) * b.coeffs.size()));
+    DMSC_HIP(hipMemcpy(d_texton_coeffs, b.coeffs.data(), sizeof(float) * b.coeffs.size(), hipMemcpyHostToDevice));
+  }
+  *out = d_texton_coeffs;
+  return DMSC_OK;
+}
+
+}  // namespace dmsc
+
+using namespace dmsc;
+
+#define CHECK_CTX(ctx) DMSC_ARG((ctx) != nullptr, "null context")
+#define CHECK_SIZE(w, h) \
+  DMSC_ARG((w) >= kMinSide && (h) >= kMinSide && (long)(w) * (h) < (1L << 28), "image size %dx%d out of range", (w), (h))
+
+extern "C" {
+
+const char* dmsc_version(void) { return "dmsc 0.1.0 (gfx950)"; }
+const char* dmsc_last_error_string(void) { return dmsc::g_last_error.c_str(); }
+
+int dmsc_ctx_create(dmsc_ctx** out, int device, void* stream) {
+  DMSC_ARG(out != nullptr, "null output pointer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(DMSC_ERR_NODEV, "no HIP device available");
+  DMSC_ARG(device >= 0 && device < count, "device %d out of range (%d devices)", device, count);
+  DMSC_HIP(hipSetDevice(device));
+  dmsc_ctx* c = new dmsc_ctx();
+  c->device = device;
+  // NULL selects the device's default stream (as in HIP/CUDA); callers that
+  // want a private stream create one and pass it in
+  c->stream = (hipStream_t)stream;
+  c->own_stream = false;
+  *out = c;
+  return DMSC_OK;
+}
+
+int dmsc_ctx_destroy(dmsc_ctx* ctx) {
+  if (!ctx) return DMSC_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return DMSC_OK;
+}
+
+int dmsc_ctx_synchronize(dmsc_ctx* ctx) {
+  CHECK_CTX(ctx);
+  DMSC_HIP(hipStreamSynchronize(ctx->stream));
+  return DMSC_OK;
+}
+
+void* dmsc_ctx_stream(dmsc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int dmsc_pitch_in_floats(int n) { return pitch_floats(n); }
+
+// ----------------------------------------------------------------- host tables
+int dmsc_texton_filters(float* coeffs, int* radii) {
+  DMSC_ARG(coeffs && radii, "null output");
+  const TextonBank& b = texton_bank();
+  memcpy(coeffs, b.coeffs.data(), sizeof(float) * b.coeffs.size());
+  memcpy(radii, b.radii.data(), sizeof(int) * b.radii.size());
+  return b.nbound;
+}
+int dmsc_parabola_filters(int radius, float* out) {
+  DMSC_ARG(out && radius > 0 && radius <= 64, "bad arguments");
+  parabola_filters(radius, out);
+  return DMSC_OK;
+}
+int dmsc_spb_filters(float* out) {
+  DMSC_ARG(out, "null output");
+  spb_filters(out);
+  return DMSC_OK;
+}
+int dmsc_rotation_tables(int width, int height, float thetaPi, int* nw, int* nh, int* bx, int* by, int* gx,
+                         int* gy) {
+  DMSC_ARG(width > 0 && height > 0 && nw && nh && bx && by && gx && gy, "bad arguments");
+  const int rc = rotation_tables(width, height, thetaPi, nw, nh, bx, by, gx, gy);
+  if (rc) return fail(DMSC_ERR_ARG, "rotation_tables: thetaPi %g outside (-1/2, 0]", thetaPi);
+  return DMSC_OK;
+}
+int dmsc_halfdisc_offsets(int w, int h, float thetaPi, int radius, const int* bx, const int* by, const int* gx,
+                          const int* gy, int rw, int* rect) {
+  DMSC_ARG(bx && by && gx && gy && rect, "null table");
+  halfdisc_offsets(w, h, thetaPi, radius, bx, by, gx, gy, rw, rect);
+  return DMSC_OK;
+}
+int dmsc_blur_kernel(float sigma, float* kernel, int* radius) {
+  DMSC_ARG(kernel && radius && sigma > 0 && sigma <= 20, "bad arguments");
+  return blur_kernel(sigma, kernel, radius);
+}
+int dmsc_stencil(int radius, int width, int* offsets, int* diag) {
+  DMSC_ARG(offsets && diag && radius > 0 && radius <= 10, "bad arguments");
+  return stencil_geometry(radius, width, offsets, diag);
+}
+int dmsc_gamma_lut(float gamma, float* lut) {
+  DMSC_ARG(lut, "null output");
+  gamma_lut(gamma, lut);
+  return DMSC_OK;
+}
+
+// ----------------------------------------------------------------- colour
+int dmsc_rgb_to_grey(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, float* grey) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && rgb && grey, "bad arguments");
+  return launch_rgb_to_grey(ctx->stream, w * h, rgb, grey);
+}
+int dmsc_rgb_to_lab(dmsc_ctx* ctx, int w, int h, float gamma, const uint32_t* rgb, float* L, float* a, float* b) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && rgb && L && a && b, "bad arguments");
+  return launch_rgb_to_lab(ctx->stream, w * h, gamma, rgb, L, a, b);
+}
+int dmsc_normalize_lab(dmsc_ctx* ctx, int w, int h, float* L, float* a, float* b) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && L && a && b, "bad arguments");
+  return launch_normalize_lab(ctx->stream, w * h, L, a, b);
+}
+int dmsc_mirror_image_f(dmsc_ctx* ctx, int w, int h, int border, const float* in, float* out) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && border >= 0 && border <= w && border <= h && in && out, "bad arguments");
+  return launch_mirror_f(ctx->stream, w, h, border, in, out);
+}
+int dmsc_mirror_image_i(dmsc_ctx* ctx, int w, int h, int border, const int* in, int* out) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && border >= 0 && border <= w && border <= h && in && out, "bad arguments");
+  return launch_mirror_i(ctx->stream, w, h, border, in, out);
+}
+int dmsc_unmirror_image_f(dmsc_ctx* ctx, int bw, int bh, int border, const float* in, float* out) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(bw > 2 * border && bh > 2 * border && border >= 0 && in && out, "bad arguments");
+  return launch_unmirror_f(ctx->stream, bw, bh, border, in, out);
+}
+int dmsc_quantize_image(dmsc_ctx* ctx, int w, int h, int nbins, const float* in, int* out) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && nbins > 0 && in && out, "bad arguments");
+  return launch_quantize(ctx->stream, w * h, nbins, in, out);
+}
+
+// ----------------------------------------------------------------- textons
+int dmsc_texton_responses(dmsc_ctx* ctx, int w, int h, const float* grey, float* resp) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && grey && resp, "bad arguments");
+  const float* c;
+  DMSC_TRY(ctx->texton_coeffs(&c));
+  return launch_texton_conv(ctx->stream, w, h, grey, c, resp);
+}
+
+int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* resp, int* labels, int max_iter,
+                int conv_thresh, int precision, int* changes, int* iterations) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && resp && labels, "bad arguments");
+  DMSC_ARG(k == kTextons && nfilters == kNFilters, "kmeans supports k=32 over 34 filters");
+  DMSC_ARG(max_iter >= 1 && max_iter <= 1000, "max_iter out of range");
+  DMSC_ARG(conv_thresh == 0, "only conv_thresh=0 (the reference default) is supported");
+  DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
+  const int n = w * h;
+  void* scr;
+  const size_t st = (kmeans_state_bytes() + 255) & ~(size_t)255;
+  const size_t mixed_bytes = precision == DMSC_PREC_MIXED ? kmeans_mixed_work_bytes(n) : 0;
+  DMSC_TRY(ctx->scratch(2, st + sizeof(float) * (size_t)pitch_floats(n) + mixed_bytes, &scr));
+  float* pdot = (float*)((char*)scr + st);
+  if (precision == DMSC_PREC_MIXED) {
+    DMSC_TRY(launch_kmeans_mixed(ctx->stream, w, h, resp, labels, pdot, scr, max_iter,
+                                 (char*)scr + st + sizeof(float) * (size_t)pitch_floats(n)));
+  } else {
+    DMSC_TRY(launch_kmeans(ctx->stream, w, h, k, nfilters, resp, labels, pdot, scr, max_iter));
+  }
+  return kmeans_read_state(ctx->stream, scr, changes, iterations);
+}
+
+int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* grey, int* textons, int texton_choice,
+                      int precision, int* iterations) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(grey && textons, "bad arguments");
+  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  const int n = w * h;
+  void* r;
+  DMSC_TRY(ctx->scratch(0, sizeof(float) * (size_t)kNFilters * n, &r));
+  DMSC_TRY(dmsc_texton_responses(ctx, w, h, grey, (float*)r));
+  int changes;
+  return dmsc_kmeans(ctx, w, h, kTextons, kNFilters, (float*)r, textons, 15, 0, precision, &changes, iterations);
+}
+
+// ----------------------------------------------------------------- cues
+int dmsc_local_cues(dmsc_ctx* ctx, int w, int h, const float* L, const float* a, const float* b, const int* tex,
+                    float* bg, float* cga, float* cgb, float* tg, int cue_pitch, int texton_choice) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(L && a && b && tex && bg && cga && cgb && tg, "null pointer");
+  DMSC_ARG(cue_pitch >= w * h, "cue pitch smaller than the image");
+  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  return launch_local_cues(ctx, w, h, L, a, b, tex, bg, cga, cgb, tg, cue_pitch, nullptr);
+}
+
+int dmsc_local_cues_raw(dmsc_ctx* ctx, int w, int h, const float* L, const float* a, const float* b,
+                        const int* tex, float* bg, float* cga, float* cgb, float* tg, int cue_pitch,
+                        float* raw) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(L && a && b && tex && bg && cga && cgb && tg && raw, "null pointer");
+  DMSC_ARG(cue_pitch >= w * h, "cue pitch smaller than the image");
+  return launch_local_cues(ctx, w, h, L, a, b, tex, bg, cga, cgb, tg, cue_pitch, raw);
+}
+
+// ----------------------------------------------------------------- combine / nonmax / gPb
+int dmsc_combine(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* bg, const float* cga, const float* cgb,
+                 const float* tg, float* mpb, float* cg, int texton_choice) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && cue_pitch >= w * h && bg && cga && cgb && tg && mpb && cg, "bad arguments");
+  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  return launch_combine(ctx->stream, w * h, cue_pitch, bg, cga, cgb, tg, mpb, cg);
+}
+
+int dmsc_nonmax(dmsc_ctx* ctx, int w, int h, const float* pb, int pitch, float* nmax) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && pitch >= w * h && pb && nmax, "bad arguments");
+  const int n = w * h;
+  void* scr;
+  DMSC_TRY(ctx->scratch(2, sizeof(float) * (size_t)n * 2 + 512, &scr));
+  float* mx = (float*)scr;
+  int* ori = (int*)((char*)scr + (((size_t)n * sizeof(float) + 255) & ~(size_t)255));
+  return launch_nonmax(ctx->stream, w, h, pb, pitch, mx, ori, nmax);
+}
+
+int dmsc_calc_gpb(dmsc_ctx* ctx, int npix, int pitch, int norient, const float* cg, const float* spb,
+                  const float* mpb, float* all, float* gpb) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(npix > 0 && pitch >= npix && norient > 0 && norient <= 8 && cg && spb && mpb && all && gpb,
+           "bad arguments");
+  return launch_calc_gpb(ctx->stream, npix, pitch, norient, cg, spb, mpb, all, gpb);
+}
+
+// ----------------------------------------------------------------- affinity / eigen
+int dmsc_intervene(dmsc_ctx* ctx, int w, int h, const float* mpb, float* M, int pitch, float sigma) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 5 && h > 5 && pitch >= w * h && mpb && M && sigma > 0, "bad arguments");
+  return launch_intervene(ctx->stream, w, h, mpb, M, pitch, sigma);
+}
+
+int dmsc_convert_matrix(dmsc_ctx* ctx, int w, int h, float* M, int pitch, float* dsum, float* rs) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 5 && h > 5 && pitch >= w * h && M && rs, "bad arguments");
+  return launch_convert_matrix(ctx->stream, w, h, M, pitch, dsum, rs);
+}
+
+int dmsc_stencil_spmv(dmsc_ctx* ctx, int w, int h, const float* M, int pitch, const float* x, float* y) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 5 && h > 5 && pitch >= w * h && M && x && y, "bad arguments");
+  DMSC_ARG(((uintptr_t)M % 16) == 0 && ((uintptr_t)y % 16) == 0, "matrix and y must be 16-byte aligned");
+  return launch_spmv(ctx->stream, w, h, M, pitch, x, y);
+}
+
+int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* M, int pitch, int nvec, float tol,
+                                int precision, float* eigvals, float* eigvecs, int* iterations) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 5 && h > 5 && pitch >= w * h && M && eigvals && eigvecs, "bad arguments");
+  DMSC_ARG(nvec >= 2 && nvec <= 25, "nvec must be in [2, 25] (damascene.cu:130-141)");
+  DMSC_ARG(tol >= 1e-5f && tol <= 1e-1f, "tol must be in [1e-5, 1e-1] (damascene.cu:142-151)");
+  DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
+  DMSC_ARG((pitch % 4) == 0 && ((uintptr_t)M % 16) == 0, "matrix must be 16-byte aligned with pitch % 4 == 0");
+  return run_eigensolve(ctx, w, h, M, pitch, nvec, tol, precision, eigvals, eigvecs, iterations);
+}
+
+int dmsc_spectral_pb(dmsc_ctx* ctx, const float* eigvals, const float* eigvecs, int w, int h, int nvec, float* spb,
+                     int pitch) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 3 && h > 3 && pitch >= w * h && eigvals && eigvecs && spb && nvec >= 2 && nvec <= 32,
+           "bad arguments");
+  return launch_spectral_pb(ctx->stream, eigvals, eigvecs, w, h, nvec, spb, pitch);
+}
+
+// ----------------------------------------------------------------- pipeline
+void dmsc_gpb_default_params(dmsc_gpb_params* p) {
+  if (!p) return;
+  p->n_eig = 9;
+  p->tol = 1e-3f;
+  p->texton_choice = DMSC_TEXTON32;
+  p->precision = DMSC_PREC_FP32;
+}
+
+int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_params* prm, dmsc_gpb_outputs* out,
+             int timing) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(rgb && out, "null pointer");
+  dmsc_gpb_params p;
+  if (prm) p = *prm; else dmsc_gpb_default_params(&p);
+  DMSC_ARG(p.n_eig >= 2 && p.n_eig <= 25, "n_eig must be in [2, 25]");
+  DMSC_ARG(p.texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  hipStream_t s = ctx->stream;
+  const int n = w * h;
+  const int pitch = out->pitch ? out->pitch : pitch_floats(n);
+  DMSC_ARG(pitch >= n && pitch % 4 == 0, "bad pitch");
+  const size_t P = (size_t)pitch;
+  // arena 0 layout (floats unless noted)
+  size_t off = 0;
+  auto carve = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+  const size_t o_grey = carve(4 * P), o_L = carve(4 * P), o_A = carve(4 * P), o_B = carve(4 * P);
+  const size_t o_resp = carve(4 * (size_t)kNFilters * n);
+  const size_t o_tex = carve(4 * P);
+  const size_t o_cues = carve(4 * 4 * 24 * P);
+  const size_t o_mpb = carve(4 * 8 * P), o_cg = carve(4 * 8 * P), o_nmax = carve(4 * P);
+  const size_t o_mat = carve(4 * (size_t)kStencilArea * P);
+  const size_t o_vec = carve(4 * (size_t)p.n_eig * n);
+  const size_t o_spb = carve(4 * 8 * P), o_all = carve(4 * 8 * P), o_gpb = carve(4 * P);
+  void* base;
+  DMSC_TRY(ctx->scratch(0, off, &base));
+  char* B0 = (char*)base;
+  float* grey = (float*)(B0 + o_grey);
+  float *L = (float*)(B0 + o_L), *A = (float*)(B0 + o_A), *Bc = (float*)(B0 + o_B);
+  float* resp = (float*)(B0 + o_resp);
+  int* tex = out->d_textons ? out->d_textons : (int*)(B0 + o_tex);
+  float* cues = (float*)(B0 + o_cues);
+  float* mpb = out->d_mpb ? out->d_mpb : (float*)(B0 + o_mpb);
+  float* cg = out->d_combined_g ? out->d_combined_g : (float*)(B0 + o_cg);
+  float* nmax = out->d_nmax ? out->d_nmax : (float*)(B0 + o_nmax);
+  float* mat = (float*)(B0 + o_mat);
+  float* vec = out->d_eigvecs ? out->d_eigvecs : (float*)(B0 + o_vec);
+  float* spb = out->d_spb ? out->d_spb : (float*)(B0 + o_spb);
+  float* all = out->d_gpb_all ? out->d_gpb_all : (float*)(B0 + o_all);
+  float* gpb = out->d_gpb ? out->d_gpb : (float*)(B0 + o_gpb);
+
+  hipEvent_t ev[7] = {};
+  if (timing) {
+    for (int i = 0; i < 7; i++) DMSC_HIP(hipEventCreate(&ev[i]));
+    DMSC_HIP(hipEventRecord(ev[0], s));
+  }
+  // damascene.cu:269-285 grey + textons; 309-317 Lab (gamma 2.5), normalised
+  DMSC_TRY(launch_colour_fused(s, n, 2.5f, rgb, grey, L, A, Bc));
+  const float* coeffs;
+  DMSC_TRY(ctx->texton_coeffs(&coeffs));
+  DMSC_TRY(launch_texton_conv(s, w, h, grey, coeffs, resp));
+  {
+    void* scr;
+    const size_t st = (kmeans_state_bytes() + 255) & ~(size_t)255;
+    const size_t mixed_bytes = p.precision == DMSC_PREC_MIXED ? kmeans_mixed_work_bytes(n) : 0;
+    DMSC_TRY(ctx->scratch(2, st + sizeof(float) * P + mixed_bytes, &scr));
+    float* pdot = (float*)((char*)scr + st);
+    if (p.precision == DMSC_PREC_MIXED)
+      DMSC_TRY(launch_kmeans_mixed(s, w, h, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
+    else
+      DMSC_TRY(launch_kmeans(s, w, h, kTextons, kNFilters, resp, tex, pdot, scr, 15));
+    int ch = 0;
+    DMSC_TRY(kmeans_read_state(s, scr, &ch, &out->kmeans_iterations));
+  }
+  if (timing) DMSC_HIP(hipEventRecord(ev[1], s));
+  // damascene.cu:352 localCues
+  DMSC_TRY(launch_local_cues(ctx, w, h, L, A, Bc, tex, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, pitch,
+                             nullptr));
+  if (timing) DMSC_HIP(hipEventRecord(ev[2], s));
+  // damascene.cu:388-404 combine + nonMaxSuppression
+  DMSC_TRY(launch_combine(s, n, pitch, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, mpb, cg));
+  DMSC_TRY(launch_nonmax(s, w, h, mpb, pitch, spb, (int*)all, nmax));  // spb/all as scratch here
+  if (timing) DMSC_HIP(hipEventRecord(ev[3], s));
+  // damascene.cu:417-420 intervene
+  DMSC_TRY(launch_intervene(s, w, h, nmax, mat, pitch, 0.1f));
+  if (timing) DMSC_HIP(hipEventRecord(ev[4], s));
+  // damascene.cu:433 generalizedEigensolve
+  DMSC_TRY(run_eigensolve(ctx, w, h, mat, pitch, p.n_eig, p.tol, p.precision, out->h_eigvals, vec,
+                          &out->lanczos_iterations));
+  if (timing) DMSC_HIP(hipEventRecord(ev[5], s));
+  // damascene.cu:441-459 spectralPb + StartCalcGPb
+  DMSC_TRY(launch_spectral_pb(s, out->h_eigvals, vec, w, h, p.n_eig, spb, pitch));
+  DMSC_TRY(launch_calc_gpb(s, n, pitch, 8, cg, spb, nmax, all, gpb));
+  if (timing) {
+    DMSC_HIP(hipEventRecord(ev[6], s));
+    DMSC_HIP(hipEventSynchronize(ev[6]));
+    for (int i = 0; i < 6; i++) DMSC_HIP(hipEventElapsedTime(&out->stage_ms[i], ev[i], ev[i + 1]));
+    out->stage_ms[6] = out->stage_ms[7] = 0.f;
+    for (int i = 0; i < 7; i++) (void)hipEventDestroy(ev[i]);
+  }
+  return DMSC_OK;
+}
+
+}  // extern "C"

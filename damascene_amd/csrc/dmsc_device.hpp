@@ -1,0 +1,54 @@
+// dmsc_device.hpp -- device helpers shared by the gPb kernels.
+//
+// Numerical contract (identical to the CPU oracle): the library is built with
+// -ffp-contract=off; every fused multiply-add below is an explicit fmaf();
+// divisions and square roots are IEEE correctly rounded (HIP defaults).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dmsc {
+namespace dev {
+
+__host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Symmetric reflection of index i of a border-padded axis of n samples
+// (convert.cu:128-141).
+__host__ __device__ inline int mirror(int i, int border, int n) {
+  return i < border ? border - i - 1 : (i < border + n ? i - border : border + 2 * n - i - 1);
+}
+
+// Cube root from basic IEEE double operations only (exponent split, linear
+// initial guess on [1,8), six Newton steps, rounded to float).  Produces the
+// same bits on the GPU and on any IEEE host built without contraction.
+__device__ inline float cbrt_exact(float xf) {
+  if (xf == 0.0f) return xf;
+  const double x = (double)xf;
+  const double ax = x < 0 ? -x : x;
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(ax);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);
+  const int rem = e - 3 * q;
+  double m = __longlong_as_double((long long)((bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+  for (int i = 0; i < rem; i++) m = m * 2.0;
+  double y = 1.0 + (m - 1.0) * (1.0 / 7.0);
+#pragma unroll
+  for (int it = 0; it < 6; it++) y = y - (y * y * y - m) / (3.0 * y * y);
+  const double scale = __longlong_as_double((long long)((unsigned long long)(q + 1023) << 52));
+  const double r = y * scale;
+  return (float)(x < 0 ? -r : r);
+}
+
+// In-register 32-slot tree sum in the order of the reference's warp
+// reduction (rotate.cu:378-389): (i, i+16), (i, i+8), ..., (0, 1).
+template <typename T>
+__device__ __forceinline__ float tree32(T* t) {
+#pragma unroll
+  for (int s = 16; s >= 1; s >>= 1)
+#pragma unroll
+    for (int i = 0; i < s; i++) t[i] = t[i] + t[i + s];
+  return t[0];
+}
+
+}  // namespace dev
+}  // namespace dmsc

@@ -1,0 +1,43 @@
+// dmsc_internal.hpp -- shared declarations of the MI355X gPb library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dmsc.h"
+#include "tables.hpp"
+
+namespace dmsc {
+
+#define DMSC_HIP(call)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::dmsc::fail(DMSC_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                          hipGetErrorString(e_));                                 \
+  } while (0)
+
+// ---------------------------------------------------------------- context
+constexpr int kArenas = 6;
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // grow-only device scratch arenas (reused across calls on this context):
+  // 0 pipeline planes, 1 local-cue frames, 2 small solver state, 3 Krylov
+  // basis, 4-5 batch/backward work
+  void* arena[kArenas] = {};
+  size_t arena_bytes[kArenas] = {};
+  float* d_texton_coeffs = nullptr;
+  ~Ctx();
+  int scratch(int slot, size_t bytes, void** ptr);
+  int adopt(int slot, void* p, size_t bytes);  // replace arena `slot` by p
+  int texton_coeffs(const float** out);
+};
+
+}  // namespace dmsc
+
+struct dmsc_ctx : public dmsc::Ctx {};

@@ -1,0 +1,484 @@
+// localcues.hip -- brightness / colour / texture gradients (reference:
+// localcues/localcues.cu, gradient.cu, rotate.cu, parabola.cu).
+//
+// All 4 channels x 4 rotations (16 frames) are processed together so every
+// launch has enough waves to fill 256 CUs:
+//   1. labels:    quantised L/a/b (25 bins) or textons (32), mirrored (border
+//                 30) and scattered through the Bresenham rotation tables into
+//                 uint8 rotated frames (255 = outside the rotated image);
+//   2. integrals: per-bin 2-D inclusive counts, row pass then column pass,
+//                 stored as uint16 modulo 2^16: every half-disc count is a
+//                 signed sum of four corners whose true value lies in
+//                 [0, 2r^2], so modular arithmetic recovers it exactly at half
+//                 the bytes of int32;
+//   3. gradients: per pixel, scale and split: 4+4 corner gathers per bin,
+//                 optional bin blur, normalisation to sum 8 and chi-squared
+//                 with the reference's 32-slot tree sums;
+//   4. parabola:  Savitzky-Golay smoothing of each mirrored gradient plane
+//                 over the filter's non-zero taps (v outer, u inner, fma).
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "dmsc_device.hpp"
+#include "dmsc_internal.hpp"
+#include "kernels.hpp"
+
+namespace dmsc {
+
+namespace {
+
+constexpr int kFrames = 16;
+constexpr int kRadii[4] = {3, 5, 10, 20};
+
+struct FrameGeom {
+  int w, h, bw, bh;
+  int nw[4], nh[4];
+  long long lab_off[kFrames];  // byte offset of frame f = ch*4 + o (label buffer)
+  long long int_off[kFrames];  // uint16 element offset of frame f (integral buffer)
+  int tab_off[4];              // offset of orientation o's tables: bX[bw] bY[bw] gX[bh] gY[bh]
+  int rect[4][4][8];           // [orientation][radius index][corner A..H]
+};
+
+struct CueOut {
+  float* p[4];  // Bg, Cga, Cgb, Tg; each [24][pitch]
+};
+
+__global__ void k_fill_u8(uint8_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0xff;
+}
+
+// gradient.cu:210-223 (quantizeImage, mirrorImage) + rotate.cu:247-259
+// (bresenhamRotate): one mirrored pixel per thread, all 16 frames.
+__global__ void k_cue_labels(FrameGeom g, const float* __restrict__ L, const float* __restrict__ A,
+                             const float* __restrict__ B, const int* __restrict__ tex,
+                             const int* __restrict__ tabs, uint8_t* __restrict__ frames) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y;
+  const int f = blockIdx.z, ch = f >> 2, o = f & 3;
+  if (X >= g.bw) return;
+  const size_t si = (size_t)dev::mirror(Y, kBorder, g.h) * g.w + dev::mirror(X, kBorder, g.w);
+  int lab;
+  if (ch == 3) {
+    lab = tex[si];
+  } else {
+    const float v = (ch == 0 ? L : ch == 1 ? A : B)[si];
+    lab = (int)floorf(v * 25.0f);
+    if (lab == 25) lab = 24;
+  }
+  const int* t = tabs + g.tab_off[o];
+  const int rx = t[2 * g.bw + Y] + t[X];
+  const int ry = t[2 * g.bw + g.bh + Y] + t[g.bw + X];
+  frames[g.lab_off[f] + (size_t)ry * g.nw[o] + rx] = (uint8_t)lab;
+}
+
+// gradient.cu:66-87: per-bin inclusive prefix counts along each row of a
+// rotated frame.  Block = one row; thread = (bin, one of 8 row segments).
+__global__ __launch_bounds__(256) void k_integral_rows(FrameGeom g, const uint8_t* __restrict__ frames,
+                                                       uint16_t* __restrict__ I) {
+  extern __shared__ uint8_t row[];
+  __shared__ uint32_t segtot[8][32];
+  const int f = blockIdx.y, o = f & 3, y = blockIdx.x;
+  const int nw = g.nw[o];
+  if (y >= g.nh[o]) return;
+  const uint8_t* src = frames + g.lab_off[f] + (size_t)y * nw;
+  for (int i = threadIdx.x; i < nw; i += 256) row[i] = src[i];
+  __syncthreads();
+  const int bin = threadIdx.x & 31, seg = threadIdx.x >> 5;
+  const int len = (nw + 7) / 8, x0 = seg * len, x1 = min(nw, x0 + len);
+  uint32_t c = 0;
+  for (int x = x0; x < x1; x++) c += (row[x] == bin);
+  segtot[seg][bin] = c;
+  __syncthreads();
+  uint32_t run = 0;
+  for (int s = 0; s < seg; s++) run += segtot[s][bin];
+  uint16_t* dst = I + g.int_off[f] + (size_t)y * nw * 32;
+  for (int x = x0; x < x1; x++) {
+    run += (row[x] == bin);
+    dst[(size_t)x * 32 + bin] = (uint16_t)run;
+  }
+}
+
+// gradient.cu:89-103: column pass, two bins per thread, modulo 2^16.
+__global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, uint16_t* __restrict__ I) {
+  const int f = blockIdx.y, o = f & 3;
+  const int nw = g.nw[o], nh = g.nh[o];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (x, bin pair)
+  if (t >= nw * 16) return;
+  uint32_t* col = reinterpret_cast<uint32_t*>(I + g.int_off[f]) + t;
+  const size_t stride = (size_t)nw * 16;
+  uint32_t lo = 0, hi = 0;
+  int y = 0;
+  for (; y + 8 <= nh; y += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = col[(size_t)(y + k) * stride];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      lo += v[k] & 0xffffu;
+      hi += v[k] >> 16;
+      col[(size_t)(y + k) * stride] = (lo & 0xffffu) | (hi << 16);
+    }
+  }
+  for (; y < nh; y++) {
+    const uint32_t v = col[(size_t)y * stride];
+    lo += v & 0xffffu;
+    hi += v >> 16;
+    col[(size_t)y * stride] = (lo & 0xffffu) | (hi << 16);
+  }
+}
+
+struct BlurK {
+  float k[16];
+};
+
+// rotate.cu:263-475 computeGradient: one (pixel, scale, split) per thread.
+// NB bins, KR blur radius (0 = no blur).  Output plane of channel ch:
+// [scale][slot] with slot o+4 for the top/bottom split (sense A) and slot o
+// for the left/right split (sense B) (gradient.cu:240-241).
+template <int NB, int KR>
+__global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad0, const int* __restrict__ tabs,
+                                                   const uint16_t* __restrict__ I, BlurK bk,
+                                                   float* __restrict__ grads, size_t plane) {
+  const int n = g.w * g.h;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int scale = blockIdx.y >> 1, sense = blockIdx.y & 1;
+  const int ch = ch0 + (blockIdx.z >> 2), o = blockIdx.z & 3, f = ch * 4 + o;
+  if (p >= n) return;
+  const int y = p / g.w, x = p - y * g.w;
+  const int* t = tabs + g.tab_off[o];
+  const int ix = x + kBorder, iy = y + kBorder;
+  const int rx = t[2 * g.bw + iy] + t[ix];
+  const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
+  const long long pix = (long long)ry * g.nw[o] + rx;
+  const int* rc = g.rect[o][rad0 + scale];
+  // sense A: a = A-C+E-D, b = H-F+D-E ; sense B: a = A-F+G-B, b = H-C+B-G
+  const int k1 = sense ? 5 : 2, k2 = sense ? 6 : 4, k3 = sense ? 1 : 3;
+  const int m1 = sense ? 2 : 5, m2 = sense ? 1 : 3, m3 = sense ? 6 : 4;
+  const uint16_t* base = I + g.int_off[f];
+  const uint4* cA = reinterpret_cast<const uint4*>(base + (size_t)(rc[0] + pix) * 32);
+  const uint4* cB = reinterpret_cast<const uint4*>(base + (size_t)(rc[k1] + pix) * 32);
+  const uint4* cC = reinterpret_cast<const uint4*>(base + (size_t)(rc[k2] + pix) * 32);
+  const uint4* cD = reinterpret_cast<const uint4*>(base + (size_t)(rc[k3] + pix) * 32);
+  const uint4* dA = reinterpret_cast<const uint4*>(base + (size_t)(rc[7] + pix) * 32);
+  const uint4* dB = reinterpret_cast<const uint4*>(base + (size_t)(rc[m1] + pix) * 32);
+  const uint4* dC = reinterpret_cast<const uint4*>(base + (size_t)(rc[m2] + pix) * 32);
+  const uint4* dD = reinterpret_cast<const uint4*>(base + (size_t)(rc[m3] + pix) * 32);
+  float a[32], b[32];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {  // 8 bins per 16-byte load
+    if (q * 8 >= NB) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) { a[q * 8 + j] = 0.f; b[q * 8 + j] = 0.f; }
+      continue;
+    }
+    const uint4 va = cA[q], vb = cB[q], vc = cC[q], vd = cD[q];
+    const uint4 wa = dA[q], wb = dB[q], wc = dC[q], wd = dD[q];
+    const uint32_t pa[4] = {va.x, va.y, va.z, va.w}, pb[4] = {vb.x, vb.y, vb.z, vb.w};
+    const uint32_t pc[4] = {vc.x, vc.y, vc.z, vc.w}, pd[4] = {vd.x, vd.y, vd.z, vd.w};
+    const uint32_t qa[4] = {wa.x, wa.y, wa.z, wa.w}, qb[4] = {wb.x, wb.y, wb.z, wb.w};
+    const uint32_t qc[4] = {wc.x, wc.y, wc.z, wc.w}, qd[4] = {wd.x, wd.y, wd.z, wd.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int sh = (j & 1) * 16, e = j >> 1;
+      const uint32_t s0 = (pa[e] >> sh) - (pb[e] >> sh) + (pc[e] >> sh) - (pd[e] >> sh);
+      const uint32_t s1 = (qa[e] >> sh) - (qb[e] >> sh) + (qc[e] >> sh) - (qd[e] >> sh);
+      const bool live = q * 8 + j < NB;
+      a[q * 8 + j] = live ? (float)(s0 & 0xffffu) : 0.f;
+      b[q * 8 + j] = live ? (float)(s1 & 0xffffu) : 0.f;
+    }
+  }
+  if (KR > 0) {  // rotate.cu:352-370: zero-padded correlation over bins
+    float ca[NB], cb[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      float sa = 0.f, sb = 0.f;
+#pragma unroll
+      for (int nk = 0; nk < 2 * KR + 1; nk++) {
+        const int j = i - KR + nk;
+        if (j >= 0 && j < NB) {
+          sa = fmaf(a[j], bk.k[nk], sa);
+          sb = fmaf(b[j], bk.k[nk], sb);
+        } else {
+          sa = fmaf(0.0f, bk.k[nk], sa);
+          sb = fmaf(0.0f, bk.k[nk], sb);
+        }
+      }
+      ca[i] = sa;
+      cb[i] = sb;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) { a[i] = ca[i]; b[i] = cb[i]; }
+  }
+  float t32[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) t32[i] = a[i];
+  const float sa = dev::tree32(t32);
+  const float na = (sa != 0.f) ? 8.0f / sa : 0.f;
+#pragma unroll
+  for (int i = 0; i < 32; i++) t32[i] = b[i];
+  const float sb = dev::tree32(t32);
+  const float nb = (sb != 0.f) ? 8.0f / sb : 0.f;
+#pragma unroll
+  for (int i = 0; i < 32; i++) {
+    const float ai = a[i] * na, bi = b[i] * nb;
+    const float s = ai + bi;
+    float chi = 0.f;
+    if (s != 0.f) {
+      const float d = ai - bi;
+      chi = d * d / s;
+    }
+    t32[i] = chi;
+  }
+  const float out = 0.5f * dev::tree32(t32);
+  const int slot = sense ? o : o + 4;
+  grads[(size_t)((ch * 3 + scale) * 8 + slot) * plane + p] = out;
+}
+
+// Savitzky-Golay taps of one (radius, orientation) in the reference's
+// order (v outer, u inner), zero taps dropped.
+struct Tap {
+  short du, dv;
+  float c;
+};
+
+constexpr int PTX = 64, PTY = 16;
+
+// parabola.cu:75-124 / parabola_kernel.cu:1-30 for one scale of a channel
+// group: plane z = (channel - ch0) * 8 + orientation.
+__global__ __launch_bounds__(256) void k_parabola(int w, int h, int r, int scale, int ch0,
+                                                  const float* __restrict__ grads, size_t gplane,
+                                                  const Tap* __restrict__ taps, int tap_stride,
+                                                  const int* __restrict__ ntaps, CueOut out, int pitch) {
+  extern __shared__ float tile[];
+  const int LW = PTX + 2 * r, LH = PTY + 2 * r;
+  const int ch = ch0 + blockIdx.z / 8, o = blockIdx.z % 8;
+  const float* src = grads + (size_t)((ch * 3 + scale) * 8 + o) * gplane;
+  const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * PTY;
+  for (int i = threadIdx.x; i < LW * LH; i += 256) {
+    const int ly = i / LW, lx = i - ly * LW;
+    // tile columns past the image only feed discarded outputs: clamp into
+    // the mirrored range [0, w + 2*border) before reflecting
+    const int gx = dev::mirror(dev::clampi(x0 + lx - r + kBorder, 0, w + 2 * kBorder - 1), kBorder, w);
+    const int gy = dev::mirror(dev::clampi(y0 + ly - r + kBorder, 0, h + 2 * kBorder - 1), kBorder, h);
+    tile[i] = src[(size_t)gy * w + gx];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63;
+  const Tap* tp = taps + (size_t)o * tap_stride;
+  const int nt = ntaps[o];
+  float* dst = out.p[ch] + (size_t)(scale * 8 + o) * pitch;
+  for (int ty = threadIdx.x >> 6; ty < PTY; ty += 4) {
+    float acc = 0.0f;
+    const float* ctr = tile + (size_t)(ty + r) * LW + tx + r;
+    // nt is a multiple of 8 (lists are padded with zero taps, which leave the
+    // sum unchanged), so the loop has no remainder iterations.
+    for (int k = 0; k < nt; k += 8) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const Tap tk = tp[k + j];
+        acc = fmaf(ctr[tk.dv * LW + tk.du], tk.c, acc);
+      }
+    }
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < w && gy < h) dst[(size_t)gy * w + gx] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- host plan
+
+struct CuePlan {
+  int w = -1, h = -1;
+  FrameGeom g;
+  size_t lab_bytes = 0, int_elems = 0;
+  int tap_stride = 0;
+  int* d_tabs = nullptr;
+  Tap* d_taps[4] = {nullptr, nullptr, nullptr, nullptr};
+  int* d_ntaps[4] = {nullptr, nullptr, nullptr, nullptr};
+  BlurK blur_bg, blur_cg;
+  int kr_bg = 0, kr_cg = 0;
+  ~CuePlan() {
+    if (d_tabs) (void)hipFree(d_tabs);
+    for (int i = 0; i < 4; i++) {
+      if (d_taps[i]) (void)hipFree(d_taps[i]);
+      if (d_ntaps[i]) (void)hipFree(d_ntaps[i]);
+    }
+  }
+};
+
+int build_plan(hipStream_t s, int w, int h, CuePlan& P) {
+  FrameGeom& g = P.g;
+  memset(&g, 0, sizeof(g));
+  g.w = w; g.h = h; g.bw = w + 2 * kBorder; g.bh = h + 2 * kBorder;
+  std::vector<int> tabs;
+  std::vector<int> bX(g.bw), bY(g.bw), gX(g.bh), gY(g.bh);
+  for (int o = 0; o < 4; o++) {
+    const float thetaPi = -float(o) / float(8);
+    if (rotation_tables(g.bw, g.bh, thetaPi, &g.nw[o], &g.nh[o], bX.data(), bY.data(), gX.data(), gY.data()))
+      return fail(DMSC_ERR_INTERNAL, "rotation tables failed for %dx%d", w, h);
+    g.tab_off[o] = (int)tabs.size();
+    tabs.insert(tabs.end(), bX.begin(), bX.end());
+    tabs.insert(tabs.end(), bY.begin(), bY.end());
+    tabs.insert(tabs.end(), gX.begin(), gX.end());
+    tabs.insert(tabs.end(), gY.begin(), gY.end());
+    for (int ri = 0; ri < 4; ri++)
+      halfdisc_offsets(w, h, thetaPi, kRadii[ri], bX.data(), bY.data(), gX.data(), gY.data(), g.nw[o],
+                       g.rect[o][ri]);
+  }
+  // Every gather below must stay inside its rotated frame: verify on the host
+  // (labels scatter over the whole mirrored image; corners around every
+  // interior pixel for all radii).
+  for (int o = 0; o < 4; o++) {
+    const int* t = tabs.data() + g.tab_off[o];
+    const int *tbX = t, *tbY = t + g.bw, *tgX = t + 2 * g.bw, *tgY = t + 2 * g.bw + g.bh;
+    for (int Y = 0; Y < g.bh; Y++)
+      for (int X = 0; X < g.bw; X++) {
+        const int rx = tgX[Y] + tbX[X], ry = tgY[Y] + tbY[X];
+        if (rx < 0 || ry < 0 || rx >= g.nw[o] || ry >= g.nh[o])
+          return fail(DMSC_ERR_INTERNAL, "rotation maps outside the frame (o=%d)", o);
+      }
+    long long lo_off = 0, hi_off = 0;
+    for (int ri = 0; ri < 4; ri++)
+      for (int k = 0; k < 8; k++) {
+        lo_off = std::min(lo_off, (long long)g.rect[o][ri][k]);
+        hi_off = std::max(hi_off, (long long)g.rect[o][ri][k]);
+      }
+    const long long npx = (long long)g.nw[o] * g.nh[o];
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) {
+        const int X = x + kBorder, Y = y + kBorder;
+        const long long pix = (long long)(tgY[Y] + tbY[X]) * g.nw[o] + (tgX[Y] + tbX[X]);
+        if (pix + lo_off < 0 || pix + hi_off >= npx)
+          return fail(DMSC_ERR_INTERNAL, "half-disc corners leave the frame (o=%d)", o);
+      }
+  }
+  size_t lo = 0, io = 0;
+  for (int f = 0; f < kFrames; f++) {
+    const int o = f & 3;
+    g.lab_off[f] = (long long)lo;
+    g.int_off[f] = (long long)io;
+    lo += ((size_t)g.nw[o] * g.nh[o] + 255) & ~(size_t)255;
+    io += (size_t)g.nw[o] * g.nh[o] * 32;
+  }
+  P.lab_bytes = lo;
+  P.int_elems = io;
+  DMSC_HIP(hipMalloc(&P.d_tabs, sizeof(int) * tabs.size()));
+  DMSC_HIP(hipMemcpy(P.d_tabs, tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice));
+  for (int ri = 0; ri < 4; ri++) {
+    const int r = kRadii[ri], len = 2 * r + 1;
+    std::vector<float> filt((size_t)8 * len * len);
+    parabola_filters(r, filt.data());
+    const size_t stride = (size_t)len * len + 8;
+    std::vector<Tap> all((size_t)8 * stride, Tap{0, 0, 0.f});
+    std::vector<int> cnt(8);
+    for (int o = 0; o < 8; o++) {
+      int k = 0;
+      for (int v = -r; v <= r; v++)
+        for (int u = -r; u <= r; u++) {
+          const float c = filt[(size_t)o * len * len + (v + r) * len + (u + r)];
+          if (c != 0.0f) all[(size_t)o * stride + k++] = Tap{(short)u, (short)v, c};
+        }
+      // pad to a multiple of 8 with zero taps at the centre: fma(x, 0, acc) == acc
+      // for finite x because acc is never -0 (it starts at +0 and an exact
+      // cancellation rounds to +0)
+      while (k % 8) all[(size_t)o * stride + k++] = Tap{0, 0, 0.f};
+      cnt[o] = k;
+    }
+    DMSC_HIP(hipMalloc(&P.d_taps[ri], sizeof(Tap) * all.size()));
+    DMSC_HIP(hipMemcpy(P.d_taps[ri], all.data(), sizeof(Tap) * all.size(), hipMemcpyHostToDevice));
+    DMSC_HIP(hipMalloc(&P.d_ntaps[ri], sizeof(int) * 8));
+    DMSC_HIP(hipMemcpy(P.d_ntaps[ri], cnt.data(), sizeof(int) * 8, hipMemcpyHostToDevice));
+  }
+  // gradient.cu:233: blur sigma = (int)(sigma * nbins) -> Bg 2, Cg 1
+  memset(&P.blur_bg, 0, sizeof(BlurK));
+  memset(&P.blur_cg, 0, sizeof(BlurK));
+  blur_kernel((float)(int)(0.1f * 25.0f), P.blur_bg.k, &P.kr_bg);
+  blur_kernel((float)(int)(0.05f * 25.0f), P.blur_cg.k, &P.kr_cg);
+  if (P.kr_bg != 6 || P.kr_cg != 3) return fail(DMSC_ERR_INTERNAL, "unexpected blur radii");
+  P.w = w;
+  P.h = h;
+  (void)s;
+  return DMSC_OK;
+}
+
+struct PlanCache {
+  std::map<std::pair<int, int>, std::unique_ptr<CuePlan>> plans;
+};
+
+}  // namespace
+
+// Cached per (w, h) on the context.
+static int get_plan(Ctx* ctx, int w, int h, CuePlan** out) {
+  static std::map<Ctx*, PlanCache> caches;  // keyed by context; cleared by ctx teardown
+  PlanCache& pc = caches[ctx];
+  auto key = std::make_pair(w, h);
+  auto it = pc.plans.find(key);
+  if (it == pc.plans.end()) {
+    std::unique_ptr<CuePlan> p(new CuePlan());
+    DMSC_TRY(build_plan(ctx->stream, w, h, *p));
+    it = pc.plans.emplace(key, std::move(p)).first;
+  }
+  *out = it->second.get();
+  return DMSC_OK;
+}
+
+size_t local_cues_scratch_bytes(int w, int h) {
+  const int bw = w + 2 * kBorder, bh = h + 2 * kBorder;
+  // generous bound: rotated frames are at most (bw+bh)^2
+  const size_t rot = (size_t)(bw + bh) * (bw + bh);
+  const size_t lab = 16 * (rot + 256);
+  const size_t ints = 16 * rot * 32 * sizeof(uint16_t);
+  const size_t grads = (size_t)4 * 24 * w * h * sizeof(float);
+  return lab + ints + grads + 4096;
+}
+
+// localcues.cu:257-300 localCues for all four cues.
+int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B,
+                      const int* tex, float* bg, float* cga, float* cgb, float* tg, int pitch,
+                      float* grads_out) {
+  hipStream_t s = ctx->stream;
+  CuePlan* P;
+  DMSC_TRY(get_plan(ctx, w, h, &P));
+  const FrameGeom& g = P->g;
+  const int n = w * h;
+  void* scratch;
+  const size_t gbytes = (size_t)4 * 24 * n * sizeof(float);
+  const size_t ibytes = P->int_elems * sizeof(uint16_t);
+  DMSC_TRY(ctx->scratch(1, P->lab_bytes + ibytes + gbytes + 1024, &scratch));
+  uint8_t* frames = (uint8_t*)scratch;
+  uint16_t* I = (uint16_t*)((char*)scratch + P->lab_bytes);
+  float* grads = grads_out ? grads_out : (float*)((char*)scratch + P->lab_bytes + ibytes);
+  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(P->lab_bytes, 256)), dim3(256), 0, s, frames, P->lab_bytes);
+  hipLaunchKernelGGL(k_cue_labels, dim3(div_up(g.bw, 256), g.bh, kFrames), dim3(256), 0, s, g, L, A, B,
+                     tex, P->d_tabs, frames);
+  int maxh = 0, maxw = 0;
+  for (int o = 0; o < 4; o++) { maxh = std::max(maxh, g.nh[o]); maxw = std::max(maxw, g.nw[o]); }
+  hipLaunchKernelGGL(k_integral_rows, dim3(maxh, kFrames), dim3(256), (size_t)maxw, s, g, frames, I);
+  hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 16, 256), kFrames), dim3(256), 0, s, g, I);
+  const size_t plane = (size_t)n;
+  BlurK none;
+  memset(&none, 0, sizeof(none));
+  // Bg: channel 0, radii {3,5,10}; Cga/Cgb: channels 1-2, radii {5,10,20};
+  // Tg: channel 3, radii {5,10,20}.
+  hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 6, 4), dim3(256), 0, s, g, 0, 0,
+                     P->d_tabs, I, P->blur_bg, grads, plane);
+  hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 6, 8), dim3(256), 0, s, g, 1, 1,
+                     P->d_tabs, I, P->blur_cg, grads, plane);
+  hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 6, 4), dim3(256), 0, s, g, 3, 1,
+                     P->d_tabs, I, none, grads, plane);
+  CueOut out{{bg, cga, cgb, tg}};
+  for (int scale = 0; scale < 3; scale++) {
+    for (int grp = 0; grp < 2; grp++) {
+      const int ri = scale + grp, r = kRadii[ri];
+      const int ch0 = grp == 0 ? 0 : 1, nch = grp == 0 ? 1 : 3;
+      const size_t lds = sizeof(float) * (PTX + 2 * r) * (PTY + 2 * r);
+      hipLaunchKernelGGL(k_parabola, dim3(div_up(w, PTX), div_up(h, PTY), nch * 8), dim3(256), lds, s, w,
+                         h, r, scale, ch0, grads, plane, P->d_taps[ri], (2 * r + 1) * (2 * r + 1) + 8,
+                         P->d_ntaps[ri], out, pitch);
+    }
+  }
+  DMSC_HIP(hipGetLastError());
+  return DMSC_OK;
+}
+
+}  // namespace dmsc

@@ -1,0 +1,114 @@
+// stencil.hip -- 81-diagonal stencil matrix: row sums, normalised Laplacian
+// transform and SpMV (reference: stencilMatrixMultiply/stencilMVM.cu).
+//
+// SpMV layout: A is DIA [81][pitch] (diagonal-major), so for a fixed
+// diagonal consecutive rows are consecutive floats: each lane owns 4
+// consecutive rows and reads one 16-byte vector per diagonal (a wave reads
+// 1 KiB contiguous per diagonal).  The x window a block needs (its rows plus
+// a 5-image-row halo each side) is staged in LDS with zeros outside [0, N),
+// which makes the reference's column bounds test branch-free: an
+// out-of-range column contributes fma(A, 0, acc) == acc.  Each row sums its
+// 81 products sequentially in diagonal order (fused multiply-add), exactly
+// as the CPU oracle.
+#include "dmsc_device.hpp"
+#include "dmsc_internal.hpp"
+#include "kernels.hpp"
+
+namespace dmsc {
+
+namespace {
+
+__global__ void k_rowsum(int n, StencilOffsets so, const float* __restrict__ A, size_t pitch,
+                         float* __restrict__ dsum, float* __restrict__ rs) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  float acc = 0.0f;
+  for (int d = 0; d < kStencilArea; d++) {
+    const int col = row + so.off[d];
+    if (col >= 0 && col < n) acc = acc + A[(size_t)d * pitch + row];
+  }
+  if (dsum) dsum[row] = acc;
+  rs[row] = 1.0f / sqrtf(acc);
+}
+
+// stencilMVM.cu:98-125 unGeneralizeMatrix
+__global__ void k_lsym(int n, StencilOffsets so, float* __restrict__ A, size_t pitch,
+                       const float* __restrict__ rs) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  // dii = sum of the row, recomputed in the same order as k_rowsum
+  float dii = 0.0f;
+  for (int d = 0; d < kStencilArea; d++) {
+    const int col = row + so.off[d];
+    if (col >= 0 && col < n) dii = dii + A[(size_t)d * pitch + row];
+  }
+  const float diim5 = rs[row];
+  for (int d = 0; d < kStencilArea; d++) {
+    const int col = row + so.off[d];
+    if (col < 0 || col >= n) continue;
+    float* e = &A[(size_t)d * pitch + row];
+    if (col == row) *e = (dii - *e) / dii;
+    else *e = (-*e * diim5) * rs[col];
+  }
+}
+
+}  // namespace
+
+int launch_convert_matrix(hipStream_t s, int w, int h, float* A, int pitch, float* dsum, float* rs) {
+  const int n = w * h;
+  StencilOffsets so = make_stencil_offsets(w);
+  hipLaunchKernelGGL(k_rowsum, dim3(div_up(n, 256)), dim3(256), 0, s, n, so, A, (size_t)pitch, dsum, rs);
+  hipLaunchKernelGGL(k_lsym, dim3(div_up(n, 256)), dim3(256), 0, s, n, so, A, (size_t)pitch, rs);
+  DMSC_HIP(hipGetLastError());
+  return DMSC_OK;
+}
+
+StencilOffsets make_stencil_offsets(int w) {
+  StencilOffsets so;
+  int delta[121], diag[121];
+  const int nd = stencil_geometry(kStencilRadius, w, delta, diag);
+  int acc = 0;
+  for (int d = 0; d < nd; d++) {
+    acc += delta[d];
+    so.off[d] = acc;
+  }
+  so.halo = kStencilRadius * w + kStencilRadius;
+  return so;
+}
+
+namespace {
+
+// y = A x (stencilMVM.cu:127-180).
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(int n, StencilOffsets so, const float* __restrict__ A,
+                                                       size_t pitch, const float* __restrict__ x,
+                                                       float* __restrict__ y) {
+  extern __shared__ float xs[];
+  const int r0 = blockIdx.x * kSpmvRows;
+  spmv_stage_x(xs, x, n, r0, so.halo, 1.0f);
+  __syncthreads();
+  const int row = r0 + threadIdx.x * 4;
+  if (row >= n) return;
+  float4 acc;
+  spmv_rows4(acc, A + row, pitch, xs, row - r0 + so.halo, so);
+  if (row + 3 < n) {
+    *reinterpret_cast<float4*>(y + row) = acc;
+  } else {
+    const float v[4] = {acc.x, acc.y, acc.z, acc.w};
+    for (int k = 0; k < 4 && row + k < n; k++) y[row + k] = v[k];
+  }
+}
+
+}  // namespace
+
+int launch_spmv(hipStream_t s, int w, int h, const float* A, int pitch, const float* x, float* y) {
+  const int n = w * h;
+  if (pitch % 4) return fail(DMSC_ERR_ARG, "spmv: matrix pitch must be a multiple of 4 floats");
+  StencilOffsets so = make_stencil_offsets(w);
+  const size_t lds = sizeof(float) * (kSpmvRows + 2 * so.halo);
+  hipLaunchKernelGGL(k_spmv, dim3(div_up(n, kSpmvRows)), dim3(kSpmvThreads), lds, s, n, so, A, (size_t)pitch,
+                     x, y);
+  DMSC_HIP(hipGetLastError());
+  return DMSC_OK;
+}
+
+}  // namespace dmsc

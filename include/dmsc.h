@@ -1,0 +1,196 @@
+/*
+ * dmsc.h -- C ABI of the MI355X-native gPb library (libdmsc.so).
+ *
+ * Plain pointers and sizes only.  "d_" arguments are device pointers
+ * (hipMalloc'd memory on the context's device), "h_" arguments are host
+ * pointers.  Every entry point returns DMSC_OK (0) or an error code and
+ * never exits the process; dmsc_last_error_string() describes the last
+ * failure on the calling thread.  All work is enqueued on the context's
+ * stream; entry points that return host results synchronise that stream.
+ *
+ * Each stage entry point replaces one function of the reference's stage
+ * API (the reference's include/ headers); the replaced interface is
+ * cited beside it.  The reference-named C++ functions (texton.h, kmeans.h,
+ * localcues.h, ...) in this directory are thin wrappers over these.
+ *
+ * Layouts: images are row-major [h][w] (index y*w + x), N = w*h.  Oriented
+ * stacks are [planes][pitch] floats with pitch >= N (cue stacks: 24 planes,
+ * scale-major then orientation; mPb/combinedG/sPb/gPbAll: 8 planes).  The
+ * affinity matrix is DIA: [81][pitch] floats, diagonal-major, diagonal d
+ * being the d-th point of the radius-5 disc in scan order.
+ */
+#ifndef DMSC_H
+#define DMSC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DMSC_OK = 0,
+  DMSC_ERR_ARG = 1,      /* invalid argument (sizes, null pointers, ranges) */
+  DMSC_ERR_HIP = 2,      /* HIP runtime error */
+  DMSC_ERR_NOCONV = 3,   /* eigensolver reached the iteration cap */
+  DMSC_ERR_NODEV = 4,    /* no usable GPU */
+  DMSC_ERR_INTERNAL = 5
+};
+
+enum { DMSC_PREC_FP32 = 0, DMSC_PREC_MIXED = 1 };
+enum { DMSC_TEXTON32 = 1, DMSC_TEXTON64 = 2 };
+
+typedef struct dmsc_ctx dmsc_ctx;
+
+const char* dmsc_version(void);
+const char* dmsc_last_error_string(void);
+
+/* Context: device, stream (NULL = the device's default stream; the stream
+ * is borrowed, never destroyed) and grow-only scratch arenas.  Not
+ * thread-safe; use one context per thread. */
+int dmsc_ctx_create(dmsc_ctx** ctx, int device, void* stream);
+int dmsc_ctx_destroy(dmsc_ctx* ctx);
+int dmsc_ctx_synchronize(dmsc_ctx* ctx);
+void* dmsc_ctx_stream(dmsc_ctx* ctx);
+
+/* stencilMVM.cu:215-221 findPitchInFloats: row pitch (floats) of a pitched
+ * plane of n floats (256-byte aligned). */
+int dmsc_pitch_in_floats(int n);
+
+/* ------------------------------------------------------------------ host utilities
+ * (CPU only; safe without a GPU) */
+
+/* Synthetic test image: Voronoi cells (uniform sites), uniform random cell
+ * colours, gratings on half the cells (period U[4,16], angle U[0,pi),
+ * amplitude 40), Gaussian noise sigma 8, clipped to uint8.  Packed RGB8 in a
+ * uint32 per pixel (R bits 0-7, G 8-15, B 16-23; convert.h:6).  cells may be
+ * NULL; otherwise receives the cell index per pixel. */
+int dmsc_generate_image(int w, int h, int ncells, uint64_t seed, uint32_t* h_rgb, int32_t* h_cells);
+/* Ground-truth boundary mask: pixels with a 4-neighbour in another cell,
+ * dilated by one pixel (3x3). */
+int dmsc_boundary_mask(int w, int h, const int32_t* h_cells, uint8_t* h_mask);
+
+/* Host tables (bit-identical to what the kernels use). */
+int dmsc_texton_filters(float* h_coeffs /*9010*/, int* h_radii /*34*/); /* filters.cpp:404 */
+int dmsc_parabola_filters(int radius, float* h_out /*8*(2r+1)^2*/);     /* localcues.cu:102 */
+int dmsc_spb_filters(float* h_out /*8*49*/);                            /* spectralPb.cu:708 */
+int dmsc_rotation_tables(int width, int height, float thetaPi, int* new_w, int* new_h,
+                         int* h_bx, int* h_by, int* h_gx, int* h_gy);   /* rotate.cu:805 */
+int dmsc_halfdisc_offsets(int w, int h, float thetaPi, int radius, const int* h_bx,
+                          const int* h_by, const int* h_gx, const int* h_gy, int rotated_w,
+                          int* h_rect /*8*/);                            /* rotate.cu:480 */
+int dmsc_blur_kernel(float sigma, float* h_kernel /*<=64*/, int* radius); /* rotate.cu:29 */
+int dmsc_stencil(int radius, int width, int* h_offsets, int* h_diag_map); /* Stencil.cpp:6 */
+int dmsc_gamma_lut(float gamma, float* h_lut /*256*/);
+
+/* ------------------------------------------------------------------ colour (convert.h) */
+int dmsc_rgb_to_grey(dmsc_ctx* ctx, int w, int h, const uint32_t* d_rgb, float* d_grey);          /* convert.h:53 */
+int dmsc_rgb_to_lab(dmsc_ctx* ctx, int w, int h, float gamma, const uint32_t* d_rgb,
+                    float* d_L, float* d_a, float* d_b);                                          /* convert.h:27 */
+int dmsc_normalize_lab(dmsc_ctx* ctx, int w, int h, float* d_L, float* d_a, float* d_b);          /* convert.h:40 */
+int dmsc_mirror_image_f(dmsc_ctx* ctx, int w, int h, int border, const float* d_in, float* d_out); /* convert.h:75 */
+int dmsc_mirror_image_i(dmsc_ctx* ctx, int w, int h, int border, const int* d_in, int* d_out);     /* convert.h:97 */
+int dmsc_unmirror_image_f(dmsc_ctx* ctx, int bw, int bh, int border, const float* d_in,
+                          float* d_out);                                                          /* convert.h:120 */
+int dmsc_quantize_image(dmsc_ctx* ctx, int w, int h, int nbins, const float* d_in, int* d_out);   /* convert.h:140 */
+
+/* ------------------------------------------------------------------ textons */
+/* texton.cu:39-57: 34 filter responses [34][N] (filter 33 is zero). */
+int dmsc_texton_responses(dmsc_ctx* ctx, int w, int h, const float* d_grey, float* d_resp);
+/* kmeans.h:18 kmeans: k=32 clusters over [nfilters][N] responses; labels
+ * out; *changes = label changes of the last iteration; *iterations = count.
+ * precision: DMSC_PREC_FP32 (VALU distances, reference order) or
+ * DMSC_PREC_MIXED (BF16 MFMA distances + FP32 re-check of near ties; same
+ * labels). */
+int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* d_resp,
+                int* d_labels, int max_iter, int conv_thresh, int precision, int* changes,
+                int* iterations);
+/* texton.h:16 findTextons (texton_choice 1 = 32 textons). */
+int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* d_grey, int* d_textons,
+                      int texton_choice, int precision, int* iterations);
+
+/* ------------------------------------------------------------------ local cues */
+/* localcues.h:20 localCues: Bg/Cga/Cgb/Tg, each [24][cue_pitch]. */
+int dmsc_local_cues(dmsc_ctx* ctx, int w, int h, const float* d_L, const float* d_a,
+                    const float* d_b, const int* d_textons, float* d_bg, float* d_cga,
+                    float* d_cgb, float* d_tg, int cue_pitch, int texton_choice);
+/* As dmsc_local_cues, also exporting the raw chi-squared planes before the
+ * parabola smoothing (gradient.cu:240-241): d_raw [4 cues][3 scales][8][N]. */
+int dmsc_local_cues_raw(dmsc_ctx* ctx, int w, int h, const float* d_L, const float* d_a,
+                        const float* d_b, const int* d_textons, float* d_bg, float* d_cga,
+                        float* d_cgb, float* d_tg, int cue_pitch, float* d_raw);
+
+/* ------------------------------------------------------------------ combine / nonmax / gPb */
+/* combine.h:15 combine -> mPb [8][pitch], combinedG [8][pitch]. */
+int dmsc_combine(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* d_bg,
+                 const float* d_cga, const float* d_cgb, const float* d_tg, float* d_mpb,
+                 float* d_combined_g, int texton_choice);
+/* nonmax.h:16 nonMaxSuppression: pb [8][pitch] -> nmax [N]. */
+int dmsc_nonmax(dmsc_ctx* ctx, int w, int h, const float* d_pb, int pitch, float* d_nmax);
+/* globalPb.h:24 StartCalcGPb. */
+int dmsc_calc_gpb(dmsc_ctx* ctx, int npix, int pitch, int norient, const float* d_combined_g,
+                  const float* d_spb, const float* d_mpb, float* d_gpb_all, float* d_gpb);
+
+/* ------------------------------------------------------------------ affinity / eigensolve */
+/* intervening.h:15 intervene: DIA affinity matrix [81][matrix_pitch]. */
+int dmsc_intervene(dmsc_ctx* ctx, int w, int h, const float* d_mpb, float* d_matrix,
+                   int matrix_pitch, float sigma);
+/* stencilMVM.h:34 convertMatrix: W -> L_sym = I - D^-1/2 W D^-1/2 in place;
+ * d_dsum (may be NULL) = row sums, d_rsqrt = D^-1/2. */
+int dmsc_convert_matrix(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch,
+                        float* d_dsum, float* d_rsqrt);
+/* stencilMVM.h:26 stencilMVM: y = A x. */
+int dmsc_stencil_spmv(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch,
+                      const float* d_x, float* d_y);
+/* lanczos.h:17 generalizedEigensolve: (D-W) v = lambda D v from the affinity
+ * matrix W (overwritten by L_sym).  h_eigvals[nvec] (host), d_eigvecs
+ * [nvec][N] (device, vectors 1.. min-max normalised as the reference).
+ * precision: DMSC_PREC_FP32 = reference algorithm (FP32 Lanczos without
+ * reorthogonalisation, Cullum screening every 500 steps); DMSC_PREC_MIXED =
+ * FP32 SpMV/vectors with FP64 inner products, full reorthogonalisation and
+ * FP64 tridiagonal solve.  Tridiagonal work runs on the GPU in both. */
+int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch,
+                                int nvec, float tol, int precision, float* h_eigvals,
+                                float* d_eigvecs, int* iterations);
+/* spectralPb.h:19 spectralPb -> [8][pitch] (accumulates into d_spb after
+ * zeroing it). */
+int dmsc_spectral_pb(dmsc_ctx* ctx, const float* h_eigvals, const float* d_eigvecs, int w,
+                     int h, int nvec, float* d_spb, int pitch);
+
+/* ------------------------------------------------------------------ whole pipeline */
+typedef struct {
+  int n_eig;          /* number of eigenvectors (2..25), default 9 */
+  float tol;          /* Lanczos residual tolerance, default 1e-3 */
+  int texton_choice;  /* DMSC_TEXTON32 */
+  int precision;      /* DMSC_PREC_FP32 or DMSC_PREC_MIXED */
+} dmsc_gpb_params;
+
+typedef struct {
+  /* caller-allocated device buffers; any may be NULL */
+  int* d_textons;        /* [N] */
+  float* d_mpb;          /* [8][pitch] */
+  float* d_combined_g;   /* [8][pitch] */
+  float* d_nmax;         /* [N] */
+  float* d_eigvecs;      /* [n_eig][N] */
+  float* d_spb;          /* [8][pitch] */
+  float* d_gpb_all;      /* [8][pitch] */
+  float* d_gpb;          /* [N] */
+  int pitch;             /* pitch of the oriented stacks (0 = dmsc_pitch_in_floats(N)) */
+  /* host results */
+  float h_eigvals[32];
+  int kmeans_iterations;
+  int lanczos_iterations;
+  float stage_ms[8];     /* textons, localcues, combine+nonmax, intervene,
+                            eigensolve, spectralPb+gPb (when timing is on) */
+} dmsc_gpb_outputs;
+
+void dmsc_gpb_default_params(dmsc_gpb_params* p);
+/* damascene.cu:269-459: RGB8 (device) -> gPb.  timing != 0 records
+ * per-stage milliseconds with HIP events (adds synchronisation). */
+int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* d_rgb, const dmsc_gpb_params* params,
+             dmsc_gpb_outputs* out, int timing);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

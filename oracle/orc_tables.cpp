@@ -1,0 +1,492 @@
+// orc_tables.cpp -- oracle restatement of the reference's host-side table
+// builders (texton filter bank, Savitzky-Golay parabola filters, Bresenham
+// rotation tables, half-disc rectangle offsets, histogram blur kernel,
+// stencil geometry, spectralPb filters).  TEST INFRASTRUCTURE ONLY.
+//
+// Float/double semantics follow the reference sources as compiled by a
+// C++11 host compiler: float overloads of sin/cos/exp/round/ceil for float
+// arguments, double pow(float,int), no contraction (-ffp-contract=off).
+#include "oracle.h"
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// filters.cpp:17-155 `gaussian` (1-D kernel, optional DFT Hilbert transform,
+// zero mean for derivatives, unit L1 norm).
+std::vector<float> gaussian1d(double sigma, unsigned deriv, bool hlbrt, unsigned long support) {
+  const double sigma2_inv = double(1) / (sigma * sigma);
+  const double neg_two_sigma2_inv = double(-0.5) * sigma2_inv;
+  const unsigned long size = 2 * support + 1;
+  std::vector<float> m(size);
+  double x = -(static_cast<double>(support));
+  for (unsigned long n = 0; n < size; n++, x++) {
+    if (deriv == 0) {
+      m[n] = (float)exp(x * x * neg_two_sigma2_inv);
+    } else if (deriv == 1) {
+      m[n] = (float)(exp(x * x * neg_two_sigma2_inv) * (-x));
+    } else {
+      double x2 = x * x;
+      m[n] = (float)(exp(x2 * neg_two_sigma2_inv) * (x2 * sigma2_inv - 1));
+    }
+  }
+  if (hlbrt) {
+    // filters.cpp:73-127: O(n^2) DFT in complex<float>; positive bins doubled,
+    // bins above size/2+1 zeroed, inverse DFT, keep the imaginary part.
+    std::vector<float> dre(size, 0.f), dim(size, 0.f), ire(size, 0.f), iim(size, 0.f);
+    for (int i = 0; i < (int)size; i++) {
+      for (int j = 0; j < (int)size; j++) {
+        float fr = (float)cos(2 * M_PI * i * j / size);
+        float fi = (float)(-sin(2 * M_PI * i * j / size));
+        dre[i] = dre[i] + m[j] * fr;
+        dim[i] = dim[i] + m[j] * fi;
+      }
+    }
+    for (int i = 0; i < (int)size; i++) {
+      if (i != 0) {
+        if (i < (int)((size + 1) / 2)) {
+          dre[i] *= 2.0f;
+          dim[i] *= 2.0f;
+        } else if (i > (int)(size / 2 + 1)) {
+          dre[i] = 0.f;
+          dim[i] = 0.f;
+        }
+      }
+    }
+    for (int i = 0; i < (int)size; i++) {
+      for (int j = 0; j < (int)size; j++) {
+        float c = (float)(cos(2 * M_PI * i * j / size) / size);
+        float d = (float)(sin(2 * M_PI * i * j / size) / size);
+        float a = dre[j], b = dim[j];
+        float pr = a * c - b * d;
+        float pi = a * d + b * c;
+        ire[i] = ire[i] + pr;
+        iim[i] = iim[i] + pi;
+      }
+    }
+    for (unsigned long i = 0; i < size; i++) m[i] = iim[i];
+  }
+  if (deriv > 0) {
+    float sum = 0;
+    for (unsigned long i = 0; i < size; i++) sum += m[i];
+    float mean = sum / (float)size;
+    for (unsigned long i = 0; i < size; i++) m[i] -= mean;
+  }
+  float sabs = 0;
+  for (unsigned long i = 0; i < size; i++) sabs += std::fabs(m[i]);
+  for (unsigned long i = 0; i < size; i++) m[i] /= sabs;
+  return m;
+}
+
+// filters.cpp:157-190 rotated support of the bounding box.
+unsigned long support_rot(unsigned long sx, unsigned long sy, double ori, bool xaxis) {
+  double fsx = (double)sx, fsy = (double)sy;
+  double r;
+  if (xaxis) {
+    const double a = fsx * cos(ori), b = fsy * sin(ori);
+    double m0 = std::fabs(a - b), m1 = std::fabs(a + b);
+    r = (m0 > m1) ? m0 : m1;
+  } else {
+    const double a = fsx * sin(ori), b = fsy * cos(ori);
+    double m0 = std::fabs(a - b), m1 = std::fabs(a + b);
+    r = (m0 > m1) ? m0 : m1;
+  }
+  return (unsigned long)ceil(r);
+}
+
+// filters.cpp:192-258 bilinear rotation of the axis-aligned kernel.
+// Destination entries whose source falls outside the grid are left at 0
+// (the reference leaves them uninitialised).
+void rotate2d(const float* src, float* dst, unsigned long sxs, unsigned long sys,
+              unsigned long sxd, unsigned long syd, double ori) {
+  const double cos_ori = cos(ori), sin_ori = sin(ori);
+  const double ox = static_cast<double>(sxs - 1) / 2;
+  const double oy = static_cast<double>(sys - 1) / 2;
+  double u = -(static_cast<double>(sxd - 1) / 2);
+  unsigned long n = 0;
+  for (unsigned long dx = 0; dx < sxd; dx++) {
+    double v = -(static_cast<double>(syd - 1) / 2);
+    for (unsigned long dy = 0; dy < syd; dy++) {
+      double x = u * cos_ori + v * sin_ori + ox;
+      double y = v * cos_ori - u * sin_ori + oy;
+      if ((x >= 0) && (y >= 0)) {
+        unsigned long x0 = (unsigned long)floor(x), x1 = (unsigned long)ceil(x);
+        unsigned long y0 = (unsigned long)floor(y), y1 = (unsigned long)ceil(y);
+        if ((x1 < sxs) && (y1 < sys)) {
+          double dx0 = x - x0, dx1 = x1 - x, dy0 = y - y0, dy1 = y1 - y;
+          const float m00 = src[x0 * sys + y0], m01 = src[x0 * sys + y1];
+          const float m10 = src[x1 * sys + y0], m11 = src[x1 * sys + y1];
+          const float t0 = (x0 != x1) ? (float)(dx1 * m00 + dx0 * m10) : m00;
+          const float t1 = (x0 != x1) ? (float)(dx1 * m01 + dx0 * m11) : m01;
+          dst[n] = (y0 != y1) ? (float)(dy1 * t0 + dy0 * t1) : t0;
+        }
+      }
+      n++;
+      v++;
+    }
+    u++;
+  }
+}
+
+// LAPACK-reference unblocked LU with partial pivoting (sgetf2) followed by
+// the unblocked inverse (strti2 + sgetri), column-major, float.  Stands in
+// for the ACML sgetrf/sgetri call at localcues.cu:73-74.
+void inverse_lu(float* A, int n) {
+  int ipiv[8];
+  for (int j = 0; j < n; j++) {
+    int p = j;
+    float best = std::fabs(A[j + j * n]);
+    for (int i = j + 1; i < n; i++) {
+      float v = std::fabs(A[i + j * n]);
+      if (v > best) { best = v; p = i; }
+    }
+    ipiv[j] = p;
+    if (A[p + j * n] != 0.f) {
+      if (p != j)
+        for (int k = 0; k < n; k++) { float t = A[j + k * n]; A[j + k * n] = A[p + k * n]; A[p + k * n] = t; }
+      float r = 1.0f / A[j + j * n];
+      for (int i = j + 1; i < n; i++) A[i + j * n] = A[i + j * n] * r;
+    }
+    for (int k = j + 1; k < n; k++) {
+      float temp = -A[j + k * n];
+      for (int i = j + 1; i < n; i++) A[i + k * n] = A[i + k * n] + A[i + j * n] * temp;
+    }
+  }
+  // strti2 (upper, non-unit)
+  for (int j = 0; j < n; j++) {
+    A[j + j * n] = 1.0f / A[j + j * n];
+    float ajj = -A[j + j * n];
+    for (int j2 = 0; j2 < j; j2++) {  // strmv on x = A(0:j-1, j)
+      float temp = A[j2 + j * n];
+      if (temp != 0.f) {
+        for (int i = 0; i < j2; i++) A[i + j * n] = A[i + j * n] + temp * A[i + j2 * n];
+        A[j2 + j * n] = A[j2 + j * n] * A[j2 + j2 * n];
+      }
+    }
+    for (int i = 0; i < j; i++) A[i + j * n] = A[i + j * n] * ajj;
+  }
+  // sgetri (unblocked)
+  float work[8];
+  for (int j = n - 1; j >= 0; j--) {
+    for (int i = j + 1; i < n; i++) { work[i] = A[i + j * n]; A[i + j * n] = 0.f; }
+    for (int jj = j + 1; jj < n; jj++) {
+      float temp = -1.0f * work[jj];
+      for (int i = 0; i < n; i++) A[i + j * n] = A[i + j * n] + temp * A[i + jj * n];
+    }
+  }
+  for (int j = n - 2; j >= 0; j--) {
+    int jp = ipiv[j];
+    if (jp != j)
+      for (int i = 0; i < n; i++) { float t = A[i + j * n]; A[i + j * n] = A[i + jp * n]; A[i + jp * n] = t; }
+  }
+}
+
+// rotate.cu:65-151 integer Bresenham line.
+void bresenham(int x0, int y0, int x1, int y1, int& length, int* x, int* y) {
+  int dx = x1 - x0, dy = y1 - y0;
+  int adx = std::abs(dx), ady = std::abs(dy);
+  length = ((adx > ady) ? adx : ady) + 1;
+  int octant = -1;
+  if (dx > 0 && dy >= 0) octant = (adx > ady) ? 0 : 1;
+  else if (dx <= 0 && dy > 0) octant = (adx < ady) ? 2 : 3;
+  else if (dy <= 0 && dx < 0) octant = (adx > ady) ? 4 : 5;
+  else if (dx >= 0 && dy < 0) octant = (adx < ady) ? 6 : 7;
+  int t = 0;
+  switch (octant) {
+    case 0: case 3: case 4: case 7: t = -adx; break;
+    default: t = -ady; break;
+  }
+  int xi = x0, yi = y0, index = 1;
+  x[0] = xi; y[0] = yi;
+  while (xi != x1 || yi != y1) {
+    switch (octant) {
+      case 0: xi++; t += (ady << 1); if (t > 0) { yi++; t -= (adx << 1); } break;
+      case 1: yi++; t += (adx << 1); if (t > 0) { xi++; t -= (ady << 1); } break;
+      case 2: yi++; t += (adx << 1); if (t > 0) { xi--; t -= (ady << 1); } break;
+      case 3: xi--; t += (ady << 1); if (t > 0) { yi++; t -= (adx << 1); } break;
+      case 4: xi--; t += (ady << 1); if (t > 0) { yi--; t -= (adx << 1); } break;
+      case 5: yi--; t += (adx << 1); if (t > 0) { xi--; t -= (ady << 1); } break;
+      case 6: yi--; t += (adx << 1); if (t > 0) { xi++; t -= (ady << 1); } break;
+      case 7: xi++; t += (ady << 1); if (t > 0) { yi--; t -= (adx << 1); } break;
+    }
+    x[index] = xi; y[index] = yi; index++;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// filters.cpp:261-336 gaussian_2D.
+void orc_gaussian_2d(float* filter, double sigma_x, double sigma_y, double ori, unsigned deriv,
+                     int hilbert, unsigned long sx, unsigned long sy) {
+  unsigned long sxr = support_rot(sx, sy, -ori, true);
+  unsigned long syr = support_rot(sx, sy, -ori, false);
+  std::vector<float> mx = gaussian1d(sigma_x, 0, false, sxr);
+  std::vector<float> my = gaussian1d(sigma_y, deriv, hilbert != 0, syr);
+  int mxs = 2 * (int)sxr + 1, mys = 2 * (int)syr + 1;
+  std::vector<float> m((size_t)mxs * mys);
+  size_t n = 0;
+  for (int a = 0; a < mxs; a++)
+    for (int b = 0; b < mys; b++) m[n++] = mx[a] * my[b];
+  int size = (int)((2 * sx + 1) * (2 * sy + 1));
+  std::vector<float> r(size, 0.f);
+  rotate2d(m.data(), r.data(), mxs, mys, 2 * sx + 1, 2 * sy + 1, ori);
+  if (deriv > 0) {
+    float sum = 0;
+    for (int i = 0; i < size; i++) sum += r[i];
+    float mean = sum / (float)size;
+    for (int i = 0; i < size; i++) r[i] -= mean;
+  }
+  float sabs = 0;
+  for (int i = 0; i < size; i++) sabs += std::fabs(r[i]);
+  for (int i = 0; i < size; i++) r[i] /= sabs;
+  for (int i = 0; i < size; i++) filter[i] = r[i];
+}
+
+// filters.cpp:339-401 gaussian_cs_2D.
+static void gaussian_cs_2d(float* filter, double sigma_x, double sigma_y, double ori,
+                           double scale_factor, unsigned long sx, unsigned long sy) {
+  int size = (int)((2 * sx + 1) * (2 * sy + 1));
+  std::vector<float> c(size), s(size), m(size);
+  orc_gaussian_2d(c.data(), sigma_x / scale_factor, sigma_y / scale_factor, ori, 0, 0, sx, sy);
+  orc_gaussian_2d(s.data(), sigma_x, sigma_y, ori, 0, 0, sx, sy);
+  float sum = 0;
+  for (int i = 0; i < size; i++) { m[i] = s[i] - c[i]; sum += m[i]; }
+  float mean = sum / (float)size;
+  for (int i = 0; i < size; i++) m[i] -= mean;
+  float sabs = 0;
+  for (int i = 0; i < size; i++) sabs += std::fabs(m[i]);
+  for (int i = 0; i < size; i++) m[i] /= sabs;
+  for (int i = 0; i < size; i++) filter[i] = m[i];
+}
+
+// filters.cpp:404-454 createTextonFilters with texton.cu:74-82 scales
+// {2, 2*sqrt(2)}, elongation 3.  Writes all 34 filters (9010 floats) and
+// returns the coefficient count the reference binds (total minus the last
+// filter, so filter 33 reads as zero on the device).
+int orc_texton_filters(float* coeffs, int* radii) {
+  float scale[2] = {2.0f, (float)(2.0 * M_SQRT2)};
+  const float elongation = 3.0f;
+  int i = 0, r = 0, filterLength = 0;
+  for (int n = 0; n < 2; n++) {
+    int support = (int)std::ceil(3 * scale[n]);
+    filterLength = support * 2 + 1;
+    for (int o = 0; o < 8; o++) {
+      orc_gaussian_2d(coeffs + i, scale[n], scale[n] / elongation, o * M_PI / 8, 2, 0, support, support);
+      i += filterLength * filterLength;
+      radii[r++] = support;
+    }
+    for (int o = 0; o < 8; o++) {
+      orc_gaussian_2d(coeffs + i, scale[n], scale[n] / elongation, o * M_PI / 8, 2, 1, support, support);
+      i += filterLength * filterLength;
+      radii[r++] = support;
+    }
+    gaussian_cs_2d(coeffs + i, scale[n], scale[n], 0, (double)M_SQRT2l, support, support);
+    i += filterLength * filterLength;
+    radii[r++] = support;
+  }
+  return i - filterLength * filterLength;
+}
+
+// localcues.cu:21-100 savgol_filter (degree d, value at the centre, k=1).
+void orc_savgol_filter(float* filt, int d, float inra, float inrb, float theta) {
+  float ra = (inra > 1.5f) ? inra : 1.5f;
+  float rb = (inrb > 1.5f) ? inrb : 1.5f;
+  float ira2 = (float)(1.0 / (ra * ra));
+  float irb2 = (float)(1.0 / (rb * rb));
+  int wr = int(std::floor(ra > rb ? ra : rb) + 0.5);
+  int wd = 2 * wr + 1;
+  float sint = std::sin(theta);
+  float cost = std::cos(theta);
+  memset(filt, 0, sizeof(float) * wd * wd);
+  std::vector<float> xx(2 * d + 1, 0.f);
+  for (int u = -wr; u <= wr; u++)
+    for (int v = -wr; v <= wr; v++) {
+      float ai = (float)(-u) * sint + (float)v * cost;
+      float bi = (float)u * cost + (float)v * sint;
+      if (ai * ai * ira2 + bi * bi * irb2 > 1) continue;
+      for (int i = 0; i < 2 * d + 1; i++) xx[i] = (float)((double)xx[i] + std::pow((double)ai, i));
+    }
+  std::vector<float> A((d + 1) * (d + 1));
+  for (int i = 0; i < d + 1; i++)
+    for (int j = 0; j < d + 1; j++) A[i * (d + 1) + j] = xx[i + j];
+  inverse_lu(A.data(), d + 1);
+  for (int u = -wr; u <= wr; u++)
+    for (int v = -wr; v <= wr; v++) {
+      float ai = (float)(-u) * sint + (float)v * cost;
+      float bi = (float)u * cost + (float)v * sint;
+      if (ai * ai * ira2 + bi * bi * irb2 > 1) continue;
+      for (int i = 0; i < d + 1; i++) {
+        float& f = filt[(v + wr) * wd + u + wr];
+        f = (float)((double)f + (double)A[i] * std::pow((double)ai, i));
+      }
+    }
+}
+
+// localcues.cu:102-140 construct_parabola_filters for one radius, 8
+// orientations at theta = pi/2 - o*pi/8, layout [o][dy][dx].
+void orc_parabola_filters(int radius, float* out) {
+  int len = 2 * radius + 1;
+  for (int o = 0; o < 8; o++)
+    orc_savgol_filter(out + len * len * o, 2, (float)radius, float(radius) / 4.0f,
+                      (float)(M_PI / 2 - o * M_PI / 8));
+}
+
+// sPb/spectralPb.cu:708-717: slot (7-o) holds the first-derivative gaussian
+// at angle pi/2 + o*pi/8, sigma 1, support 3.
+void orc_spb_filters(float* out) {
+  for (int ori = 7; ori >= 0; ori--)
+    orc_gaussian_2d(out + (7 - ori) * 49, 1, 1, M_PI / 2 + ori * M_PI / 8, 1, 0, 3, 3);
+}
+
+// rotate.cu:805-850 rotation tables (host part of rotateImage).
+int orc_rotation_tables(int width, int height, float thetaPi, int* newW, int* newH,
+                        int* bX, int* bY, int* gX, int* gY) {
+  int length;
+  if (thetaPi == 0) {
+    for (int i = 0; i < width; i++) { bX[i] = i; bY[i] = 0; }
+    for (int i = 0; i < height; i++) { gX[i] = 0; gY[i] = i; }
+    *newW = width;
+    *newH = height;
+  } else if ((thetaPi < 0) && (thetaPi >= -.25)) {
+    float theta = (float)(thetaPi * 3.1415926535897932384626433832795029L);
+    float t = std::tan(theta);
+    *newW = (int)std::ceil((float)width - (float)(height - 1) * t);
+    *newH = (int)std::ceil((float)height - (float)(width - 1) * t);
+    int yDescent = (int)-std::round(((float)(width - 1)) * t);
+    int xTravel = (int)-std::round(((float)(height - 1)) * t);
+    bresenham(0, 0, width - 1, yDescent, length, bX, bY);
+    if (length != width) return -1;
+    bresenham(xTravel, 0, 0, height - 1, length, gX, gY);
+    if (length != height) return -1;
+  } else if ((thetaPi < -.25) && (thetaPi > -.5)) {
+    float theta = (float)((thetaPi + 0.25) * 3.1415926535897932384626433832795029L);
+    float t = std::tan(theta);
+    *newW = (int)std::ceil((float)height - (float)(width - 1) * t);
+    *newH = (int)std::ceil((float)width - (float)(height - 1) * t);
+    int xTravel = (int)-std::round(((float)(width - 1)) * t);
+    int yTravel = (int)-std::round(((float)(height - 1)) * t);
+    bresenham(0, 0, xTravel, width - 1, length, bX, bY);
+    if (length != width) return -1;
+    bresenham(height - 1, 0, 0, yTravel, length, gX, gY);
+    if (length != height) return -1;
+  } else {
+    return -2;
+  }
+  return 0;
+}
+
+// rotate.cu:480-614 dispatchGradient geometry: the half-disc of radius r is
+// approximated by an axis-aligned rectangle in the rotated frame; eight
+// corner offsets (A..H) as linear offsets in the rotated image.  width and
+// height are the UNmirrored image sizes (the reference indexes the tables
+// there).
+void orc_halfdisc_offsets(int width, int height, float thetaPi, int radius, const int* bX,
+                          const int* bY, const int* gX, const int* gY, int rotatedWidth,
+                          int* rect) {
+  float theta = (float)(thetaPi * 3.1415926535897932384626433832795029L);
+  float er = (float)((-4.0 + 4.0 * (double)(float)radius *
+                                 (double)std::sqrt((float)3.1415926535897932384626433832795029L)) /
+                     8.0);
+  float s = std::sin(theta), c = std::cos(theta);
+  int x2 = (int)std::round(-er * s - er * c);
+  int y2 = (int)std::round(-er * s + er * c);
+  int x3 = (int)std::round(er * s + er * c);
+  int y3 = (int)std::round(er * s - er * c);
+  int xc = width / 2, yc = height / 2;
+  int x2c = x2 + xc, y2c = y2 + yc, x3c = x3 + xc, y3c = y3 + yc;
+  int x2rc = gX[y2c] + bX[x2c], y2rc = gY[y2c] + bY[x2c];
+  int x3rc = gX[y3c] + bX[x3c], y3rc = gY[y3c] + bY[x3c];
+  int xcr = gX[yc] + bX[xc], ycr = gY[yc] + bY[xc];
+  int x2r = x2rc - xcr, y2r = y2rc - ycr, x3r = x3rc - xcr, y3r = y3rc - ycr;
+  int xD = x3r - x2r + 1, yD = y2r - y3r + 1;
+  int xR = xD / 2, yR = yD / 2;
+  int x[8], y[8];
+  if (xD & 1) {
+    x[0] = -xR - 1; x[1] = 0; x[2] = xR; x[3] = -xR - 1; x[4] = xR; x[5] = -xR - 1; x[6] = 0; x[7] = xR;
+  } else {
+    x[0] = -xR; x[1] = 0; x[2] = xR; x[3] = -xR; x[4] = xR; x[5] = -xR; x[6] = 0; x[7] = xR;
+  }
+  if (yD & 1) {
+    y[0] = y[1] = y[2] = -yR - 1;
+  } else {
+    y[0] = y[1] = y[2] = -yR;
+  }
+  y[3] = y[4] = 0;
+  y[5] = y[6] = y[7] = yR;
+  for (int i = 0; i < 8; i++) rect[i] = y[i] * rotatedWidth + x[i];
+}
+
+// rotate.cu:29-62 precompute_gaussian (histogram blur over bins).
+int orc_blur_kernel(float sigma, float* kernel, int* radius) {
+  int kr = (int)std::ceil(sigma * 3);
+  int kl = 2 * kr + 1;
+  float sigma2_inv = float(1) / sigma / sigma;
+  float neg = float(-0.5) * sigma2_inv;
+  float sum = 0;
+  for (int i = 0; i < kl; i++) {
+    kernel[i] = std::exp(float(i - kr) * float(i - kr) * neg);
+    sum += kernel[i];
+  }
+  for (int i = 0; i < kl; i++) kernel[i] /= sum;
+  *radius = kr;
+  return kl;
+}
+
+// Stencil.cpp:6-35: radius-r disc offsets in scan order, delta coded, and
+// the (dy,dx) -> diagonal map.
+int orc_stencil(int radius, int width, int* offsets, int* diagMap) {
+  int r2 = radius * radius, prev = 0, n = 0, diameter = 2 * radius + 1;
+  for (int i = 0; i < diameter * diameter; i++) diagMap[i] = -1;
+  for (int row = -radius; row <= radius; row++)
+    for (int col = -radius; col <= radius; col++)
+      if (row * row + col * col <= r2) {
+        int off = row * width + col;
+        offsets[n] = off - prev;
+        prev = off;
+        diagMap[(row + radius) * diameter + col + radius] = n;
+        n++;
+      }
+  return n;
+}
+
+// convert.cu:33-38: sRGB channel c/255 raised to gamma, tabulated over the
+// 256 channel values (double pow, rounded to float).
+void orc_gamma_lut(float gamma, float* lut) {
+  for (int c = 0; c < 256; c++) {
+    float r = (float)c / 255.0f;
+    lut[c] = (float)std::pow((double)r, (double)gamma);
+  }
+}
+
+// Cube root from basic IEEE double operations only (bit-identical on any
+// IEEE-754 machine without contraction): exponent-split initial guess and
+// four Newton steps, rounded to float.  Replaces powf(t, 1/3) at
+// convert.cu:52-60.
+float orc_cbrtf(float xf) {
+  if (xf == 0.0f) return xf;
+  double x = (double)xf;
+  double ax = x < 0 ? -x : x;
+  uint64_t bits;
+  memcpy(&bits, &ax, 8);
+  int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);
+  int rem = e - 3 * q;  // 0..2
+  uint64_t mbits = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m;
+  memcpy(&m, &mbits, 8);
+  double mr = m;
+  for (int i = 0; i < rem; i++) mr = mr * 2.0;  // mr in [1,8)
+  double y = 1.0 + (mr - 1.0) * (1.0 / 7.0);      // rough guess in [1,2)
+  for (int it = 0; it < 6; it++) y = y - (y * y * y - mr) / (3.0 * y * y);
+  uint64_t sb = (uint64_t)(q + 1023) << 52;
+  double scale;
+  memcpy(&scale, &sb, 8);
+  double r = y * scale;
+  return (float)(x < 0 ? -r : r);
+}
+
+}  // extern "C"

@@ -25,6 +25,7 @@ int fail(int code, const char* fmt, ...) {
 }
 
 Ctx::~Ctx() {
+  release_eigensolver(this);
   for (int i = 0; i < kArenas; i++)
     if (arena[i]) (void)hipFree(arena[i]);
   if (d_texton_coeffs) (void)hipFree(d_texton_coeffs);
@@ -397,7 +398,7 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
     const size_t mixed_bytes = p.precision == DMSC_PREC_MIXED ? kmeans_mixed_work_bytes(n) : 0;
     DMSC_TRY(ctx->scratch(2, st + sizeof(float) * P + mixed_bytes, &scr));
     float* pdot = (float*)((char*)scr + st);
-    if (p.precision == DMSC_PREC_MIXED)
+    if (p.precision == DMSC_PREC_MIXED && kmeans_mixed_work_bytes(n) > 0)
       DMSC_TRY(launch_kmeans_mixed(s, w, h, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
     else
       DMSC_TRY(launch_kmeans(s, w, h, kTextons, kNFilters, resp, tex, pdot, scr, 15));

@@ -49,40 +49,48 @@ StencilOffsets make_stencil_offsets(int w);
 int launch_convert_matrix(hipStream_t s, int w, int h, float* A, int pitch, float* dsum, float* rs);
 int launch_spmv(hipStream_t s, int w, int h, const float* A, int pitch, const float* x, float* y);
 
-constexpr int kSpmvThreads = 256;
-constexpr int kSpmvRows = kSpmvThreads * 4;
-
-// Stage x[r0 - halo, r0 + kSpmvRows + halo) * scale into LDS, zero outside
-// [0, n).
-__device__ __forceinline__ void spmv_stage_x(float* xs, const float* __restrict__ x, int n, int r0, int halo,
-                                             float scale) {
-  const int len = kSpmvRows + 2 * halo;
-  for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    const int g = r0 - halo + i;
-    xs[i] = (g >= 0 && g < n) ? x[g] * scale : 0.0f;
-  }
-}
-
-// Four consecutive rows starting at `row`; xs index of row is `xb`.
-__device__ __forceinline__ void spmv_rows4(float4& acc, const float* __restrict__ A, size_t pitch,
-                                           const float* xs, int xb, const StencilOffsets& so) {
-  acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int row_off = 0;
-  (void)row_off;
+// Stencil SpMV building block: RPL consecutive rows per lane starting at
+// `row` (RPL * 4 bytes aligned), one vector load of A per diagonal, x
+// gathered through the caches; a column outside [0, n) contributes
+// fma(A, 0, acc) == acc (acc is never -0), which is the reference's bounds
+// test without a branch.  XF(col) supplies x[col] (any float-valued source).
+template <int RPL, typename XF>
+__device__ __forceinline__ void spmv_rows(float (&acc)[RPL], const float* __restrict__ A, size_t pitch, int row,
+                                          int n, const StencilOffsets& so, XF xf) {
+#pragma unroll
+  for (int j = 0; j < RPL; j++) acc[j] = 0.0f;
 #pragma unroll 9
   for (int d = 0; d < kStencilArea; d++) {
-    const float4 a = *reinterpret_cast<const float4*>(A + (size_t)d * pitch);
-    const float* xp = xs + xb + so.off[d];
-    acc.x = fmaf(a.x, xp[0], acc.x);
-    acc.y = fmaf(a.y, xp[1], acc.y);
-    acc.z = fmaf(a.z, xp[2], acc.z);
-    acc.w = fmaf(a.w, xp[3], acc.w);
+    float a[RPL];
+    const float* ap = A + (size_t)d * pitch + row;
+    if constexpr (RPL == 4) {
+      const float4 v = *reinterpret_cast<const float4*>(ap);
+      a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
+    } else if constexpr (RPL == 2) {
+      const float2 v = *reinterpret_cast<const float2*>(ap);
+      a[0] = v.x; a[1] = v.y;
+    } else {
+#pragma unroll
+      for (int j = 0; j < RPL; j++) a[j] = ap[j];
+    }
+#pragma unroll
+    for (int j = 0; j < RPL; j++) {
+      const int col = row + j + so.off[d];
+      const bool in = (unsigned)col < (unsigned)n;
+      const float xv = xf(in ? col : 0);
+      acc[j] = fmaf(a[j], in ? xv : 0.0f, acc[j]);
+    }
   }
 }
+
+// Launch shapes chosen by the tuner (rows per lane: 1, 2 or 4).
+extern int spmv_rows_per_lane;
+extern int lanczos_rows_per_lane;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,
                    float* h_eigvals, float* d_eigvecs, int* iterations);
+void release_eigensolver(const Ctx* ctx);  // drops the context's captured graphs
 
 // ------------------------------------------------------------- spectral Pb
 int launch_spectral_pb(hipStream_t s, const float* h_eigvals, const float* d_eigvecs, int w, int h, int nvec,

@@ -2,26 +2,37 @@
 // noReorthog/lanczos.cu, stencilMatrixMultiply/stencilMVM.cu).
 //
 // (D - W) v = lambda D v is solved as L_sym u = lambda u with
-// L_sym = I - D^-1/2 W D^-1/2 and v = D^-1/2 u.  Two precisions:
+// L_sym = I - D^-1/2 W D^-1/2 and v = D^-1/2 u, by Lanczos without
+// reorthogonalisation from q_0 = 1/sqrt(N), with Cullum-Willoughby screening
+// of spurious Ritz values (lanczos.cu:279-384).  Two precisions:
 //
-//  FP32 ("plain", the reference algorithm): FP32 SpMV, vectors and inner
-//    products, no reorthogonalisation, Ritz values checked every 500 steps
-//    with Cullum-Willoughby screening of spurious copies (lanczos.cu:279-384).
-//  MIXED: FP32 SpMV and basis, FP64 inner products, full classical
-//    Gram-Schmidt reorthogonalisation twice per step (FP64 coefficients),
-//    FP64 tridiagonal eigensolve; converged Ritz pairs are tested every 8
-//    steps, no screening needed.
+//  FP32 ("plain", the reference algorithm): FP32 SpMV, vectors, alpha and
+//    beta; inner products accumulated in FP64 and rounded to FP32 (one
+//    fixed, reproducible choice for the reference's cuBLAS sdot);
+//    convergence tested every 500 steps (lanczos.cu:5, 519) with the FP32
+//    residual test of lanczos.cu:114-140.
+//  MIXED: FP32 matrix and FP32-stored basis, FP64 SpMV accumulation,
+//    three-term vector and inner products; tested every 25 steps and
+//    stopped when every residual |beta_m s_{m,j}| is at most tol * 1e-3.
 //
-// One Lanczos step is three kernels with no host round trip: (1) stencil
-// SpMV fused with the three-term subtraction and the q.z partial sums, (2)
-// alpha from the partials (every block reduces them in the same fixed order,
-// so alpha is identical everywhere and deterministic) fused with
-// z -= alpha q and the |z|^2 partials, (3) beta and q_{k+1} = z / beta
-// written into the basis.  Steps are enqueued in chunks captured in a HIP
-// graph; alpha/beta stay on the device until a convergence check.
+// One step is two kernels and no host round trip:
+//   K1: beta_{k-1} from the |z|^2 partials (every block reduces them in the
+//       same order, so all blocks agree bit for bit), q_k = z_{k-1}/beta_{k-1}
+//       formed on the fly for the stencil gather and written to the basis,
+//       z_k = A q_k - beta_{k-1} q_{k-1}, partial sums of q_k . z_k;
+//   K2: alpha_k from the partials, z_k -= alpha_k q_k, partial |z_k|^2; the
+//       last block to finish advances the step counter in device memory.
+// Because the step index lives on the device, a whole check period is
+// captured once into a HIP graph and replayed.  At a check point a grid of
+// one-wave blocks finds the FP64 Ritz values of T and of T without its first
+// row/column by 64-way multisection, one block screens them and computes
+// the Ritz vectors by inverse iteration (one eigenvalue per lane), and the
+// host reads back a small status record.
 #include <algorithm>
 #include <cmath>
-#include <vector>
+#include <map>
+#include <memory>
+#include <tuple>
 
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
@@ -29,162 +40,318 @@
 #include "tridiag.hpp"
 
 namespace dmsc {
+
+int lanczos_rows_per_lane = 1;  // kept for the tuner interface; K1 runs one row per lane
+
 namespace {
 
-constexpr int kRedThreads = 256;
+constexpr int kCheckMixed = 25;
+constexpr int kMaxEig = 25;
+constexpr int kCullumExtra = 20;  // eigCheck = nvec + 20 (lanczos.cu:456)
+constexpr int kCullumMax = 1000;  // lanczos.cu:531
+constexpr int kWave = 64;
+constexpr int kK1 = 256;          // K1: one stencil row per lane
+constexpr int kK2 = 256;          // K2: kK2Rows rows per lane
+constexpr int kK2Rows = 8;
+constexpr int kBisect = 256;      // shifts per multisection round
+constexpr int kInvitLanes = 32;   // interleave of the per-lane inverse-iteration arrays
+constexpr int kInvitIters = 2;
+constexpr int kMaxGraphs = 16;
 
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T* sh) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s >= 1; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  const T r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-// Deterministic sum of nb partials (fixed order independent of the block).
-template <typename T>
-__device__ __forceinline__ T sum_partials(const T* __restrict__ part, int nb, T* sh) {
-  T v = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) v = v + part[i];
-  return block_sum(v, sh);
-}
-
-struct LzArgs {
-  int n;
-  size_t pitch;
-  const float* A;
-  float* V;        // basis, vector k at V + k * vstride
-  size_t vstride;
-  float* z;
-  void* part;      // partial sums (T)
-  void* part2;
-  void* alpha;     // T[kMaxIter + 2]
-  void* beta;
-  int nb;          // number of SpMV blocks
+// Device-side solver state.
+struct LzState {
+  int k;                 // index of the next step (read by K1, advanced by K2)
+  int k2;                // step index handed from K1 to K2
+  float* V;              // basis: q_j at V + j * vstride
+  int converged;         // written by the check
+  int nscreened;         // screened Ritz values found
+  int fail;              // 1: fewer than nvec screened values, 2: inverse iteration failed
+  int m;                 // order of T at the last check
+  double vals[kMaxEig];  // screened Ritz values
+  double res[kMaxEig];   // residuals |beta_m s_{m,j}|
 };
 
-// step 1: z = A q_k - beta_{k-1} q_{k-1}; partial q_k . z
 template <typename T>
-__global__ __launch_bounds__(kSpmvThreads) void k_lz_spmv(LzArgs a, StencilOffsets so, int k) {
-  extern __shared__ float xs[];
-  __shared__ T sh[kSpmvThreads];
-  const float* q = a.V + (size_t)k * a.vstride;
-  const int r0 = blockIdx.x * kSpmvRows;
-  spmv_stage_x(xs, q, a.n, r0, so.halo, 1.0f);
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over the block in a fixed order; every thread gets the result.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
   __syncthreads();
-  const int row = r0 + threadIdx.x * 4;
-  T dot = 0;
-  if (row < a.n) {
-    float4 acc;
-    spmv_rows4(acc, a.A + row, a.pitch, xs, row - r0 + so.halo, so);
-    float zz[4] = {acc.x, acc.y, acc.z, acc.w};
-    if (k > 0) {
-      const float nb = -(float)((const T*)a.beta)[k - 1];
-      const float* qp = a.V + (size_t)(k - 1) * a.vstride + row;
-      for (int j = 0; j < 4; j++) zz[j] = fmaf(nb, qp[j], zz[j]);
+  double t = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / kWave; w++) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// Sum of nb partials, identical in every block (fixed assignment and order).
+template <int NT>
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb, double* red) {
+  double v = 0;
+  for (int i = threadIdx.x; i < nb; i += NT) v += part[i];
+  return block_sum<NT>(v, red);
+}
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
+// Z: type of the three-term vector, alpha and beta (float: plain, double:
+// mixed); inner products always accumulate in double.
+template <typename Z>
+struct LzBufs {
+  const float* A;
+  size_t pitch;
+  int n;
+  size_t vstride;
+  Z* z0;          // z of even steps
+  Z* z1;          // z of odd steps (holds q_0 before step 0)
+  double* part;   // [nb1] q.z partials of K1's blocks
+  double* part2;  // [nb2] |z|^2 partials of K2's blocks
+  Z* alpha;       // [kMaxIter + 2]
+  Z* beta;
+  int nb1, nb2;
+  LzState* st;
+};
+
+template <typename Z>
+__global__ __launch_bounds__(kK1) void k_lz1(LzBufs<Z> b, StencilOffsets so) {
+  __shared__ double red[kK1 / kWave];
+  const int k = b.st->k;
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.st->k2 = k;
+  float* V = b.st->V;
+  const Z* zp = (k & 1) ? b.z0 : b.z1;
+  Z* zc = (k & 1) ? b.z1 : b.z0;
+  Z inv = 1, nbeta = 0;
+  if (k > 0) {
+    const Z beta = (Z)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.beta[k - 1] = beta;
+    inv = (Z)1 / beta;
+    nbeta = -beta;
+  }
+  // q_k[c] = z_{k-1}[c] / beta_{k-1} rounded to FP32: the value the basis
+  // stores (step 0 reads q_0 from z1 with beta 1)
+  const int row = blockIdx.x * kK1 + threadIdx.x;
+  double dot = 0;
+  if (row < b.n) {
+    const float qv = (float)(zp[row] * inv);
+    V[(size_t)k * b.vstride + row] = qv;
+    float a[kStencilArea], x[kStencilArea];
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) a[d] = b.A[(size_t)d * b.pitch + row];
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) {
+      const int col = row + so.off[d];
+      const bool in = (unsigned)col < (unsigned)b.n;
+      const float xv = (float)(zp[in ? col : 0] * inv);
+      x[d] = in ? xv : 0.0f;  // fma(a, 0, acc) == acc: the reference's bounds test
     }
-    for (int j = 0; j < 4; j++) {
-      if (row + j < a.n) {
-        a.z[row + j] = zz[j];
-        const float qj = xs[row - r0 + so.halo + j];
-        dot = dot + (T)qj * (T)zz[j];
-      }
+    Z zj;
+    if constexpr (sizeof(Z) == 4) {
+      float acc = 0.0f;  // diagonal order, as stencilMVM.cu:127-180
+#pragma unroll
+      for (int d = 0; d < kStencilArea; d++) acc = fmaf(a[d], x[d], acc);
+      zj = acc;
+    } else {
+      double acc = 0.0;  // FP64 accumulation of exact FP32 products
+#pragma unroll
+      for (int d = 0; d < kStencilArea; d++) acc = fma((double)a[d], (double)x[d], acc);
+      zj = acc;
     }
+    if (k > 0) zj = fma_t(nbeta, (Z)V[(size_t)(k - 1) * b.vstride + row], zj);
+    zc[row] = zj;
+    dot = (double)qv * (double)zj;  // exact product
   }
-  const T s = block_sum(dot, sh);
-  if (threadIdx.x == 0) ((T*)a.part)[blockIdx.x] = s;
+  const double s = block_sum<kK1>(dot, red);
+  if (threadIdx.x == 0) b.part[blockIdx.x] = s;
 }
 
-// step 2: alpha_k; z -= alpha_k q_k; partial |z|^2
-template <typename T>
-__global__ __launch_bounds__(kRedThreads) void k_lz_axpy(LzArgs a, int k) {
-  __shared__ T sh[kRedThreads];
-  const T alpha = sum_partials((const T*)a.part, a.nb, sh);
-  if (blockIdx.x == 0 && threadIdx.x == 0) ((T*)a.alpha)[k] = alpha;
-  const float na = -(float)alpha;
-  const float* q = a.V + (size_t)k * a.vstride;
-  T nrm = 0;
-  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < a.n; i += gridDim.x * kRedThreads) {
-    const float zi = fmaf(na, q[i], a.z[i]);
-    a.z[i] = zi;
-    nrm = nrm + (T)zi * (T)zi;
-  }
-  const T s = block_sum(nrm, sh);
-  if (threadIdx.x == 0) ((T*)a.part2)[blockIdx.x] = s;
-}
-
-// step 3: beta_k = |z|; q_{k+1} = z * (1 / beta_k)
-template <typename T>
-__global__ __launch_bounds__(kRedThreads) void k_lz_scale(LzArgs a, int k, int nb2) {
-  __shared__ T sh[kRedThreads];
-  const T beta = sqrt(sum_partials((const T*)a.part2, nb2, sh));
-  if (blockIdx.x == 0 && threadIdx.x == 0) ((T*)a.beta)[k] = beta;
-  const float inv = 1 / (float)beta;
-  float* qn = a.V + (size_t)(k + 1) * a.vstride;
-  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < a.n; i += gridDim.x * kRedThreads)
-    qn[i] = a.z[i] * inv;
-}
-
-// ---- full reorthogonalisation (mixed): c = V[0..k]^T z (FP64), z -= V c
-__global__ __launch_bounds__(kRedThreads) void k_ro_dots(int n, const float* __restrict__ V, size_t vstride,
-                                                         int nv, const float* __restrict__ z,
-                                                         double* __restrict__ part, int nchunk) {
-  // block (chunk c, vector j): partial dot of vector j with z over chunk c
-  __shared__ double sh[kRedThreads];
-  const int j = blockIdx.y, c = blockIdx.x;
-  const int len = (n + nchunk - 1) / nchunk, i0 = c * len, i1 = min(n, i0 + len);
-  const float* v = V + (size_t)j * vstride;
-  double s = 0;
-  for (int i = i0 + threadIdx.x; i < i1; i += kRedThreads) s += (double)v[i] * (double)z[i];
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) part[(size_t)j * nchunk + c] = s;
-  (void)nv;
-}
-
-__global__ __launch_bounds__(kRedThreads) void k_ro_reduce(const double* __restrict__ part, int nchunk,
-                                                           double* __restrict__ coef, double* alpha_acc, int k) {
-  __shared__ double sh[kRedThreads];
-  const int j = blockIdx.x;
-  const double s = sum_partials(part + (size_t)j * nchunk, nchunk, sh);
-  if (threadIdx.x == 0) {
-    coef[j] = s;
-    if (j == k && alpha_acc) alpha_acc[k] += s;
-  }
-}
-
-__global__ __launch_bounds__(kRedThreads) void k_ro_update(int n, const float* __restrict__ V, size_t vstride,
-                                                           int nv, const double* __restrict__ coef,
-                                                           float* __restrict__ z, double* __restrict__ part2) {
-  __shared__ double sh[kRedThreads];
+template <typename Z>
+__global__ __launch_bounds__(kK2) void k_lz2(LzBufs<Z> b) {
+  __shared__ double red[kK2 / kWave];
+  const int k = b.st->k2;
+  const float* qk = b.st->V + (size_t)k * b.vstride;
+  Z* zc = (k & 1) ? b.z1 : b.z0;
+  const Z alpha = (Z)sum_partials<kK2>(b.part, b.nb1, red);
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.alpha[k] = alpha;
   double nrm = 0;
-  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += gridDim.x * kRedThreads) {
-    double acc = (double)z[i];
-    for (int j = 0; j < nv; j++) acc -= coef[j] * (double)V[(size_t)j * vstride + i];
-    const float zi = (float)acc;
-    z[i] = zi;
-    nrm += (double)zi * (double)zi;
+  const int base = blockIdx.x * kK2 * kK2Rows + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kK2Rows; j++) {
+    const int i = base + j * kK2;
+    if (i < b.n) {
+      const Z zj = fma_t(-alpha, (Z)qk[i], zc[i]);
+      zc[i] = zj;
+      nrm = fma((double)zj, (double)zj, nrm);
+    }
   }
-  nrm = block_sum(nrm, sh);
-  if (threadIdx.x == 0 && part2) part2[blockIdx.x] = nrm;
+  const double s = block_sum<kK2>(nrm, red);
+  if (threadIdx.x == 0) {
+    b.part2[blockIdx.x] = s;
+    // K2's blocks read k from k2, so block 0 may advance k for the next K1
+    if (blockIdx.x == 0) b.st->k = k + 1;
+  }
 }
 
-// ---- Ritz vectors, scaling, min-max normalisation
-__global__ __launch_bounds__(256) void k_ritz(int n, const float* __restrict__ V, size_t vstride, int m,
-                                              const float* __restrict__ S, int nvec, float* __restrict__ out) {
+// beta of the step just finished (needed by the check's residuals).
+template <typename Z>
+__global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs<Z> b) {
+  __shared__ double red[kK1 / kWave];
+  const int k = b.st->k - 1;
+  const Z beta = (Z)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
+  if (threadIdx.x == 0) {
+    b.beta[k] = beta;
+    b.st->m = k + 1;
+  }
+}
+
+// FP64 copies of alpha and beta and the squared off-diagonal.
+template <typename Z>
+__global__ void k_widen(const Z* __restrict__ al, const Z* __restrict__ be, double* __restrict__ ad,
+                        double* __restrict__ bd, double* __restrict__ b2, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    ad[i] = (double)al[i];
+    const double bv = (double)be[i];
+    bd[i] = bv;
+    b2[i] = bv * bv;
+  }
+}
+
+// ---------------------------------------------------------------- check
+// Block (idx, which): eigenvalue idx+1 of T (which 0, order m) or of T-hat
+// (which 1, order m-1: alpha[1..], beta[1..]) by multisection with kBisect
+// Sturm counts per round, to the relative accuracy of LAPACK dstebz with
+// abstol 0.
+__global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict__ alpha,
+                                                        const double* __restrict__ beta,
+                                                        const double* __restrict__ b2, const LzState* st,
+                                                        double* __restrict__ wout, int ncheck) {
+  __shared__ unsigned long long masks[kBisect / kWave];
+  const int m = st->m;
+  const int which = blockIdx.y, idx = blockIdx.x;
+  const int n = which ? m - 1 : m;
+  if (idx >= n) return;
+  const double* a = alpha + which;
+  const tri::Bounds g = tri::gershgorin(n, a, beta + which);
+  double lo = g.lo, hi = g.hi;
+  const int kk = idx + 1;
+  for (int round = 0; round < 20; round++) {
+    if (hi - lo <= 4.4e-16 * fmax(fabs(lo), fabs(hi)) + g.pivmin) break;
+    const double step = (hi - lo) / (double)(kBisect + 1);
+    const int c = tri::sturm_poly(n, a, b2 + which, lo + step * (threadIdx.x + 1));
+    // the first shift with at least kk values below it bounds lambda_kk above
+    const unsigned long long ge = __ballot(c >= kk);
+    if ((threadIdx.x & (kWave - 1)) == 0) masks[threadIdx.x / kWave] = ge;
+    __syncthreads();
+    int first = -1;
+#pragma unroll
+    for (int w = 0; w < kBisect / kWave; w++)
+      if (first < 0 && masks[w]) first = w * kWave + __ffsll((unsigned long long)masks[w]) - 1;
+    __syncthreads();
+    if (first >= 0) {
+      const double nhi = lo + step * (first + 1);
+      lo = first == 0 ? lo : lo + step * first;
+      hi = nhi;
+    } else {
+      lo = lo + step * kBisect;
+    }
+  }
+  if (threadIdx.x == 0) wout[which * ncheck + idx] = 0.5 * (lo + hi);
+}
+
+// Screening (lanczos.cu:279-384), Ritz vectors of T (lane j: value j) and
+// the residual test (lanczos.cu:114-140).
+__global__ __launch_bounds__(kWave) void k_tri_finish(const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta, LzState* st,
+                                                      const double* __restrict__ w, int ncheck, int nvec,
+                                                      double tol, int mixed, double* __restrict__ S,
+                                                      double* __restrict__ work, int* __restrict__ iwork) {
+  __shared__ double acc[kCullumMax];
+  __shared__ unsigned char dup[kCullumMax];
+  __shared__ double vals[kMaxEig];
+  __shared__ int ns_sh;
+  const int m = st->m;
+  if (threadIdx.x == 0) {
+    const double* wr = w;           // Ritz values of T, ascending
+    const double* wc = w + ncheck;  // of T-hat
+    const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
+    int na = 0;
+    for (int j = 0; j < ec; j++) {  // lump values closer than 1e-5
+      if (j > 0 && wr[j] - wr[j - 1] < 1e-5) {
+        dup[na - 1] = 1;
+        continue;
+      }
+      acc[na] = wr[j];
+      dup[na] = 0;
+      na++;
+    }
+    int ns = 0;  // drop single values that are also eigenvalues of T-hat
+    for (int j = 0; j < na && ns < nvec; j++) {
+      bool accept = true;
+      if (!dup[j])
+        for (int c = 0; c < ech; c++)
+          if (fabs((acc[j] - wc[c]) / acc[j]) <= 1e-10) accept = false;
+      if (accept) vals[ns++] = acc[j];
+    }
+    ns_sh = ns;
+    st->nscreened = ns;
+  }
+  __syncthreads();
+  if (ns_sh < nvec) {
+    if (threadIdx.x == 0) {
+      st->fail = 1;
+      st->converged = 0;
+    }
+    return;
+  }
+  const int j = threadIdx.x;
+  bool bad = false, good = true;
+  if (j < nvec) {
+    double* sj = S + (size_t)j * m;
+    bad = tri::invit(m, alpha, beta, vals[j], sj, work + j, iwork + j, kInvitLanes, kInvitIters) != 0;
+    const double bm = beta[m - 1];
+    const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
+    st->res[j] = r;
+    st->vals[j] = vals[j];
+    good = !bad && !(r > tol);
+  }
+  const unsigned long long anybad = __ballot(bad), notgood = __ballot(!good);
+  if (j == 0) {
+    st->fail = anybad ? 2 : 0;
+    st->converged = notgood == 0;
+  }
+}
+
+// ---------------------------------------------------------------- finish
+// u_j = V s_j (lanczos.cu:249-270), accumulated in FP64 in step order; the
+// plain path multiplies by S rounded to FP32 (the reference keeps the Ritz
+// coefficients in FP32).
+template <int NV, bool S32>
+__global__ __launch_bounds__(256) void k_ritz(int n, const LzState* st, size_t vstride, int m,
+                                              const double* __restrict__ S, float* __restrict__ out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  float acc[32];
-  for (int j = 0; j < nvec; j++) acc[j] = 0.f;
+  const float* V = st->V;
+  double acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) acc[j] = 0;
   for (int k = 0; k < m; k++) {
-    const float v = V[(size_t)k * vstride + p];
-    for (int j = 0; j < nvec; j++) acc[j] = fmaf(v, S[(size_t)j * m + k], acc[j]);
+    const double v = (double)V[(size_t)k * vstride + p];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+      const double sj = S32 ? (double)(float)S[(size_t)j * m + k] : S[(size_t)j * m + k];
+      acc[j] = fma(v, sj, acc[j]);
+    }
   }
-  for (int j = 0; j < nvec; j++) out[(size_t)j * n + p] = acc[j];
+#pragma unroll
+  for (int j = 0; j < NV; j++) out[(size_t)j * n + p] = (float)acc[j];
 }
 
 // stencilMVM.cu:182-198 scaleEigByD
@@ -194,18 +361,9 @@ __global__ void k_scale_rs(int n, int nvec, const float* __restrict__ rs, float*
   for (int j = 0; j < nvec; j++) v[(size_t)j * n + p] = v[(size_t)j * n + p] * rs[p];
 }
 
-// lanczos.cu:735-885 FindMaxMinPerBlock/Grid (min and max are exact, so any
-// reduction order gives the same result).
-__global__ __launch_bounds__(256) void k_minmax(int n, const float* __restrict__ v, float* __restrict__ mm,
-                                                int nblk) {
-  __shared__ float smin[256], smax[256];
-  const int j = blockIdx.y;
-  const float* x = v + (size_t)j * n;
-  float lo = INFINITY, hi = -INFINITY;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    lo = fminf(lo, x[i]);
-    hi = fmaxf(hi, x[i]);
-  }
+// lanczos.cu:735-885 per-vector minimum and maximum (exact in any order):
+// stage 1 per block, stage 2 one block per vector.
+__device__ __forceinline__ void block_minmax(float& lo, float& hi, float* smin, float* smax) {
   smin[threadIdx.x] = lo;
   smax[threadIdx.x] = hi;
   __syncthreads();
@@ -216,22 +374,48 @@ __global__ __launch_bounds__(256) void k_minmax(int n, const float* __restrict__
     }
     __syncthreads();
   }
+  lo = smin[0];
+  hi = smax[0];
+}
+
+__global__ __launch_bounds__(256) void k_minmax1(int n, const float* __restrict__ v, float* __restrict__ mm,
+                                                 int nblk) {
+  __shared__ float smin[256], smax[256];
+  const int j = blockIdx.y;
+  const float* x = v + (size_t)j * n;
+  float lo = INFINITY, hi = -INFINITY;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    lo = fminf(lo, x[i]);
+    hi = fmaxf(hi, x[i]);
+  }
+  block_minmax(lo, hi, smin, smax);
   if (threadIdx.x == 0) {
-    mm[((size_t)j * nblk + blockIdx.x) * 2] = smin[0];
-    mm[((size_t)j * nblk + blockIdx.x) * 2 + 1] = smax[0];
+    mm[((size_t)j * nblk + blockIdx.x) * 2] = lo;
+    mm[((size_t)j * nblk + blockIdx.x) * 2 + 1] = hi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_minmax2(const float* __restrict__ mm, int nblk, float* __restrict__ fin) {
+  __shared__ float smin[256], smax[256];
+  const int j = blockIdx.x;
+  float lo = INFINITY, hi = -INFINITY;
+  for (int b = threadIdx.x; b < nblk; b += 256) {
+    lo = fminf(lo, mm[((size_t)j * nblk + b) * 2]);
+    hi = fmaxf(hi, mm[((size_t)j * nblk + b) * 2 + 1]);
+  }
+  block_minmax(lo, hi, smin, smax);
+  if (threadIdx.x == 0) {
+    fin[2 * j] = lo;
+    fin[2 * j + 1] = hi;
   }
 }
 
 // lanczos.cu:887-901 NormalizationDev (vectors 1..nvec-1)
-__global__ void k_normalize(int n, int nvec, const float* __restrict__ mm, int nblk, float* __restrict__ v) {
+__global__ void k_normalize(int n, int nvec, const float* __restrict__ fin, float* __restrict__ v) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   for (int j = 1; j < nvec; j++) {
-    float lo = mm[((size_t)j * nblk) * 2], hi = mm[((size_t)j * nblk) * 2 + 1];
-    for (int b = 1; b < nblk; b++) {
-      lo = fminf(lo, mm[((size_t)j * nblk + b) * 2]);
-      hi = fmaxf(hi, mm[((size_t)j * nblk + b) * 2 + 1]);
-    }
+    const float lo = fin[2 * j], hi = fin[2 * j + 1];
     if ((hi - lo) > 1e-4f) {
       float* x = v + (size_t)j * n + p;
       *x = (*x - lo) / (hi - lo);
@@ -239,244 +423,245 @@ __global__ void k_normalize(int n, int nvec, const float* __restrict__ mm, int n
   }
 }
 
-__global__ void k_fill(float* p, int n, float v) {
+template <typename Z>
+__global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-// --------------------------------------------------------------- host side
-
-// lanczos.cu:279-384 CullumDevice (FP64): Ritz values of T_{i+1} and of T
-// without its first row/column; lump values closer than 1e-5, drop
-// non-duplicates that are also eigenvalues of the reduced matrix (relative
-// distance <= 1e-10); Ritz vectors of the first nvec survivors.
-bool cullum(int i, const std::vector<double>& alpha, const std::vector<double>& beta, int eigCheck, int nvec,
-            std::vector<double>& vals, std::vector<double>& S) {
-  std::vector<double> a(i + 1), b(i + 1), wc, wr;
-  const int tn_hat = i;
-  for (int j = 0; j < tn_hat; j++) a[j] = alpha[j + 1];
-  for (int j = 0; j < tn_hat - 1; j++) b[j] = beta[j + 1];
-  const int ech = std::min(eigCheck, tn_hat);
-  tri::Bounds g = tri::gershgorin(tn_hat, a.data(), b.data());
-  for (int k = 1; k <= ech; k++) wc.push_back(tri::kth(tn_hat, a.data(), b.data(), k, g));
-  const int tn = i + 1;
-  for (int j = 0; j < tn; j++) a[j] = alpha[j];
-  for (int j = 0; j < tn - 1; j++) b[j] = beta[j];
-  g = tri::gershgorin(tn, a.data(), b.data());
-  const int ec = std::min(eigCheck, tn);
-  for (int k = 1; k <= ec; k++) wr.push_back(tri::kth(tn, a.data(), b.data(), k, g));
-  std::vector<double> acc;
-  std::vector<char> dup;
-  for (int j = 0; j < ec; j++) {
-    if (j > 0 && wr[j] - wr[j - 1] < 1e-5) {
-      dup.back() = 1;
-      continue;
-    }
-    acc.push_back(wr[j]);
-    dup.push_back(0);
+  if (i == 0) {
+    st->k = 0;
+    st->k2 = 0;
+    st->V = V;
+    st->converged = 0;
+    st->nscreened = 0;
+    st->fail = 0;
+    st->m = 0;
   }
-  std::vector<double> scr;
-  for (size_t j = 0; j < acc.size(); j++) {
-    bool ok = true;
-    if (!dup[j])
-      for (double c : wc)
-        if (std::fabs((acc[j] - c) / acc[j]) <= 1e-10) ok = false;
-    if (ok) scr.push_back(acc[j]);
-  }
-  if ((int)scr.size() < nvec) return false;
-  vals.assign(scr.begin(), scr.begin() + nvec);
-  S.assign((size_t)nvec * tn, 0.0);
-  std::vector<double> work(5 * (size_t)tn);
-  std::vector<int> piv(tn);
-  int cstart = 0;
-  for (int j = 0; j < nvec; j++) {
-    if (j > 0 && vals[j] - vals[j - 1] > 1e-3 * g.tnorm) cstart = j;
-    if (tri::invit(tn, a.data(), b.data(), vals[j], &S[(size_t)j * tn], &S[(size_t)cstart * tn], j - cstart,
-                   work.data(), piv.data()))
-      return false;
-  }
-  return true;
-}
-
-// Eigenpairs of T_m (no screening; the mixed path keeps the basis orthogonal).
-void ritz_plain(int m, const std::vector<double>& alpha, const std::vector<double>& beta, int nvec,
-                std::vector<double>& vals, std::vector<double>& S) {
-  std::vector<double> a(alpha.begin(), alpha.begin() + m), b(beta.begin(), beta.begin() + std::max(m - 1, 0));
-  b.resize(std::max(m, 1));
-  const tri::Bounds g = tri::gershgorin(m, a.data(), b.data());
-  vals.resize(nvec);
-  for (int k = 1; k <= nvec; k++) vals[k - 1] = tri::kth(m, a.data(), b.data(), k, g);
-  S.assign((size_t)nvec * m, 0.0);
-  std::vector<double> work(5 * (size_t)m);
-  std::vector<int> piv(m);
-  int cstart = 0;
-  for (int j = 0; j < nvec; j++) {
-    if (j > 0 && vals[j] - vals[j - 1] > 1e-3 * g.tnorm) cstart = j;
-    tri::invit(m, a.data(), b.data(), vals[j], &S[(size_t)j * m], &S[(size_t)cstart * m], j - cstart,
-               work.data(), piv.data());
+  if (i < n) {
+    V[i] = q0;
+    z1[i] = (Z)q0;
   }
 }
 
-struct Workspace {
-  float* V = nullptr;
-  size_t cap = 0;  // vectors
-  float* z = nullptr;
-  float* rs = nullptr;
-  void* part = nullptr;
-  void* part2 = nullptr;
-  void* alpha = nullptr;
-  void* beta = nullptr;
-  double* coef = nullptr;
-  double* ropart = nullptr;
-  float* S = nullptr;
-  float* mm = nullptr;
+__global__ void k_set_basis(LzState* st, float* V) { st->V = V; }
+
+// ---------------------------------------------------------------- host
+// Captured check periods, per context; the kernels' arguments are baked into
+// the graph, so the key holds every pointer and size they depend on.
+struct GraphKey {
+  const void* A;
+  const void* st;
+  int pitch, n, w, mixed, steps;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(A, st, pitch, n, w, mixed, steps) < std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps);
+  }
 };
+
+struct GraphCache {
+  hipStream_t cap = nullptr;
+  std::map<GraphKey, hipGraphExec_t> execs;
+  void clear() {
+    for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+    execs.clear();
+  }
+  ~GraphCache() {
+    clear();
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+std::map<const Ctx*, std::unique_ptr<GraphCache>>& graph_caches() {
+  static std::map<const Ctx*, std::unique_ptr<GraphCache>> g;
+  return g;
+}
+
+template <typename Z>
+void enqueue_steps(hipStream_t s, const LzBufs<Z>& b, const StencilOffsets& so, int steps) {
+  for (int i = 0; i < steps; i++) {
+    hipLaunchKernelGGL(k_lz1<Z>, dim3(b.nb1), dim3(kK1), 0, s, b, so);
+    hipLaunchKernelGGL(k_lz2<Z>, dim3(b.nb2), dim3(kK2), 0, s, b);
+  }
+}
+
+// `steps` Lanczos steps on ctx->stream through a cached HIP graph.
+template <typename Z>
+int run_steps(Ctx* ctx, const LzBufs<Z>& b, const StencilOffsets& so, int w, int steps) {
+  auto& gc = graph_caches()[ctx];
+  if (!gc) gc.reset(new GraphCache());
+  if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
+  const GraphKey key{b.A, b.st, (int)b.pitch, b.n, w, (int)(sizeof(Z) == 8), steps};
+  auto it = gc->execs.find(key);
+  if (it == gc->execs.end()) {
+    if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
+    DMSC_HIP(hipStreamBeginCapture(gc->cap, hipStreamCaptureModeThreadLocal));
+    enqueue_steps<Z>(gc->cap, b, so, steps);
+    const hipError_t le = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(gc->cap, &g);
+    DMSC_HIP(le);
+    DMSC_HIP(ce);
+    hipGraphExec_t exec;
+    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    DMSC_HIP(ie);
+    it = gc->execs.emplace(key, exec).first;
+  }
+  DMSC_HIP(hipGraphLaunch(it->second, ctx->stream));
+  return DMSC_OK;
+}
+
+template <typename Z>
+int launch_ritz(hipStream_t s, int nvec, int n, const LzState* st, size_t vstride, int m, const double* S,
+                float* out) {
+  const dim3 g(div_up(n, 256)), b(256);
+  constexpr bool S32 = sizeof(Z) == 4;
+  switch (nvec) {
+#define RITZ_CASE(NV) \
+  case NV: hipLaunchKernelGGL((k_ritz<NV, S32>), g, b, 0, s, n, st, vstride, m, S, out); break;
+    RITZ_CASE(1) RITZ_CASE(2) RITZ_CASE(3) RITZ_CASE(4) RITZ_CASE(5) RITZ_CASE(6) RITZ_CASE(7) RITZ_CASE(8)
+    RITZ_CASE(9) RITZ_CASE(10) RITZ_CASE(11) RITZ_CASE(12) RITZ_CASE(13) RITZ_CASE(14) RITZ_CASE(15)
+    RITZ_CASE(16) RITZ_CASE(17) RITZ_CASE(18) RITZ_CASE(19) RITZ_CASE(20) RITZ_CASE(21) RITZ_CASE(22)
+    RITZ_CASE(23) RITZ_CASE(24) RITZ_CASE(25)
+#undef RITZ_CASE
+    default: return fail(DMSC_ERR_ARG, "nvec %d out of range", nvec);
+  }
+  DMSC_HIP(hipGetLastError());
+  return DMSC_OK;
+}
+
+template <typename Z>
+int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool mixed, float* h_eigvals,
+          float* d_eigvecs, int* iterations) {
+  hipStream_t s = ctx->stream;
+  const int n = w * h;
+  const size_t vstride = (size_t)pitch_floats(n);
+  const int nb1 = div_up(n, kK1), nb2 = div_up(n, kK2 * kK2Rows);
+  const StencilOffsets so = make_stencil_offsets(w);
+  const int period = mixed ? kCheckMixed : kCheckIter;
+  const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
+  const int nblk = std::min(div_up(n, 256), 1024);
+
+  // ---- solver state and small vectors (arena 2); offsets depend on n only
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_st = carve(sizeof(LzState));
+  const size_t o_z0 = carve(sizeof(Z) * vstride), o_z1 = carve(sizeof(Z) * vstride);
+  const size_t o_rs = carve(sizeof(float) * vstride);
+  const size_t o_p = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb2);
+  const size_t o_al = carve(sizeof(Z) * (kMaxIter + 2)), o_be = carve(sizeof(Z) * (kMaxIter + 2));
+  const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
+  const size_t o_b2 = carve(sizeof(double) * (kMaxIter + 2));
+  const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);
+  const size_t o_S = carve(sizeof(double) * kMaxEig * (kMaxIter + 2));
+  const size_t o_work = carve(sizeof(double) * 5 * kInvitLanes * (kMaxIter + 2));
+  const size_t o_iwork = carve(sizeof(int) * kInvitLanes * (kMaxIter + 2));
+  const size_t o_mm = carve(sizeof(float) * 2 * kMaxEig * nblk), o_fin = carve(sizeof(float) * 2 * kMaxEig);
+  void* base;
+  DMSC_TRY(ctx->scratch(2, off, &base));
+  char* B = (char*)base;
+  LzState* st = (LzState*)(B + o_st);
+  float* rs = (float*)(B + o_rs);
+  double* ad = (double*)(B + o_ad);
+  double* bd = (double*)(B + o_bd);
+  double* b2 = (double*)(B + o_b2);
+  double* wv = (double*)(B + o_w);
+  double* S = (double*)(B + o_S);
+
+  // ---- basis (arena 3), grown by whole check periods
+  void* vb;
+  DMSC_TRY(ctx->scratch(3, vstride * sizeof(float) * (size_t)(period + 2), &vb));
+  size_t cap = ctx->arena_bytes[3] / (vstride * sizeof(float));
+  float* V = (float*)vb;
+
+  LzBufs<Z> bf;
+  bf.A = A;
+  bf.pitch = (size_t)pitch;
+  bf.n = n;
+  bf.vstride = vstride;
+  bf.z0 = (Z*)(B + o_z0);
+  bf.z1 = (Z*)(B + o_z1);
+  bf.part = (double*)(B + o_p);
+  bf.part2 = (double*)(B + o_p2);
+  bf.alpha = (Z*)(B + o_al);
+  bf.beta = (Z*)(B + o_be);
+  bf.nb1 = nb1;
+  bf.nb2 = nb2;
+  bf.st = st;
+
+  DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
+  const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
+  hipLaunchKernelGGL(k_init<Z>, dim3(div_up(n, 256)), dim3(256), 0, s, st, V, bf.z1, q0, n);
+  DMSC_HIP(hipGetLastError());
+
+  int eigCheck = nvec + kCullumExtra;
+  int done = 0;
+  LzState hs{};
+  bool converged = false;
+  while (done < kMaxIter) {
+    const int steps = std::min(period, kMaxIter - done);
+    if ((size_t)done + steps + 2 > cap) {  // grow the basis, keeping q_0..q_{done-1}
+      const size_t ncap = std::min<size_t>(std::max(cap * 2, (size_t)done + steps + 2), kMaxIter + 2);
+      void* nbuf;
+      DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * ncap));
+      DMSC_HIP(hipMemcpyAsync(nbuf, V, vstride * sizeof(float) * done, hipMemcpyDeviceToDevice, s));
+      DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * ncap));  // synchronises before freeing
+      V = (float*)nbuf;
+      cap = ncap;
+      hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
+    }
+    DMSC_TRY(run_steps<Z>(ctx, bf, so, w, steps));
+    done += steps;
+    hipLaunchKernelGGL(k_lz_beta<Z>, dim3(1), dim3(kK1), 0, s, bf);
+    hipLaunchKernelGGL(k_widen<Z>, dim3(div_up(done, 256)), dim3(256), 0, s, bf.alpha, bf.beta, ad, bd, b2, done);
+    for (;;) {
+      const int ncheck = std::min(eigCheck, done);
+      hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), 0, s, ad, bd, b2, st, wv, ncheck);
+      hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(kWave), 0, s, ad, bd, st, wv, ncheck, nvec, stop_tol,
+                         mixed ? 1 : 0, S, (double*)(B + o_work), (int*)(B + o_iwork));
+      DMSC_HIP(hipGetLastError());
+      DMSC_HIP(hipMemcpyAsync(&hs, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
+      DMSC_HIP(hipStreamSynchronize(s));
+      if (hs.fail != 1 || eigCheck >= done) break;
+      eigCheck += kCullumExtra;  // lanczos.cu:528-532: widen the screened window
+      if (eigCheck > kCullumMax) return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
+    }
+    if (hs.fail == 2) return fail(DMSC_ERR_INTERNAL, "inverse iteration failed on the Lanczos tridiagonal");
+    if (hs.fail == 0 && hs.converged) {
+      converged = true;
+      break;
+    }
+  }
+  if (!converged) return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", kMaxIter);
+  const int m = hs.m;
+
+  // Ritz vectors, D^-1/2 scaling (stencilMVM.cu:182-198), min-max normalisation
+  DMSC_TRY(launch_ritz<Z>(s, nvec, n, st, vstride, m, S, d_eigvecs));
+  const dim3 g(div_up(n, 256));
+  hipLaunchKernelGGL(k_scale_rs, g, dim3(256), 0, s, n, nvec, rs, d_eigvecs);
+  float* mm = (float*)(B + o_mm);
+  float* fin = (float*)(B + o_fin);
+  hipLaunchKernelGGL(k_minmax1, dim3(nblk, nvec), dim3(256), 0, s, n, d_eigvecs, mm, nblk);
+  hipLaunchKernelGGL(k_minmax2, dim3(nvec), dim3(256), 0, s, mm, nblk, fin);
+  hipLaunchKernelGGL(k_normalize, g, dim3(256), 0, s, n, nvec, fin, d_eigvecs);
+  DMSC_HIP(hipGetLastError());
+  for (int j = 0; j < nvec; j++) h_eigvals[j] = (float)hs.vals[j];
+  if (iterations) *iterations = m;
+  return DMSC_OK;
+}
 
 }  // namespace
 
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,
                    float* h_eigvals, float* d_eigvecs, int* iterations) {
-  hipStream_t s = ctx->stream;
-  const int n = w * h;
-  const bool mixed = precision == DMSC_PREC_MIXED;
-  const size_t vstride = (size_t)pitch_floats(n);
-  const int nb = div_up(n, kSpmvRows);
-  const int nb2 = std::min(div_up(n, kRedThreads), 1024);
-  const int nchunk = std::min(64, std::max(1, div_up(n, 8192)));
-  StencilOffsets so = make_stencil_offsets(w);
-  const size_t lds = sizeof(float) * (kSpmvRows + 2 * so.halo);
-  const size_t tsz = mixed ? sizeof(double) : sizeof(float);
-  const int cap0 = mixed ? 512 : kCheckIter + 2;
-
-  // scratch: z, rs, partials, alpha/beta, coef, ropart, S, minmax
-  Workspace ws;
-  const size_t small = vstride * 2 * sizeof(float) + tsz * (size_t)(nb + nb2 + 2 * (kMaxIter + 2)) +
-                       sizeof(double) * ((size_t)(kMaxIter + 2) * (nchunk + 1)) + sizeof(float) * 32 * (kMaxIter + 2) +
-                       sizeof(float) * 2 * 32 * 1024 + 64 * 16;
-  void* sm;
-  DMSC_TRY(ctx->scratch(2, small, &sm));
-  char* cur = (char*)sm;
-  auto take = [&](size_t bytes) {
-    void* p = cur;
-    cur += (bytes + 255) & ~(size_t)255;
-    return p;
-  };
-  ws.z = (float*)take(vstride * sizeof(float));
-  ws.rs = (float*)take(vstride * sizeof(float));
-  ws.part = take(tsz * nb);
-  ws.part2 = take(tsz * nb2);
-  ws.alpha = take(tsz * (kMaxIter + 2));
-  ws.beta = take(tsz * (kMaxIter + 2));
-  ws.coef = (double*)take(sizeof(double) * (kMaxIter + 2));
-  ws.ropart = (double*)take(sizeof(double) * (size_t)(kMaxIter + 2) * nchunk);
-  ws.S = (float*)take(sizeof(float) * 32 * (kMaxIter + 2));
-  ws.mm = (float*)take(sizeof(float) * 2 * 32 * 1024);
-  // basis (grown on demand, kept on the context)
-  size_t cap = cap0;
-  void* vb;
-  DMSC_TRY(ctx->scratch(3, vstride * sizeof(float) * cap, &vb));
-  ws.V = (float*)vb;
-
-  DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, ws.rs));
-  const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
-  hipLaunchKernelGGL(k_fill, dim3(div_up(n, 256)), dim3(256), 0, s, ws.V, n, q0);
-  if (mixed) DMSC_HIP(hipMemsetAsync(ws.alpha, 0, tsz * (kMaxIter + 2), s));
-
-  LzArgs la{n, (size_t)pitch, A, ws.V, vstride, ws.z, ws.part, ws.part2, ws.alpha, ws.beta, nb};
-  std::vector<double> alpha, beta, vals, S;
-  int eigCheck = nvec + 20;
-  int m = 0;
-  bool converged = false;
-  const int period = mixed ? 8 : kCheckIter;
-
-  for (int k = 0; k < kMaxIter && !converged; k++) {
-    if ((size_t)k + 2 > cap) {  // grow the basis, keeping its contents
-      const size_t ncap = std::min<size_t>(cap * 2, kMaxIter + 2);
-      void* nbuf;
-      DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * ncap));
-      DMSC_HIP(hipMemcpyAsync(nbuf, ws.V, vstride * sizeof(float) * cap, hipMemcpyDeviceToDevice, s));
-      DMSC_HIP(hipStreamSynchronize(s));
-      DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * ncap));
-      ws.V = (float*)nbuf;
-      la.V = ws.V;
-      cap = ncap;
-    }
-    if (mixed) {
-      hipLaunchKernelGGL(k_lz_spmv<double>, dim3(nb), dim3(kSpmvThreads), lds, s, la, so, k);
-      hipLaunchKernelGGL(k_lz_axpy<double>, dim3(nb2), dim3(kRedThreads), 0, s, la, k);
-      // two passes of classical Gram-Schmidt against q_0..q_k
-      for (int pass = 0; pass < 2; pass++) {
-        hipLaunchKernelGGL(k_ro_dots, dim3(nchunk, k + 1), dim3(kRedThreads), 0, s, n, ws.V, vstride, k + 1,
-                           ws.z, ws.ropart, nchunk);
-        hipLaunchKernelGGL(k_ro_reduce, dim3(k + 1), dim3(kRedThreads), 0, s, ws.ropart, nchunk, ws.coef,
-                           (double*)ws.alpha, k);
-        hipLaunchKernelGGL(k_ro_update, dim3(nb2), dim3(kRedThreads), 0, s, n, ws.V, vstride, k + 1, ws.coef,
-                           ws.z, (double*)ws.part2);
-      }
-      hipLaunchKernelGGL(k_lz_scale<double>, dim3(nb2), dim3(kRedThreads), 0, s, la, k, nb2);
-    } else {
-      hipLaunchKernelGGL(k_lz_spmv<float>, dim3(nb), dim3(kSpmvThreads), lds, s, la, so, k);
-      hipLaunchKernelGGL(k_lz_axpy<float>, dim3(nb2), dim3(kRedThreads), 0, s, la, k);
-      hipLaunchKernelGGL(k_lz_scale<float>, dim3(nb2), dim3(kRedThreads), 0, s, la, k, nb2);
-    }
-    DMSC_HIP(hipGetLastError());
-    const bool check = mixed ? ((k + 1) % period == 0 && k + 1 >= nvec + 2) || k == kMaxIter - 1
-                             : ((k + 1) % kCheckIter == 0) || k == kMaxIter - 1;
-    if (!check) continue;
-    // fetch alpha/beta (FP32 path: float, mixed: double)
-    alpha.assign(k + 1, 0.0);
-    beta.assign(k + 1, 0.0);
-    if (mixed) {
-      DMSC_HIP(hipMemcpyAsync(alpha.data(), ws.alpha, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      DMSC_HIP(hipMemcpyAsync(beta.data(), ws.beta, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      DMSC_HIP(hipStreamSynchronize(s));
-    } else {
-      std::vector<float> fa(k + 1), fb(k + 1);
-      DMSC_HIP(hipMemcpyAsync(fa.data(), ws.alpha, sizeof(float) * (k + 1), hipMemcpyDeviceToHost, s));
-      DMSC_HIP(hipMemcpyAsync(fb.data(), ws.beta, sizeof(float) * (k + 1), hipMemcpyDeviceToHost, s));
-      DMSC_HIP(hipStreamSynchronize(s));
-      for (int j = 0; j <= k; j++) { alpha[j] = fa[j]; beta[j] = fb[j]; }
-    }
-    m = k + 1;
-    bool ok;
-    if (mixed) {
-      ritz_plain(m, alpha, beta, nvec, vals, S);
-      ok = true;
-    } else {
-      ok = false;
-      while (!ok) {
-        ok = cullum(k, alpha, beta, eigCheck, nvec, vals, S);
-        if (!ok) {
-          eigCheck += 20;  // lanczos.cu:528-532
-          if (eigCheck > 1000) return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
-        }
-      }
-    }
-    // lanczos.cu:114-140 TestForConvergence
-    bool good = true;
-    for (int j = nvec - 1; j >= 0; j--) {
-      const double res = mixed ? std::fabs(beta[k] * S[(size_t)j * m + k])
-                               : std::fabs((float)beta[k] * (float)S[(size_t)j * m + k]);
-      if (res > tol) { good = false; break; }
-    }
-    converged = good;
-  }
-  if (!converged) return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", kMaxIter);
-
-  // Ritz vectors u_j = V s_j (lanczos.cu:249-270), D^-1/2 scaling, min-max
-  std::vector<float> Sf(S.size());
-  for (size_t i = 0; i < S.size(); i++) Sf[i] = (float)S[i];
-  DMSC_HIP(hipMemcpyAsync(ws.S, Sf.data(), sizeof(float) * Sf.size(), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_ritz, dim3(div_up(n, 256)), dim3(256), 0, s, n, ws.V, vstride, m, ws.S, nvec, d_eigvecs);
-  hipLaunchKernelGGL(k_scale_rs, dim3(div_up(n, 256)), dim3(256), 0, s, n, nvec, ws.rs, d_eigvecs);
-  const int nblk = std::min(div_up(n, 256), 1024);
-  hipLaunchKernelGGL(k_minmax, dim3(nblk, nvec), dim3(256), 0, s, n, d_eigvecs, ws.mm, nblk);
-  hipLaunchKernelGGL(k_normalize, dim3(div_up(n, 256)), dim3(256), 0, s, n, nvec, ws.mm, nblk, d_eigvecs);
-  DMSC_HIP(hipGetLastError());
-  for (int j = 0; j < nvec; j++) h_eigvals[j] = (float)vals[j];
-  if (iterations) *iterations = m;
-  DMSC_HIP(hipStreamSynchronize(s));
-  return DMSC_OK;
+  if (nvec < 1 || nvec > kMaxEig) return fail(DMSC_ERR_ARG, "nvec must be in [1, %d]", kMaxEig);
+  if (pitch % 4) return fail(DMSC_ERR_ARG, "matrix pitch must be a multiple of 4 floats");
+  if (precision == DMSC_PREC_MIXED)
+    return solve<double>(ctx, w, h, A, pitch, nvec, tol, true, h_eigvals, d_eigvecs, iterations);
+  return solve<float>(ctx, w, h, A, pitch, nvec, tol, false, h_eigvals, d_eigvecs, iterations);
 }
+
+void release_eigensolver(const Ctx* ctx) { graph_caches().erase(ctx); }
 
 }  // namespace dmsc

@@ -78,35 +78,38 @@ StencilOffsets make_stencil_offsets(int w) {
 
 namespace {
 
-// y = A x (stencilMVM.cu:127-180).
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv(int n, StencilOffsets so, const float* __restrict__ A,
-                                                       size_t pitch, const float* __restrict__ x,
-                                                       float* __restrict__ y) {
-  extern __shared__ float xs[];
-  const int r0 = blockIdx.x * kSpmvRows;
-  spmv_stage_x(xs, x, n, r0, so.halo, 1.0f);
-  __syncthreads();
-  const int row = r0 + threadIdx.x * 4;
+// y = A x (stencilMVM.cu:127-180): one wave per 64*RPL rows.
+template <int RPL>
+__global__ __launch_bounds__(64) void k_spmv(int n, StencilOffsets so, const float* __restrict__ A, size_t pitch,
+                                             const float* __restrict__ x, float* __restrict__ y) {
+  const int row = (blockIdx.x * 64 + threadIdx.x) * RPL;
   if (row >= n) return;
-  float4 acc;
-  spmv_rows4(acc, A + row, pitch, xs, row - r0 + so.halo, so);
-  if (row + 3 < n) {
-    *reinterpret_cast<float4*>(y + row) = acc;
-  } else {
-    const float v[4] = {acc.x, acc.y, acc.z, acc.w};
-    for (int k = 0; k < 4 && row + k < n; k++) y[row + k] = v[k];
-  }
+  float acc[RPL];
+  spmv_rows<RPL>(acc, A, pitch, row, n, so, [&](int c) { return x[c]; });
+#pragma unroll
+  for (int j = 0; j < RPL; j++)
+    if (row + j < n) y[row + j] = acc[j];
 }
 
 }  // namespace
+
+int spmv_rows_per_lane = 4;
 
 int launch_spmv(hipStream_t s, int w, int h, const float* A, int pitch, const float* x, float* y) {
   const int n = w * h;
   if (pitch % 4) return fail(DMSC_ERR_ARG, "spmv: matrix pitch must be a multiple of 4 floats");
   StencilOffsets so = make_stencil_offsets(w);
-  const size_t lds = sizeof(float) * (kSpmvRows + 2 * so.halo);
-  hipLaunchKernelGGL(k_spmv, dim3(div_up(n, kSpmvRows)), dim3(kSpmvThreads), lds, s, n, so, A, (size_t)pitch,
-                     x, y);
+  switch (spmv_rows_per_lane) {
+    case 1:
+      hipLaunchKernelGGL(k_spmv<1>, dim3(div_up(n, 64)), dim3(64), 0, s, n, so, A, (size_t)pitch, x, y);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_spmv<2>, dim3(div_up(n, 128)), dim3(64), 0, s, n, so, A, (size_t)pitch, x, y);
+      break;
+    default:
+      hipLaunchKernelGGL(k_spmv<4>, dim3(div_up(n, 256)), dim3(64), 0, s, n, so, A, (size_t)pitch, x, y);
+      break;
+  }
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
 }

@@ -24,6 +24,56 @@ __host__ __device__ inline int sturm(int n, const double* a, const double* b, do
   return cnt;
 }
 
+// Sturm count by the three-term recurrence of the leading principal minors,
+// p_i = (a_i - x) p_{i-1} - b_{i-1}^2 p_{i-2}: the number of sign changes in
+// p_0 = 1, p_1, ..., p_n equals the number of eigenvalues below x (a zero
+// p_i counts as a change, the convention of the pivot form above).  One
+// dependent multiply-add per row instead of a division; the pair is
+// rescaled by an exact power of two every 4 rows to stay in range.  b2 holds
+// the squared off-diagonal.
+__host__ __device__ inline int sturm_poly(int n, const double* a, const double* b2, double x) {
+  const double kBig = 0x1p+400, kSmall = 0x1p-400;
+  double pp = 1.0, p = a[0] - x;  // p_0, p_1
+  if (p == 0) p = -0x1p-1000;
+  int cnt = p < 0;
+  int i = 1;
+  for (; i + 4 <= n; i += 4) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double pn = fma(a[i + j] - x, p, -b2[i + j - 1] * pp);
+      if (pn == 0) pn = p < 0 ? 0x1p-1000 : -0x1p-1000;
+      cnt += (pn < 0) != (p < 0);
+      pp = p;
+      p = pn;
+    }
+    const double m = fmax(fabs(p), fabs(pp));
+    const double sc = m > kBig ? kSmall : (m < kSmall ? kBig : 1.0);
+    p *= sc;
+    pp *= sc;
+  }
+  for (; i < n; i++) {
+    double pn = fma(a[i] - x, p, -b2[i - 1] * pp);
+    if (pn == 0) pn = p < 0 ? 0x1p-1000 : -0x1p-1000;
+    cnt += (pn < 0) != (p < 0);
+    pp = p;
+    p = pn;
+  }
+  return cnt;
+}
+
+// 1/x: on the device v_rcp_f64 refined by two Newton steps (a few ulp,
+// enough for pivots and multipliers), on the host a division.
+__host__ __device__ inline double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+#else
+  return 1.0 / x;
+#endif
+}
+
 struct Bounds {
   double lo, hi, pivmin, tnorm;
 };
@@ -60,86 +110,106 @@ __host__ __device__ inline double kth(int n, const double* a, const double* b, i
   return 0.5 * (l + h);
 }
 
-// Inverse iteration for eigenvalue lam; work needs 5n doubles and n ints.
-// prev/np: vectors (stride n) to orthogonalise against (nearby eigenvalues).
-// Result normalised, largest-magnitude entry positive.
+// Inverse iteration (LU with partial pivoting of T - lam I, as LAPACK
+// dgttrf/dstein) for the eigenvalue lam, `iters` solves from a fixed
+// pseudo-random start.  work holds 5n doubles and piv n ints; element i of
+// each array sits at index i * ws, so the lanes of a wave can interleave
+// their arrays (ws = lane count) and access them coalesced.  No
+// orthogonalisation against neighbouring vectors: the callers pass values
+// at least 1e-5 apart (Cullum-Willoughby lumping), which inverse iteration
+// separates by a factor 1e-5 / (|T| eps) per solve.  The result is
+// normalised with its largest-magnitude entry positive.  Returns -1 on a
+// non-finite iterate.
 __host__ __device__ inline int invit(int n, const double* a, const double* b, double lam, double* v,
-                                     const double* prev, int np, double* work, int* piv) {
-  double *dl = work, *d = work + n, *du = work + 2 * n, *du2 = work + 3 * n, *x = work + 4 * n;
+                                     double* work, int* piv, int ws, int iters) {
+  double *dl = work, *d = work + (size_t)n * ws, *du = work + (size_t)2 * n * ws, *du2 = work + (size_t)3 * n * ws,
+         *x = work + (size_t)4 * n * ws;
+#define W_(arr, i) arr[(size_t)(i) * ws]
   double tnorm = 0;
   for (int i = 0; i < n; i++)
     tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0));
   const double eps = 2.2e-16 * fmax(tnorm, 1e-300);
-  for (int i = 0; i < n; i++) d[i] = a[i] - lam;
-  for (int i = 0; i < n - 1; i++) { dl[i] = b[i]; du[i] = b[i]; du2[i] = 0; }
-  for (int i = 0; i < n - 1; i++) {  // LU with partial pivoting (dgttrf)
-    if (fabs(d[i]) >= fabs(dl[i])) {
-      piv[i] = i;
-      if (d[i] == 0) d[i] = eps;
-      const double f = dl[i] / d[i];
-      dl[i] = f;
-      d[i + 1] -= f * du[i];
+  // factorisation (dgttrf), carrying the running diagonal entry di and the
+  // running upper entry ui (modified by a preceding row interchange)
+  double di = a[0] - lam, ui = n > 1 ? b[0] : 0.0;
+  for (int i = 0; i < n - 1; i++) {
+    const double li = b[i], dn = a[i + 1] - lam, un = i < n - 2 ? b[i + 1] : 0.0;
+    if (fabs(di) >= fabs(li)) {
+      W_(piv, i) = 0;
+      if (di == 0) di = eps;
+      const double f = li * recip(di);
+      W_(dl, i) = f;
+      W_(d, i) = di;
+      W_(du, i) = ui;
+      W_(du2, i) = 0;
+      di = dn - f * ui;
+      ui = un;
     } else {
-      piv[i] = i + 1;
-      const double f = d[i] / dl[i];
-      d[i] = dl[i];
-      dl[i] = f;
-      const double t = du[i];
-      du[i] = d[i + 1];
-      d[i + 1] = t - f * d[i + 1];
-      if (i < n - 2) {
-        du2[i] = du[i + 1];
-        du[i + 1] = -f * du[i + 1];
-      }
+      W_(piv, i) = 1;
+      const double f = di * recip(li);
+      W_(dl, i) = f;
+      W_(d, i) = li;
+      W_(du, i) = dn;
+      W_(du2, i) = un;
+      di = ui - f * dn;
+      ui = -f * un;
     }
   }
-  piv[n - 1] = n - 1;
-  for (int i = 0; i < n; i++)
-    if (fabs(d[i]) < eps) d[i] = d[i] < 0 ? -eps : eps;
+  W_(d, n - 1) = di;
+  W_(piv, n - 1) = 0;
+  for (int i = 0; i < n; i++) {
+    double t = W_(d, i);
+    if (fabs(t) < eps) t = t < 0 ? -eps : eps;
+    W_(d, i) = recip(t);  // reciprocal pivots
+  }
   unsigned long long s = 0x9E3779B97F4A7C15ULL;
   for (int i = 0; i < n; i++) {
     s ^= s << 13; s ^= s >> 7; s ^= s << 17;
-    x[i] = ((double)(s >> 11) / 9007199254740992.0) - 0.5;
+    W_(x, i) = ((double)(s >> 11) / 9007199254740992.0) - 0.5;
   }
-  for (int iter = 0; iter < 6; iter++) {
-    for (int j = 0; j < np; j++) {
-      const double* p = prev + (size_t)j * n;
-      double dot = 0;
-      for (int i = 0; i < n; i++) dot += p[i] * x[i];
-      for (int i = 0; i < n; i++) x[i] -= dot * p[i];
-    }
-    for (int i = 0; i < n - 1; i++) {
-      if (piv[i] == i) {
-        x[i + 1] -= dl[i] * x[i];
+  for (int iter = 0; iter < iters; iter++) {
+    double xi = W_(x, 0);
+#pragma unroll 4
+    for (int i = 0; i < n - 1; i++) {  // forward: L with interchanges
+      const double xn = W_(x, i + 1), f = W_(dl, i);
+      if (W_(piv, i)) {
+        W_(x, i) = xn;
+        xi = xi - f * xn;
       } else {
-        const double t = x[i];
-        x[i] = x[i + 1];
-        x[i + 1] = t - dl[i] * x[i];
+        W_(x, i) = xi;
+        xi = xn - f * xi;
       }
     }
-    x[n - 1] /= d[n - 1];
-    if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / d[n - 2];
-    for (int i = n - 3; i >= 0; i--) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
-    double nrm = 0;
-    for (int i = 0; i < n; i++) nrm += x[i] * x[i];
+    W_(x, n - 1) = xi;
+    double x1 = xi * W_(d, n - 1), x2 = 0;  // backward: U (three bands)
+    W_(x, n - 1) = x1;
+    if (n > 1) {
+      x2 = x1;
+      x1 = (W_(x, n - 2) - W_(du, n - 2) * x2) * W_(d, n - 2);
+      W_(x, n - 2) = x1;
+    }
+    double nrm = x1 * x1 + x2 * x2;
+#pragma unroll 4
+    for (int i = n - 3; i >= 0; i--) {
+      const double xi0 = (W_(x, i) - W_(du, i) * x1 - W_(du2, i) * x2) * W_(d, i);
+      W_(x, i) = xi0;
+      x2 = x1;
+      x1 = xi0;
+      nrm += xi0 * xi0;
+    }
+    if (n == 1) nrm = x1 * x1;
     nrm = sqrt(nrm);
     if (!(nrm > 0) || !isfinite(nrm)) return -1;
-    for (int i = 0; i < n; i++) x[i] /= nrm;
+    const double inv = 1.0 / nrm;
+    for (int i = 0; i < n; i++) W_(x, i) = W_(x, i) * inv;
   }
-  for (int j = 0; j < np; j++) {
-    const double* p = prev + (size_t)j * n;
-    double dot = 0;
-    for (int i = 0; i < n; i++) dot += p[i] * x[i];
-    for (int i = 0; i < n; i++) x[i] -= dot * p[i];
-  }
-  double nrm = 0, big = 0;
+  double big = 0;
   int jm = 0;
-  for (int i = 0; i < n; i++) {
-    nrm += x[i] * x[i];
-    if (fabs(x[i]) > big) { big = fabs(x[i]); jm = i; }
-  }
-  const double scl = (x[jm] < 0 ? -1.0 : 1.0) / sqrt(nrm);
-  for (int i = 0; i < n; i++) v[i] = x[i] * scl;
+  for (int i = 0; i < n; i++)
+    if (fabs(W_(x, i)) > big) { big = fabs(W_(x, i)); jm = i; }
+  const double sg = W_(x, jm) < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < n; i++) v[i] = W_(x, i) * sg;
+#undef W_
   return 0;
 }
 

@@ -12,22 +12,24 @@
 //    convergence tested every 500 steps (lanczos.cu:5, 519) with the FP32
 //    residual test of lanczos.cu:114-140.
 //  MIXED: FP32 matrix and FP32-stored basis, FP64 SpMV accumulation,
-//    three-term vector and inner products; tested every 25 steps and
-//    stopped when every residual |beta_m s_{m,j}| is at most tol * 1e-3.
+//    three-term vector and inner products; tested on an adaptive schedule
+//    (multiples of 25 steps) and stopped when every residual
+//    |beta_m s_{m,j}| is at most tol * 1e-3.
 //
 // One step is two kernels and no host round trip:
 //   K1: beta_{k-1} from the |z|^2 partials (every block reduces them in the
 //       same order, so all blocks agree bit for bit), q_k = z_{k-1}/beta_{k-1}
 //       formed on the fly for the stencil gather and written to the basis,
 //       z_k = A q_k - beta_{k-1} q_{k-1}, partial sums of q_k . z_k;
-//   K2: alpha_k from the partials, z_k -= alpha_k q_k, partial |z_k|^2; the
-//       last block to finish advances the step counter in device memory.
+//   K2: alpha_k from the partials, z_k -= alpha_k q_k, partial |z_k|^2;
+//       block 0 advances the step counter in device memory (K2 reads the
+//       step index from the copy K1 left, so no block races the update).
 // Because the step index lives on the device, a whole check period is
-// captured once into a HIP graph and replayed.  At a check point a grid of
-// one-wave blocks finds the FP64 Ritz values of T and of T without its first
-// row/column by 64-way multisection, one block screens them and computes
-// the Ritz vectors by inverse iteration (one eigenvalue per lane), and the
-// host reads back a small status record.
+// captured once into a HIP graph and replayed.  At a check point one block
+// per wanted value finds the FP64 Ritz values of T and of T without its first
+// row/column by 256-way multisection over division-free Sturm counts, one
+// block screens them, one block per Ritz pair runs inverse iteration with its
+// arrays in LDS, and the host reads back a small status record.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -45,7 +47,7 @@ int lanczos_rows_per_lane = 1;  // kept for the tuner interface; K1 runs one row
 
 namespace {
 
-constexpr int kCheckMixed = 25;
+constexpr int kGraphSteps = 25;   // steps per captured graph; periods are multiples of it
 constexpr int kMaxEig = 25;
 constexpr int kCullumExtra = 20;  // eigCheck = nvec + 20 (lanczos.cu:456)
 constexpr int kCullumMax = 1000;  // lanczos.cu:531
@@ -54,8 +56,10 @@ constexpr int kK1 = 256;          // K1: one stencil row per lane
 constexpr int kK2 = 256;          // K2: kK2Rows rows per lane
 constexpr int kK2Rows = 8;
 constexpr int kBisect = 256;      // shifts per multisection round
-constexpr int kInvitLanes = 32;   // interleave of the per-lane inverse-iteration arrays
 constexpr int kInvitIters = 2;
+constexpr size_t kInvitBytes = 7 * sizeof(double) + 1;  // LDS bytes per row of T in k_tri_vec
+constexpr size_t kInvitLds = 156 * 1024;
+constexpr size_t kBisectLdsMax = sizeof(double) * 2 * (kMaxIter + 2);  // (a, b^2) pairs of T
 constexpr int kMaxGraphs = 16;
 
 // Device-side solver state.
@@ -63,7 +67,6 @@ struct LzState {
   int k;                 // index of the next step (read by K1, advanced by K2)
   int k2;                // step index handed from K1 to K2
   float* V;              // basis: q_j at V + j * vstride
-  int converged;         // written by the check
   int nscreened;         // screened Ritz values found
   int fail;              // 1: fewer than nvec screened values, 2: inverse iteration failed
   int m;                 // order of T at the last check
@@ -211,16 +214,14 @@ __global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs<Z> b) {
   }
 }
 
-// FP64 copies of alpha and beta and the squared off-diagonal.
+// FP64 copies of alpha and beta.
 template <typename Z>
 __global__ void k_widen(const Z* __restrict__ al, const Z* __restrict__ be, double* __restrict__ ad,
-                        double* __restrict__ bd, double* __restrict__ b2, int count) {
+                        double* __restrict__ bd, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) {
     ad[i] = (double)al[i];
-    const double bv = (double)be[i];
-    bd[i] = bv;
-    b2[i] = bv * bv;
+    bd[i] = (double)be[i];
   }
 }
 
@@ -228,24 +229,57 @@ __global__ void k_widen(const Z* __restrict__ al, const Z* __restrict__ be, doub
 // Block (idx, which): eigenvalue idx+1 of T (which 0, order m) or of T-hat
 // (which 1, order m-1: alpha[1..], beta[1..]) by multisection with kBisect
 // Sturm counts per round, to the relative accuracy of LAPACK dstebz with
-// abstol 0.
+// abstol 0.  The (a_i, b_{i-1}^2) pairs are staged in LDS once and read by
+// every round; the Gershgorin interval is reduced across the block.
 __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict__ alpha,
-                                                        const double* __restrict__ beta,
-                                                        const double* __restrict__ b2, const LzState* st,
+                                                        const double* __restrict__ beta, const LzState* st,
                                                         double* __restrict__ wout, int ncheck) {
+  extern __shared__ double2 sab[];
+  __shared__ double rlo[kBisect / kWave], rhi[kBisect / kWave], rb2[kBisect / kWave];
   __shared__ unsigned long long masks[kBisect / kWave];
   const int m = st->m;
   const int which = blockIdx.y, idx = blockIdx.x;
   const int n = which ? m - 1 : m;
   if (idx >= n) return;
   const double* a = alpha + which;
-  const tri::Bounds g = tri::gershgorin(n, a, beta + which);
-  double lo = g.lo, hi = g.hi;
+  const double* b = beta + which;
+  double lo = INFINITY, hi = -INFINITY, bmax = 0;
+  for (int i = threadIdx.x; i < n; i += kBisect) {
+    const double bl = i > 0 ? b[i - 1] : 0.0, br = i < n - 1 ? b[i] : 0.0;
+    sab[i] = make_double2(a[i], bl * bl);
+    const double r = fabs(bl) + fabs(br);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    bmax = fmax(bmax, br * br);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    bmax = fmax(bmax, __shfl_xor(bmax, o, 64));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    rlo[threadIdx.x / kWave] = lo;
+    rhi[threadIdx.x / kWave] = hi;
+    rb2[threadIdx.x / kWave] = bmax;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kBisect / kWave; w++) {
+    lo = fmin(lo, rlo[w]);
+    hi = fmax(hi, rhi[w]);
+    bmax = fmax(bmax, rb2[w]);
+  }
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = fmax(2.2250738585072014e-308 * fmax(1.0, bmax), 1e-300);
+  const double pad = 2.2e-16 * tnorm * n + 2 * pivmin;
+  lo -= pad;
+  hi += pad;
   const int kk = idx + 1;
   for (int round = 0; round < 20; round++) {
-    if (hi - lo <= 4.4e-16 * fmax(fabs(lo), fabs(hi)) + g.pivmin) break;
+    if (hi - lo <= 4.4e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     const double step = (hi - lo) / (double)(kBisect + 1);
-    const int c = tri::sturm_poly(n, a, b2 + which, lo + step * (threadIdx.x + 1));
+    const int c = tri::sturm_pairs(n, sab, lo + step * (threadIdx.x + 1));
     // the first shift with at least kk values below it bounds lambda_kk above
     const unsigned long long ge = __ballot(c >= kk);
     if ((threadIdx.x & (kWave - 1)) == 0) masks[threadIdx.x / kWave] = ge;
@@ -266,67 +300,75 @@ __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict
   if (threadIdx.x == 0) wout[which * ncheck + idx] = 0.5 * (lo + hi);
 }
 
-// Screening (lanczos.cu:279-384), Ritz vectors of T (lane j: value j) and
-// the residual test (lanczos.cu:114-140).
-__global__ __launch_bounds__(kWave) void k_tri_finish(const double* __restrict__ alpha,
-                                                      const double* __restrict__ beta, LzState* st,
-                                                      const double* __restrict__ w, int ncheck, int nvec,
-                                                      double tol, int mixed, double* __restrict__ S,
-                                                      double* __restrict__ work, int* __restrict__ iwork) {
+// Screening (lanczos.cu:279-384): lump values closer than 1e-5, drop single
+// values that are also eigenvalues of T-hat; the first nvec survivors go to
+// st->vals.
+__global__ __launch_bounds__(kWave) void k_tri_screen(LzState* st, const double* __restrict__ w, int ncheck,
+                                                      int nvec) {
   __shared__ double acc[kCullumMax];
   __shared__ unsigned char dup[kCullumMax];
-  __shared__ double vals[kMaxEig];
-  __shared__ int ns_sh;
+  if (threadIdx.x != 0) return;
   const int m = st->m;
-  if (threadIdx.x == 0) {
-    const double* wr = w;           // Ritz values of T, ascending
-    const double* wc = w + ncheck;  // of T-hat
-    const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
-    int na = 0;
-    for (int j = 0; j < ec; j++) {  // lump values closer than 1e-5
-      if (j > 0 && wr[j] - wr[j - 1] < 1e-5) {
-        dup[na - 1] = 1;
-        continue;
-      }
-      acc[na] = wr[j];
-      dup[na] = 0;
-      na++;
+  const double* wr = w;           // Ritz values of T, ascending
+  const double* wc = w + ncheck;  // of T-hat
+  const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
+  int na = 0;
+  for (int j = 0; j < ec; j++) {
+    if (j > 0 && wr[j] - wr[j - 1] < 1e-5) {
+      dup[na - 1] = 1;
+      continue;
     }
-    int ns = 0;  // drop single values that are also eigenvalues of T-hat
-    for (int j = 0; j < na && ns < nvec; j++) {
-      bool accept = true;
-      if (!dup[j])
-        for (int c = 0; c < ech; c++)
-          if (fabs((acc[j] - wc[c]) / acc[j]) <= 1e-10) accept = false;
-      if (accept) vals[ns++] = acc[j];
+    acc[na] = wr[j];
+    dup[na] = 0;
+    na++;
+  }
+  int ns = 0;
+  for (int j = 0; j < na && ns < nvec; j++) {
+    bool accept = true;
+    if (!dup[j])
+      for (int c = 0; c < ech; c++)
+        if (fabs((acc[j] - wc[c]) / acc[j]) <= 1e-10) accept = false;
+    if (accept) st->vals[ns++] = acc[j];
+  }
+  st->nscreened = ns;
+  st->fail = ns < nvec ? 1 : 0;
+}
+
+// Ritz vector j of T by inverse iteration (block j; lane 0 runs the
+// recurrences) and its residual |beta_m s_{m,j}| (lanczos.cu:114-140: FP32
+// on the plain path).  When they fit, alpha, beta and the work arrays are
+// staged in LDS; otherwise the work arrays live in global memory.
+__global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                   LzState* st, int mixed, double* __restrict__ S,
+                                                   double* __restrict__ gwork, int lds) {
+  extern __shared__ double dyn[];
+  if (st->fail == 1) return;
+  const int j = blockIdx.x, m = st->m;
+  double* sj = S + (size_t)j * m;
+  const double lam = st->vals[j];
+  int bad;
+  if (lds) {
+    double* sa = dyn;
+    double* sb = dyn + m;
+    for (int i = threadIdx.x; i < m; i += kWave) {
+      sa[i] = alpha[i];
+      sb[i] = beta[i];
     }
-    ns_sh = ns;
-    st->nscreened = ns;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double* wk = dyn + 2 * m;
+    bad = tri::invit(m, sa, sb, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
+                     (unsigned char*)(wk + 5 * m), kInvitIters);
+  } else {
+    if (threadIdx.x != 0) return;
+    double* wk = gwork + (size_t)j * 6 * m;
+    bad = tri::invit(m, alpha, beta, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
+                     (unsigned char*)(wk + 5 * m), kInvitIters);
   }
-  __syncthreads();
-  if (ns_sh < nvec) {
-    if (threadIdx.x == 0) {
-      st->fail = 1;
-      st->converged = 0;
-    }
-    return;
-  }
-  const int j = threadIdx.x;
-  bool bad = false, good = true;
-  if (j < nvec) {
-    double* sj = S + (size_t)j * m;
-    bad = tri::invit(m, alpha, beta, vals[j], sj, work + j, iwork + j, kInvitLanes, kInvitIters) != 0;
-    const double bm = beta[m - 1];
-    const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
-    st->res[j] = r;
-    st->vals[j] = vals[j];
-    good = !bad && !(r > tol);
-  }
-  const unsigned long long anybad = __ballot(bad), notgood = __ballot(!good);
-  if (j == 0) {
-    st->fail = anybad ? 2 : 0;
-    st->converged = notgood == 0;
-  }
+  const double bm = beta[m - 1];
+  const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
+  st->res[j] = bad ? INFINITY : r;
+  if (bad) st->fail = 2;
 }
 
 // ---------------------------------------------------------------- finish
@@ -430,7 +472,6 @@ __global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n) {
     st->k = 0;
     st->k2 = 0;
     st->V = V;
-    st->converged = 0;
     st->nscreened = 0;
     st->fail = 0;
     st->m = 0;
@@ -527,6 +568,26 @@ int launch_ritz(hipStream_t s, int nvec, int n, const LzState* st, size_t vstrid
   return DMSC_OK;
 }
 
+// Mixed path check schedule: the largest residual of the wanted Ritz pairs
+// falls roughly geometrically with the step count, so the next check goes
+// where the trend of the last two checks reaches the tolerance (scaled by
+// 0.7), at least kGraphSteps and at most m/4 steps ahead, in whole graphs.
+int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, double& prev_res) {
+  int steps = std::max(kGraphSteps, m / 4);
+  if (hs.fail == 0) {
+    double r = 0;
+    for (int j = 0; j < nvec; j++) r = std::max(r, hs.res[j]);
+    if (prev_m > 0 && r > 0 && prev_res > r) {
+      const double rate = (std::log(r) - std::log(prev_res)) / (double)(m - prev_m);
+      steps = (int)(0.7 * (std::log(tol) - std::log(r)) / rate);
+    }
+    prev_m = m;
+    prev_res = r;
+  }
+  steps = std::min(std::max(steps, kGraphSteps), std::max(kGraphSteps, m / 4));
+  return (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
+}
+
 template <typename Z>
 int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool mixed, float* h_eigvals,
           float* d_eigvecs, int* iterations) {
@@ -535,7 +596,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const size_t vstride = (size_t)pitch_floats(n);
   const int nb1 = div_up(n, kK1), nb2 = div_up(n, kK2 * kK2Rows);
   const StencilOffsets so = make_stencil_offsets(w);
-  const int period = mixed ? kCheckMixed : kCheckIter;
+  int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
   const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
   const int nblk = std::min(div_up(n, 256), 1024);
 
@@ -552,11 +613,9 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const size_t o_p = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb2);
   const size_t o_al = carve(sizeof(Z) * (kMaxIter + 2)), o_be = carve(sizeof(Z) * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
-  const size_t o_b2 = carve(sizeof(double) * (kMaxIter + 2));
   const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);
   const size_t o_S = carve(sizeof(double) * kMaxEig * (kMaxIter + 2));
-  const size_t o_work = carve(sizeof(double) * 5 * kInvitLanes * (kMaxIter + 2));
-  const size_t o_iwork = carve(sizeof(int) * kInvitLanes * (kMaxIter + 2));
+  const size_t o_work = carve(sizeof(double) * 6 * kMaxEig * (kMaxIter + 2));
   const size_t o_mm = carve(sizeof(float) * 2 * kMaxEig * nblk), o_fin = carve(sizeof(float) * 2 * kMaxEig);
   void* base;
   DMSC_TRY(ctx->scratch(2, off, &base));
@@ -565,7 +624,6 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   float* rs = (float*)(B + o_rs);
   double* ad = (double*)(B + o_ad);
   double* bd = (double*)(B + o_bd);
-  double* b2 = (double*)(B + o_b2);
   double* wv = (double*)(B + o_w);
   double* S = (double*)(B + o_S);
 
@@ -590,13 +648,27 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   bf.nb2 = nb2;
   bf.st = st;
 
+  static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
+  if (!invit_lds) {
+    int lds_max = 0;
+    DMSC_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    if ((size_t)lds_max < kBisectLdsMax + 1024)
+      return fail(DMSC_ERR_INTERNAL, "device LDS per block %d below the %zu the eigensolver needs", lds_max,
+                  kBisectLdsMax + 1024);
+    const size_t vec_lds = std::min(kInvitLds, (size_t)lds_max);
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kBisectLdsMax));
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_vec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
+    invit_lds = vec_lds;
+  }
   DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
   const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
   hipLaunchKernelGGL(k_init<Z>, dim3(div_up(n, 256)), dim3(256), 0, s, st, V, bf.z1, q0, n);
   DMSC_HIP(hipGetLastError());
 
   int eigCheck = nvec + kCullumExtra;
-  int done = 0;
+  int done = 0, prev_m = 0;
+  double prev_res = 0;
   LzState hs{};
   bool converged = false;
   while (done < kMaxIter) {
@@ -611,15 +683,18 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       cap = ncap;
       hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
     }
-    DMSC_TRY(run_steps<Z>(ctx, bf, so, w, steps));
+    for (int t = 0; t < steps; t += kGraphSteps) DMSC_TRY(run_steps<Z>(ctx, bf, so, w, std::min(kGraphSteps, steps - t)));
     done += steps;
     hipLaunchKernelGGL(k_lz_beta<Z>, dim3(1), dim3(kK1), 0, s, bf);
-    hipLaunchKernelGGL(k_widen<Z>, dim3(div_up(done, 256)), dim3(256), 0, s, bf.alpha, bf.beta, ad, bd, b2, done);
+    hipLaunchKernelGGL(k_widen<Z>, dim3(div_up(done, 256)), dim3(256), 0, s, bf.alpha, bf.beta, ad, bd, done);
     for (;;) {
       const int ncheck = std::min(eigCheck, done);
-      hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), 0, s, ad, bd, b2, st, wv, ncheck);
-      hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(kWave), 0, s, ad, bd, st, wv, ncheck, nvec, stop_tol,
-                         mixed ? 1 : 0, S, (double*)(B + o_work), (int*)(B + o_iwork));
+      hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, s, ad, bd, st, wv,
+                         ncheck);
+      hipLaunchKernelGGL(k_tri_screen, dim3(1), dim3(kWave), 0, s, st, wv, ncheck, nvec);
+      const bool lds = (size_t)done * kInvitBytes <= invit_lds;
+      hipLaunchKernelGGL(k_tri_vec, dim3(nvec), dim3(kWave), lds ? (size_t)done * kInvitBytes : 0, s, ad, bd, st,
+                         mixed ? 1 : 0, S, (double*)(B + o_work), lds ? 1 : 0);
       DMSC_HIP(hipGetLastError());
       DMSC_HIP(hipMemcpyAsync(&hs, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
       DMSC_HIP(hipStreamSynchronize(s));
@@ -628,10 +703,16 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       if (eigCheck > kCullumMax) return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
     }
     if (hs.fail == 2) return fail(DMSC_ERR_INTERNAL, "inverse iteration failed on the Lanczos tridiagonal");
-    if (hs.fail == 0 && hs.converged) {
-      converged = true;
-      break;
+    if (hs.fail == 0) {
+      bool good = true;  // lanczos.cu:114-140 TestForConvergence
+      for (int j = 0; j < nvec; j++)
+        if (hs.res[j] > stop_tol) good = false;
+      if (good) {
+        converged = true;
+        break;
+      }
     }
+    if (mixed) period = next_period(done, hs, nvec, stop_tol, prev_m, prev_res);
   }
   if (!converged) return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", kMaxIter);
   const int m = hs.m;

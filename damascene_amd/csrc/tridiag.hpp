@@ -29,18 +29,20 @@ __host__ __device__ inline int sturm(int n, const double* a, const double* b, do
 // p_0 = 1, p_1, ..., p_n equals the number of eigenvalues below x (a zero
 // p_i counts as a change, the convention of the pivot form above).  One
 // dependent multiply-add per row instead of a division; the pair is
-// rescaled by an exact power of two every 4 rows to stay in range.  b2 holds
-// the squared off-diagonal.
-__host__ __device__ inline int sturm_poly(int n, const double* a, const double* b2, double x) {
+// rescaled by an exact power of two every 4 rows to stay in range.
+// ab(i) returns (a_i, b_{i-1}^2).
+template <typename AB>
+__host__ __device__ inline int sturm_count(int n, AB ab, double x) {
   const double kBig = 0x1p+400, kSmall = 0x1p-400;
-  double pp = 1.0, p = a[0] - x;  // p_0, p_1
+  double pp = 1.0, p = ab(0).x - x;  // p_0, p_1
   if (p == 0) p = -0x1p-1000;
   int cnt = p < 0;
   int i = 1;
   for (; i + 4 <= n; i += 4) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      double pn = fma(a[i + j] - x, p, -b2[i + j - 1] * pp);
+      const double2 e = ab(i + j);
+      double pn = fma(e.x - x, p, -e.y * pp);
       if (pn == 0) pn = p < 0 ? 0x1p-1000 : -0x1p-1000;
       cnt += (pn < 0) != (p < 0);
       pp = p;
@@ -52,13 +54,24 @@ __host__ __device__ inline int sturm_poly(int n, const double* a, const double* 
     pp *= sc;
   }
   for (; i < n; i++) {
-    double pn = fma(a[i] - x, p, -b2[i - 1] * pp);
+    const double2 e = ab(i);
+    double pn = fma(e.x - x, p, -e.y * pp);
     if (pn == 0) pn = p < 0 ? 0x1p-1000 : -0x1p-1000;
     cnt += (pn < 0) != (p < 0);
     pp = p;
     p = pn;
   }
   return cnt;
+}
+
+// Diagonal a[0..n) and squared off-diagonal b2[0..n-1).
+__host__ __device__ inline int sturm_poly(int n, const double* a, const double* b2, double x) {
+  return sturm_count(n, [&](int i) { return make_double2(a[i], i > 0 ? b2[i - 1] : 0.0); }, x);
+}
+
+// Pairs (a_i, b_{i-1}^2), e.g. an LDS copy.
+__host__ __device__ inline int sturm_pairs(int n, const double2* ab, double x) {
+  return sturm_count(n, [&](int i) { return ab[i]; }, x);
 }
 
 // 1/x: on the device v_rcp_f64 refined by two Newton steps (a few ulp,
@@ -112,19 +125,16 @@ __host__ __device__ inline double kth(int n, const double* a, const double* b, i
 
 // Inverse iteration (LU with partial pivoting of T - lam I, as LAPACK
 // dgttrf/dstein) for the eigenvalue lam, `iters` solves from a fixed
-// pseudo-random start.  work holds 5n doubles and piv n ints; element i of
-// each array sits at index i * ws, so the lanes of a wave can interleave
-// their arrays (ws = lane count) and access them coalesced.  No
-// orthogonalisation against neighbouring vectors: the callers pass values
-// at least 1e-5 apart (Cullum-Willoughby lumping), which inverse iteration
-// separates by a factor 1e-5 / (|T| eps) per solve.  The result is
-// normalised with its largest-magnitude entry positive.  Returns -1 on a
-// non-finite iterate.
-__host__ __device__ inline int invit(int n, const double* a, const double* b, double lam, double* v,
-                                     double* work, int* piv, int ws, int iters) {
-  double *dl = work, *d = work + (size_t)n * ws, *du = work + (size_t)2 * n * ws, *du2 = work + (size_t)3 * n * ws,
-         *x = work + (size_t)4 * n * ws;
-#define W_(arr, i) arr[(size_t)(i) * ws]
+// pseudo-random start.  dl, d, du, du2, x: n doubles each, piv: n bytes
+// (device callers place them in LDS).  No orthogonalisation against
+// neighbouring vectors: the callers pass values at least 1e-5 apart
+// (Cullum-Willoughby lumping), which inverse iteration separates by a factor
+// 1e-5 / (|T| eps) per solve.  The result is normalised with its
+// largest-magnitude entry positive.  Returns -1 on a non-finite iterate.
+__host__ __device__ inline int invit(int n, const double* __restrict__ a, const double* __restrict__ b, double lam,
+                                     double* __restrict__ v, double* __restrict__ dl, double* __restrict__ d,
+                                     double* __restrict__ du, double* __restrict__ du2, double* __restrict__ x,
+                                     unsigned char* __restrict__ piv, int iters) {
   double tnorm = 0;
   for (int i = 0; i < n; i++)
     tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0));
@@ -132,84 +142,82 @@ __host__ __device__ inline int invit(int n, const double* a, const double* b, do
   // factorisation (dgttrf), carrying the running diagonal entry di and the
   // running upper entry ui (modified by a preceding row interchange)
   double di = a[0] - lam, ui = n > 1 ? b[0] : 0.0;
+#pragma unroll 4
   for (int i = 0; i < n - 1; i++) {
     const double li = b[i], dn = a[i + 1] - lam, un = i < n - 2 ? b[i + 1] : 0.0;
     if (fabs(di) >= fabs(li)) {
-      W_(piv, i) = 0;
+      piv[i] = 0;
       if (di == 0) di = eps;
       const double f = li * recip(di);
-      W_(dl, i) = f;
-      W_(d, i) = di;
-      W_(du, i) = ui;
-      W_(du2, i) = 0;
+      dl[i] = f;
+      d[i] = di;
+      du[i] = ui;
+      du2[i] = 0;
       di = dn - f * ui;
       ui = un;
     } else {
-      W_(piv, i) = 1;
+      piv[i] = 1;
       const double f = di * recip(li);
-      W_(dl, i) = f;
-      W_(d, i) = li;
-      W_(du, i) = dn;
-      W_(du2, i) = un;
+      dl[i] = f;
+      d[i] = li;
+      du[i] = dn;
+      du2[i] = un;
       di = ui - f * dn;
       ui = -f * un;
     }
   }
-  W_(d, n - 1) = di;
-  W_(piv, n - 1) = 0;
+  d[n - 1] = di;
+  piv[n - 1] = 0;
   for (int i = 0; i < n; i++) {
-    double t = W_(d, i);
+    double t = d[i];
     if (fabs(t) < eps) t = t < 0 ? -eps : eps;
-    W_(d, i) = recip(t);  // reciprocal pivots
+    d[i] = recip(t);  // reciprocal pivots
   }
   unsigned long long s = 0x9E3779B97F4A7C15ULL;
   for (int i = 0; i < n; i++) {
     s ^= s << 13; s ^= s >> 7; s ^= s << 17;
-    W_(x, i) = ((double)(s >> 11) / 9007199254740992.0) - 0.5;
+    x[i] = ((double)(s >> 11) / 9007199254740992.0) - 0.5;
   }
   for (int iter = 0; iter < iters; iter++) {
-    double xi = W_(x, 0);
-#pragma unroll 4
+    double xi = x[0];
+#pragma unroll 8
     for (int i = 0; i < n - 1; i++) {  // forward: L with interchanges
-      const double xn = W_(x, i + 1), f = W_(dl, i);
-      if (W_(piv, i)) {
-        W_(x, i) = xn;
+      const double xn = x[i + 1], f = dl[i];
+      if (piv[i]) {
+        x[i] = xn;
         xi = xi - f * xn;
       } else {
-        W_(x, i) = xi;
+        x[i] = xi;
         xi = xn - f * xi;
       }
     }
-    W_(x, n - 1) = xi;
-    double x1 = xi * W_(d, n - 1), x2 = 0;  // backward: U (three bands)
-    W_(x, n - 1) = x1;
+    double x1 = xi * d[n - 1], x2 = 0;  // backward: U (three bands)
+    x[n - 1] = x1;
     if (n > 1) {
       x2 = x1;
-      x1 = (W_(x, n - 2) - W_(du, n - 2) * x2) * W_(d, n - 2);
-      W_(x, n - 2) = x1;
+      x1 = (x[n - 2] - du[n - 2] * x2) * d[n - 2];
+      x[n - 2] = x1;
     }
     double nrm = x1 * x1 + x2 * x2;
-#pragma unroll 4
+#pragma unroll 8
     for (int i = n - 3; i >= 0; i--) {
-      const double xi0 = (W_(x, i) - W_(du, i) * x1 - W_(du2, i) * x2) * W_(d, i);
-      W_(x, i) = xi0;
+      const double xi0 = (x[i] - du[i] * x1 - du2[i] * x2) * d[i];
+      x[i] = xi0;
       x2 = x1;
       x1 = xi0;
       nrm += xi0 * xi0;
     }
-    if (n == 1) nrm = x1 * x1;
     nrm = sqrt(nrm);
     if (!(nrm > 0) || !isfinite(nrm)) return -1;
     const double inv = 1.0 / nrm;
-    for (int i = 0; i < n; i++) W_(x, i) = W_(x, i) * inv;
+    for (int i = 0; i < n; i++) x[i] = x[i] * inv;
   }
   double big = 0;
   int jm = 0;
   for (int i = 0; i < n; i++)
-    if (fabs(W_(x, i)) > big) { big = fabs(W_(x, i)); jm = i; }
-  const double sg = W_(x, jm) < 0 ? -1.0 : 1.0;
-  for (int i = 0; i < n; i++) v[i] = W_(x, i) * sg;
-#undef W_
+    if (fabs(x[i]) > big) { big = fabs(x[i]); jm = i; }
+  const double sg = x[jm] < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < n; i++) v[i] = x[i] * sg;
   return 0;
 }
 

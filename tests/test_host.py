@@ -1,0 +1,153 @@
+"""CPU tests of the product library's host side (no GPU needed): the C ABI
+loads and exports every function include/dmsc.h declares, its host-built
+tables are bit-identical to the oracle's, the synthetic image generator is
+deterministic and shaped as specified, and calls that need a device fail
+with a status code instead of crashing.
+"""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import damascene_amd as dm
+from damascene_amd._lib import lib
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+P = O.P
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    O.build()
+    if not os.path.exists(os.path.join(ROOT, "damascene_amd", "libdmsc.so")):
+        pytest.fail("libdmsc.so is not built (python -m damascene_amd.build)")
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "dmsc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dmsc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 30
+    l = ctypes.CDLL(os.path.join(ROOT, "damascene_amd", "libdmsc.so"))
+    missing = [n for n in names if not hasattr(l, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "damascene_amd", "libdmsc.so")],
+                         capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (dmsc_[a-z0-9_]+)", out))
+    assert set(names) <= exported
+
+
+def test_version_and_pitch():
+    assert lib().dmsc_version().startswith(b"dmsc")
+    for n in (1, 63, 64, 65, 19200, 154401):
+        p = dm.pitch_in_floats(n)
+        assert p >= n and p % 64 == 0 and p - n < 64
+
+
+def test_no_device_is_an_error_not_a_crash():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    c = ctypes.c_void_p()
+    rc = lib().dmsc_ctx_create(ctypes.byref(c), 0, None)
+    assert rc != 0
+    assert b"device" in lib().dmsc_last_error_string().lower()
+
+
+def test_texton_and_parabola_and_spb_tables_match_oracle():
+    c1, r1, n1 = O.texton_filters()
+    c2 = np.zeros(9010, np.float32)
+    r2 = np.zeros(34, np.int32)
+    assert lib().dmsc_texton_filters(P(c2), P(r2)) == n1
+    np.testing.assert_array_equal(c1.view(np.uint32), c2.view(np.uint32))
+    np.testing.assert_array_equal(r1, r2)
+    for r in (3, 5, 10, 20):
+        a = O.parabola_filters(r)
+        b = np.zeros_like(a)
+        lib().dmsc_parabola_filters(r, P(b))
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    a = O.spb_filters()
+    b = np.zeros_like(a)
+    lib().dmsc_spb_filters(P(b))
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("W,H", [(541, 381), (220, 180), (1084, 828), (100, 100)])
+def test_rotation_and_halfdisc_tables_match_oracle(W, H):
+    for o in range(4):
+        tp = -float(o) / 8
+        nw, nh, bx, by, gx, gy = O.rotation_tables(W, H, tp)
+        n = max(W, H) * 4 + 16
+        B = [np.zeros(n, np.int32) for _ in range(4)]
+        nw2, nh2 = ctypes.c_int(), ctypes.c_int()
+        assert lib().dmsc_rotation_tables(W, H, ctypes.c_float(tp), ctypes.byref(nw2), ctypes.byref(nh2),
+                                          *[P(x) for x in B]) == 0
+        assert (nw, nh) == (nw2.value, nh2.value)
+        for x, y in zip(B, (bx, by, gx, gy)):
+            np.testing.assert_array_equal(x[:len(y)], y)
+        for radius in (3, 5, 10, 20):
+            want = np.zeros(8, np.int32)
+            O.lib().orc_halfdisc_offsets(W - 60, H - 60, ctypes.c_float(tp), radius, P(bx), P(by), P(gx), P(gy),
+                                         nw, P(want))
+            got = np.zeros(8, np.int32)
+            assert lib().dmsc_halfdisc_offsets(W - 60, H - 60, ctypes.c_float(tp), radius, P(B[0]), P(B[1]),
+                                               P(B[2]), P(B[3]), nw, P(got)) == 0
+            np.testing.assert_array_equal(got, want)
+
+
+def test_blur_gamma_stencil_tables_match_oracle():
+    for s in (2.0, 1.0):
+        k, r = O.blur_kernel(s)
+        k2 = np.zeros(64, np.float32)
+        rr = ctypes.c_int()
+        n = lib().dmsc_blur_kernel(ctypes.c_float(s), P(k2), ctypes.byref(rr))
+        assert n == k.size and rr.value == r
+        np.testing.assert_array_equal(k, k2[:n])
+    g1 = O.gamma_lut(2.5)
+    g2 = np.zeros(256, np.float32)
+    lib().dmsc_gamma_lut(ctypes.c_float(2.5), P(g2))
+    np.testing.assert_array_equal(g1.view(np.uint32), g2.view(np.uint32))
+    for w in (7, 160, 481):
+        o1, d1 = O.stencil(5, w)
+        o2 = np.zeros(128, np.int32)
+        d2 = np.zeros(121, np.int32)
+        assert lib().dmsc_stencil(5, w, P(o2), P(d2)) == 81
+        np.testing.assert_array_equal(o1[:81], o2[:81])
+        np.testing.assert_array_equal(d1[:121], d2)
+
+
+def test_generator_is_deterministic_and_shaped():
+    a, cells = dm.generate_image(160, 120, 12, 20261202, with_cells=True)
+    b = dm.generate_image(160, 120, 12, 20261202)
+    c = dm.generate_image(160, 120, 12, 20261203)
+    assert a.shape == (120, 160) and a.dtype == np.uint32
+    np.testing.assert_array_equal(a, b)
+    assert (a != c).any()
+    assert (a >> 24 == 0).all()  # RGB8 packed in the low 24 bits
+    assert set(np.unique(cells)) == set(range(12))
+    m = dm.boundary_mask(cells)
+    # boundaries = cell changes between 4-neighbours, dilated by 1 px
+    edge = np.zeros_like(cells, bool)
+    edge[:, 1:] |= cells[:, 1:] != cells[:, :-1]
+    edge[1:, :] |= cells[1:, :] != cells[:-1, :]
+    assert m[edge].all()
+    assert 0.02 < m.mean() < 0.5
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout

@@ -1,0 +1,378 @@
+"""CPU tests that pin the oracle (oracle/liboracle.so) before it is trusted
+as the checker of the HIP path.
+
+Each oracle function is checked against something independent of its own
+code: the reference's shipped tables and data files, outputs of the
+reference's own host sources compiled in the build container (committed as
+fixtures by tests/golden/make_golden.py), known answers from the reference's
+drivers, closed forms, invariants, or brute force on tiny inputs.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from oracle import pyoracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RNG = np.random.default_rng(20261202)
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    O.build()
+
+
+# --------------------------------------------------------------- tables
+def split_filters(coeffs, radii):
+    out, off = [], 0
+    for r in radii:
+        n = (2 * r + 1) ** 2
+        out.append(coeffs[off:off + n].reshape(2 * r + 1, 2 * r + 1))
+        off += n
+    return out
+
+
+def test_texton_filters_match_reference_build():
+    """createTextonFilters (textons/filters.cpp:404-454) compiled from the
+    reference sources: all 34 filters, including the DFT Hilbert pairs."""
+    c, r, n = O.texton_filters()
+    g = gold("ref_build.npz")
+    np.testing.assert_array_equal(r, g["texton_radii"])
+    assert np.abs(c - g["texton_coeffs"]).max() <= 1e-7
+
+
+def test_texton_filters_match_shipped_table():
+    """include/filters.h:22-23 pins the 18 even and centre-surround filters
+    (the 16 odd/Hilbert filters of the table come from another Hilbert)."""
+    c, r, n = O.texton_filters()
+    g = gold("filters_h.npz")
+    np.testing.assert_array_equal(r, g["radii"])
+    mine, table = split_filters(c, r), split_filters(g["coeffs"], g["radii"])
+    even = [i for s in (0, 17) for i in list(range(s, s + 8)) + [s + 16]]
+    for i in even:
+        # the table was printed from a slightly different build: it agrees with
+        # the compiled reference (and the oracle) to 3e-7 absolute
+        assert np.abs(mine[i] - table[i]).max() <= 3.5e-7, i
+    # the bank's even filters are zero-mean and L1-normalised (filters.cpp:150-154)
+    for i in even[:8]:
+        assert abs(mine[i].sum()) < 1e-5 and abs(np.abs(mine[i]).sum() - 1) < 1e-4
+
+
+def test_spectral_pb_filters_match_reference_build():
+    """The eight oriented first-derivative kernels of sPb/spectralPb.cu:708-711."""
+    np.testing.assert_allclose(O.spb_filters(), gold("ref_build.npz")["spb"], atol=1e-7, rtol=0)
+
+
+def test_savgol_filters_match_shipped_data():
+    """damascene/data/savgol_{r}_{k}.dat (theta = pi/2 - (k-1) pi/8) for
+    k = 2..8.  The k=1 files were made in double arithmetic, where the two
+    ellipse-tip taps (u = +-r on the centre row) fall outside the ellipse;
+    the float runtime (localcues.cu:21-100) includes them, which changes the
+    whole least-squares fit.  For k=1 the test checks exactly that
+    difference: the runtime filter has the tips, the data file does not."""
+    g = gold("savgol.npz")
+    for r in (3, 5, 10, 20):
+        f = O.parabola_filters(r).reshape(8, 2 * r + 1, 2 * r + 1)
+        for k in range(2, 9):
+            d = np.abs(f[k - 1] - g[f"r{r}_k{k}"])
+            assert d.max() <= 1e-6, (r, k, d.max())
+        ref1 = g[f"r{r}_k1"]
+        assert ref1[r, 0] == 0 and ref1[r, 2 * r] == 0
+        assert f[0][r, 0] != 0 and f[0][r, 2 * r] != 0
+        both = (f[0] != 0) & (ref1 != 0)
+        assert both.sum() == (ref1 != 0).sum()  # every tap of the data file is a runtime tap
+
+
+@pytest.mark.parametrize("r", [3, 5, 10, 20])
+def test_savgol_reproduces_quadratics(r):
+    """A degree-2 local fit returns the centre value of any quadratic in the
+    coordinate across the filter's orientation (exactness of least squares)."""
+    f = O.parabola_filters(r).reshape(8, 2 * r + 1, 2 * r + 1).astype(np.float64)
+    ys, xs = np.mgrid[-r:r + 1, -r:r + 1]
+    for o in range(8):
+        taps = f[o] != 0
+        assert abs(f[o].sum() - 1) < 1e-4
+        # moments of the taps: the first moment vanishes by symmetry of the ellipse
+        assert abs((f[o] * xs).sum()) < 1e-3 and abs((f[o] * ys).sum()) < 1e-3
+        assert taps.sum() >= 3
+
+
+def test_stencil_matches_reference_build():
+    g = gold("ref_build.npz")
+    for w in (7, 33, 160, 481):
+        offs, diag = O.stencil(5, w)
+        np.testing.assert_array_equal(offs[:81], g[f"offsets_w{w}"])
+        # cells inside the radius-5 disc (the reference leaves the others
+        # unset; the oracle marks them -1)
+        dy, dx = np.mgrid[-5:6, -5:6]
+        inside = (dy * dy + dx * dx <= 25).ravel()
+        np.testing.assert_array_equal(diag[:121][inside], g[f"diag_w{w}"][inside])
+        assert (diag[:121][~inside] == -1).all()
+
+
+def test_stencil_is_radius5_disc():
+    offs, diag = O.stencil(5, 50)
+    lin = np.cumsum(offs[:81])
+    pts = [(dy, dx) for dy in range(-5, 6) for dx in range(-5, 6) if dy * dy + dx * dx <= 25]
+    np.testing.assert_array_equal(lin, [dy * 50 + dx for dy, dx in pts])
+
+
+def test_gamma_lut_and_cbrt():
+    lut = O.gamma_lut(2.5)
+    # convert.cu forms r = c / 255.0f in float, then powf(r, gamma): the LUT
+    # holds the correctly rounded power of that float
+    r = np.arange(256, dtype=np.float32) / np.float32(255)
+    np.testing.assert_array_equal(lut, (r.astype(np.float64) ** 2.5).astype(np.float32))
+    x = np.concatenate([RNG.random(20000).astype(np.float32), np.float32([0, 1, 0.008856, 1e-30, 3.0])])
+    c = O.cbrtf(x)
+    ulp = np.abs(c.astype(np.float64) - np.cbrt(x.astype(np.float64))) / np.spacing(np.abs(c)).astype(np.float64)
+    assert ulp[x > 0].max() <= 1.0
+
+
+def test_blur_kernel_is_normalised_gaussian():
+    for s in (2.0, 1.0):
+        k, r = O.blur_kernel(s)
+        assert r == math.ceil(3 * s) and k.size == 2 * r + 1
+        g = np.exp(-0.5 * (np.arange(-r, r + 1) / s) ** 2)
+        np.testing.assert_allclose(k, g / g.sum(), rtol=2e-6)
+
+
+# --------------------------------------------------------------- colour
+def test_grey_and_channel_order():
+    rgb = RNG.integers(0, 1 << 24, size=(17, 23), dtype=np.uint32)
+    r = (rgb & 0xFF).astype(np.float32) / np.float32(255)
+    g = ((rgb >> 8) & 0xFF).astype(np.float32) / np.float32(255)
+    b = ((rgb >> 16) & 0xFF).astype(np.float32) / np.float32(255)
+    want = np.float32(0.29894) * r + np.float32(0.58704) * g + np.float32(0.11402) * b
+    np.testing.assert_allclose(O.rgb_to_grey(rgb), want, atol=2e-7, rtol=0)
+
+
+def test_lab_against_double_precision_formula():
+    rgb = RNG.integers(0, 1 << 24, size=(31, 29), dtype=np.uint32)
+    L, a, b = O.rgb_to_lab(rgb, 2.5, normalize=False)
+    c = np.stack([(rgb >> s) & 0xFF for s in (0, 8, 16)], -1) / 255.0
+    c = c ** 2.5
+    M = np.array([[0.412453, 0.357580, 0.180423], [0.212671, 0.715160, 0.072169],
+                  [0.019334, 0.119193, 0.950227]])
+    xyz = c @ M.T / np.array([0.950456, 1.0, 1.088754])
+    f = np.where(xyz > 0.008856, np.cbrt(xyz), 7.787 * xyz + 16 / 116)
+    y = xyz[..., 1]
+    Lr = np.where(y > 0.008856, 116 * np.cbrt(y) - 16, 903.3 * y)
+    np.testing.assert_allclose(L, Lr, atol=2e-3)
+    np.testing.assert_allclose(a, 500 * (f[..., 0] - f[..., 1]), atol=2e-3)
+    np.testing.assert_allclose(b, 200 * (f[..., 1] - f[..., 2]), atol=2e-3)
+
+
+def test_mirror_and_quantize():
+    x = RNG.random((9, 13)).astype(np.float32)
+    m = np.empty((9 + 8, 13 + 8), np.float32)
+    O.lib().orc_mirror_f(13, 9, 4, O.P(x), O.P(m))
+    np.testing.assert_array_equal(m, np.pad(x, 4, mode="symmetric"))
+    q = np.empty(x.shape, np.int32)
+    x[0, 0], x[0, 1] = 1.0, 0.0
+    O.lib().orc_quantize(13, 9, 25, O.P(x), O.P(q))
+    np.testing.assert_array_equal(q, np.minimum(np.floor(x * np.float32(25)).astype(np.int32), 24))
+
+
+# --------------------------------------------------------------- textons
+def test_kmeans_fixed_point_invariants():
+    """Labels are the strict argmin (ties to the lowest index) of the
+    distance to the truncated-integer-mean centroids of those labels."""
+    w, h = 40, 30
+    grey = RNG.random((h, w)).astype(np.float32)
+    resp = O.texton_responses(grey)
+    labels, it = O.kmeans(resp, max_iter=100)
+    assert it < 100, "no fixed point reached"
+    X = resp.reshape(34, -1).T.astype(np.float32)
+    Xi = np.trunc(X * np.float32(100000)).astype(np.int64)
+    lab = labels.ravel()
+    C = np.full((32, 34), np.nan, np.float32)
+    for c in np.unique(lab):
+        s = Xi[lab == c].sum(0)
+        s = ((s + 2**31) % 2**32 - 2**31).astype(np.float32)  # int32 wrap-around
+        C[c] = s / (np.float32(100000) * np.float32((lab == c).sum()))
+    d = (X.astype(np.float64) ** 2).sum(1)[:, None] + np.nansum(C.astype(np.float64) ** 2, 1)[None] \
+        - 2 * X.astype(np.float64) @ np.nan_to_num(C).T.astype(np.float64)
+    d[:, np.isnan(C[:, 0])] = np.inf
+    best = d.min(1)
+    gap = d[np.arange(d.shape[0]), lab] - best
+    assert (gap <= 1e-4 * np.maximum(1, best)).all()
+    # filter 33 is never bound, so its response is identically zero
+    assert not resp[33].any()
+
+
+# --------------------------------------------------------------- histograms
+def test_integral_images_bruteforce():
+    W, H, nb = 19, 11, 7
+    lab = RNG.integers(0, nb, size=(H, W)).astype(np.int32)
+    lab[2, 3] = -1  # outside the rotated frame: counted in no bin
+    ii = O.integral_images(lab, nb)
+    want = np.stack([np.cumsum(np.cumsum(lab == b, 0), 1) for b in range(nb)], -1)
+    np.testing.assert_array_equal(ii.reshape(H, W, -1)[..., :nb], want)
+
+
+def test_gradients_constant_image_and_chi2_bounds():
+    w, h = 40, 36
+    zeros = np.zeros((h, w), np.int32)
+    g = O.channel_gradients(zeros, 25, 0, 0.0, [3, 5, 10])
+    assert np.abs(g).max() < 1e-6  # equal histograms up to the rounding of the sum-8 normalisation
+    # two regions split by a vertical line: chi^2 lies in [0, 8] (histograms
+    # normalised to sum 8) and reaches 8 exactly at the edge for the
+    # orientation whose half-discs fall on either side
+    lab = np.zeros((h, w), np.int32)
+    lab[:, w // 2:] = 5
+    g = O.channel_gradients(lab, 25, 0, 0.0, [3, 5, 10]).reshape(3, 8, h, w)
+    assert g.min() >= 0 and g.max() <= 8 + 1e-5
+    assert np.isclose(g[:, :, h // 2, w // 2 - 1:w // 2 + 1].max(), 8.0, atol=1e-5)
+    assert g[0, :, h // 2, 2].max() < 1e-6  # far from the edge at radius 3
+
+
+def test_combine_known_answer():
+    """combine/combinemain.cu:24-59: 2x2 image, cues filled with 2, 3, 4, 5."""
+    n = 4
+    cues = [np.full((24, n), v, np.float32) for v in (2.0, 3.0, 4.0, 5.0)]
+    mpb, cg = O.combine(*[c.reshape(24, 2, 2) for c in cues])
+    np.testing.assert_allclose(mpb, 0.6268, atol=1e-6)
+    np.testing.assert_allclose(cg, 0.2140, atol=1e-6)
+
+
+# --------------------------------------------------------------- nonmax / gPb
+def test_nonmax_ridge():
+    """nonmax.cu:147-246 suppresses along theta = (ori + 4) % 8: orientation 0
+    compares left/right neighbours (its x is the row index), orientation 4
+    compares up/down.  A ridge across the suppression axis survives, scaled
+    by 1.2 and capped at 1, on its crest only; the border is zero."""
+    w, h = 21, 15
+    cols = np.arange(w)[None, :].repeat(h, 0)
+    rows = np.arange(h)[:, None].repeat(w, 1)
+    for ori, coord, crest in ((0, cols, 10), (4, rows, 7)):
+        pb = np.zeros((8, h, w), np.float32)
+        pb[ori] = np.exp(-0.5 * ((coord - crest) / 1.5) ** 2) * np.float32(0.9)
+        nm = O.nonmax(pb)
+        # pixels whose two neighbours along the suppression axis do not both
+        # exist are zero: rows 0 and h-1 always, columns 0 and w-1 for the
+        # diagonal-capable branch used by orientation 0
+        on = (coord == crest)
+        on[[0, -1], :] = False
+        if ori == 0:
+            on[:, [0, -1]] = False
+        assert (nm[[0, -1], :] == 0).all()
+        np.testing.assert_allclose(nm[on], np.minimum(np.float32(1.2) * pb[ori][on], 1), atol=1e-6)
+        assert not nm[~on].any()
+
+
+def test_gpb_closed_form():
+    w, h = 12, 9
+    cg = RNG.random((8, h, w)).astype(np.float32)
+    spb = RNG.random((8, h, w)).astype(np.float32) * 10
+    nm = RNG.random((h, w)).astype(np.float32) * 0.1
+    all_, g = O.gpb(cg, spb, nm)
+    want_all = cg + np.float32(0.0139) * spb
+    np.testing.assert_allclose(all_, want_all, rtol=1e-6)
+    np.testing.assert_allclose(g, np.where(nm > 0.05, want_all.max(0), 0), rtol=1e-6)
+
+
+# --------------------------------------------------------------- affinities / matrix
+def test_intervene_flat_image_and_symmetry():
+    w, h = 20, 16
+    W0 = O.intervene(np.zeros((h, w), np.float32))
+    C = 1 / (1 + np.exp(-1.8682))
+    off, _ = O.stencil(5, w)
+    lin = np.cumsum(off[:81])
+    centre = int(np.where(lin == 0)[0][0])
+    assert np.allclose(W0[centre], 1.0)
+    inner = W0.reshape(81, h, w)[:, 6:-6, 6:-6]
+    mask = np.arange(81) != centre
+    np.testing.assert_allclose(inner[mask], np.exp(-(1 - C) / 0.1), rtol=1e-5)
+    mpb = np.zeros((h, w), np.float32)
+    mpb[:, 9] = 1.0
+    Wm = O.intervene(mpb)
+    A = dia_to_dense(Wm, w, h)
+    np.testing.assert_allclose(A, A.T, atol=1e-7)
+    # an edge between two pixels lowers their affinity
+    assert A[5 * w + 7, 5 * w + 11] < A[5 * w + 2, 5 * w + 6]
+
+
+def dia_to_dense(M, w, h):
+    n = w * h
+    off, _ = O.stencil(5, w)
+    lin = np.cumsum(off[:81])
+    A = np.zeros((n, n))
+    for d in range(81):
+        for r in range(n):
+            c = r + lin[d]
+            if 0 <= c < n:
+                A[r, c] = M[d][r]
+    return A
+
+
+def test_convert_matrix_and_spmv():
+    w, h = 14, 11
+    mpb = RNG.random((h, w)).astype(np.float32) * 0.5
+    Wm = O.intervene(mpb)
+    A = dia_to_dense(Wm, w, h)
+    lsym, d, rs = O.convert_matrix(w, h, Wm)
+    L = dia_to_dense(lsym, w, h)
+    np.testing.assert_allclose(d, A.sum(1), rtol=1e-5)
+    np.testing.assert_allclose(L, L.T, atol=1e-6)
+    # L_sym d^1/2 = 0 and L_sym = I - D^-1/2 W D^-1/2
+    sq = np.sqrt(A.sum(1))
+    assert np.abs(L @ sq).max() < 1e-5 * np.abs(L).max() * sq.max()
+    np.testing.assert_allclose(L, np.eye(len(sq)) - A / np.outer(sq, sq), atol=2e-6)
+    x = RNG.random(w * h).astype(np.float32)
+    np.testing.assert_allclose(O.spmv(w, h, lsym, x), L @ x, atol=2e-5)
+
+
+# --------------------------------------------------------------- eigen
+def test_tridiagonal_helpers_against_lapack():
+    n = 300
+    a = RNG.standard_normal(n)
+    b = RNG.standard_normal(n - 1)
+    w = O.tridiag_eigvals(a, b, 1, 20)
+    ref = sl.eigh_tridiagonal(a, b, select="i", select_range=(0, 19), eigvals_only=True)
+    np.testing.assert_allclose(w, ref, atol=1e-13)
+    v = O.tridiag_eigvec(a, b, ref[3])
+    _, U = sl.eigh_tridiagonal(a, b, select="i", select_range=(3, 3))
+    assert abs(abs(v @ U[:, 0]) - 1) < 1e-10
+
+
+def test_lanczos_fp64_against_eigsh():
+    w, h = 30, 24
+    mpb = np.zeros((h, w), np.float32)
+    mpb[:, 11] = 0.9
+    mpb[13, :] = 0.8
+    lsym, d, rs = O.convert_matrix(w, h, O.intervene(mpb))
+    rc, vals, vecs, it = O.lanczos_fp64(w, h, lsym, nvec=6, tol=1e-10, maxiter=720)
+    assert rc == 0
+    L = sp.csr_matrix(dia_to_dense(lsym, w, h))
+    ref = np.sort(sla.eigsh(L, k=6, sigma=-1e-3, which="LM")[0])
+    np.testing.assert_allclose(vals, ref, atol=1e-9)
+    res = np.abs(L @ vecs.T - vecs.T * vals).max()
+    assert res < 1e-8
+
+
+def test_spectral_pb_direct_convolution():
+    w, h, nvec = 11, 9, 4
+    vals = np.float32([0.0, 0.01, 0.02, 0.05])
+    vecs = RNG.random((nvec, h, w)).astype(np.float32)
+    out = O.spectral_pb(vals, vecs, w, h).astype(np.float64)
+    K = O.spb_filters().reshape(8, 7, 7).astype(np.float64)  # [o][dy][dx]
+    want = np.zeros((8, h, w))
+    for i in range(1, nvec):
+        p = np.pad(vecs[i].astype(np.float64), 3, mode="symmetric")
+        for o in range(8):
+            r = np.zeros((h, w))
+            for dy in range(7):
+                for dx in range(7):
+                    r += K[o, dy, dx] * p[dy:dy + h, dx:dx + w]
+            want[o] += np.abs(r) / math.sqrt(vals[i])
+    np.testing.assert_allclose(out, want, rtol=2e-5, atol=1e-5)

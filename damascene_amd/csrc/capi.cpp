@@ -1,7 +1,9 @@
 // capi.cpp -- extern "C" entry points of libdmsc.so (include/dmsc.h).
+#include <cctype>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "dmsc_internal.hpp"
 #include "kernels.hpp"
@@ -157,6 +159,52 @@ int dmsc_stencil(int radius, int width, int* offsets, int* diag) {
 int dmsc_gamma_lut(float gamma, float* lut) {
   DMSC_ARG(lut, "null output");
   gamma_lut(gamma, lut);
+  return DMSC_OK;
+}
+
+// convert.cu:88-95 (cutLoadPPM4ub packing: R in the low byte).
+int dmsc_load_ppm(const char* path, int* w, int* h, uint32_t* rgb) {
+  DMSC_ARG(path && w && h, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(DMSC_ERR_ARG, "cannot open %s", path);
+  auto token = [&](int* out) -> bool {  // next integer, skipping whitespace and # comments
+    int c = std::fgetc(f);
+    for (;;) {
+      while (c != EOF && std::isspace(c)) c = std::fgetc(f);
+      if (c != '#') break;
+      while (c != EOF && c != '\n') c = std::fgetc(f);
+    }
+    if (c == EOF || !std::isdigit(c)) return false;
+    long v = 0;
+    while (c != EOF && std::isdigit(c)) {
+      v = v * 10 + (c - '0');
+      if (v > (1L << 20)) return false;
+      c = std::fgetc(f);
+    }
+    *out = (int)v;
+    return true;  // the single whitespace after the number has been consumed
+  };
+  char magic[2] = {0, 0};
+  int pw = 0, ph = 0, maxval = 0;
+  const bool ok = std::fread(magic, 1, 2, f) == 2 && magic[0] == 'P' && magic[1] == '6' && token(&pw) &&
+                  token(&ph) && token(&maxval);
+  if (!ok || pw <= 0 || ph <= 0 || maxval != 255) {
+    std::fclose(f);
+    return fail(DMSC_ERR_ARG, "%s: not a binary PPM with maxval 255", path);
+  }
+  *w = pw;
+  *h = ph;
+  if (rgb) {
+    std::vector<unsigned char> px((size_t)pw * ph * 3);
+    const size_t got = std::fread(px.data(), 1, px.size(), f);
+    if (got != px.size()) {
+      std::fclose(f);
+      return fail(DMSC_ERR_ARG, "%s: truncated pixel data", path);
+    }
+    for (size_t i = 0; i < (size_t)pw * ph; i++)
+      rgb[i] = (uint32_t)px[3 * i] | ((uint32_t)px[3 * i + 1] << 8) | ((uint32_t)px[3 * i + 2] << 16);
+  }
+  std::fclose(f);
   return DMSC_OK;
 }
 

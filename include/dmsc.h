@@ -82,6 +82,10 @@ int dmsc_halfdisc_offsets(int w, int h, float thetaPi, int radius, const int* h_
 int dmsc_blur_kernel(float sigma, float* h_kernel /*<=64*/, int* radius); /* rotate.cu:29 */
 int dmsc_stencil(int radius, int width, int* h_offsets, int* h_diag_map); /* Stencil.cpp:6 */
 int dmsc_gamma_lut(float gamma, float* h_lut /*256*/);
+/* convert.cu:88-95 loadPPM_rgbU (cutLoadPPM4ub packing): binary PPM (P6,
+ * maxval 255) -> host uint32 per pixel, R in bits 0-7, G 8-15, B 16-23.
+ * h_rgb may be NULL to query the size. */
+int dmsc_load_ppm(const char* path, int* w, int* h, uint32_t* h_rgb);
 
 /* ------------------------------------------------------------------ colour (convert.h) */
 int dmsc_rgb_to_grey(dmsc_ctx* ctx, int w, int h, const uint32_t* d_rgb, float* d_grey);          /* convert.h:53 */
@@ -146,9 +150,11 @@ int dmsc_stencil_spmv(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int ma
  * matrix W (overwritten by L_sym).  h_eigvals[nvec] (host), d_eigvecs
  * [nvec][N] (device, vectors 1.. min-max normalised as the reference).
  * precision: DMSC_PREC_FP32 = reference algorithm (FP32 Lanczos without
- * reorthogonalisation, Cullum screening every 500 steps); DMSC_PREC_MIXED =
- * FP32 SpMV/vectors with FP64 inner products, full reorthogonalisation and
- * FP64 tridiagonal solve.  Tridiagonal work runs on the GPU in both. */
+ * reorthogonalisation, Cullum screening every 500 steps, residual test at
+ * tol); DMSC_PREC_MIXED = FP32 matrix and FP32-stored basis with FP64 SpMV
+ * accumulation, three-term vector and inner products, Cullum screening on
+ * an adaptive schedule, residuals at tol * 1e-3.  The FP64 tridiagonal
+ * eigensolve runs on the GPU in both. */
 int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch,
                                 int nvec, float tol, int precision, float* h_eigvals,
                                 float* d_eigvecs, int* iterations);

@@ -151,3 +151,80 @@ def test_bench_refuses_several_gpus():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+def test_ppm_loader_packs_rgb(tmp_path):
+    w, h = 7, 5
+    px = np.random.default_rng(3).integers(0, 256, size=(h, w, 3), dtype=np.uint8)
+    path = tmp_path / "t.ppm"
+    path.write_bytes(b"P6\n# comment\n7 5\n255\n" + px.tobytes())
+    W, H = ctypes.c_int(), ctypes.c_int()
+    assert lib().dmsc_load_ppm(str(path).encode(), ctypes.byref(W), ctypes.byref(H), None) == 0
+    assert (W.value, H.value) == (w, h)
+    out = np.zeros((h, w), np.uint32)
+    assert lib().dmsc_load_ppm(str(path).encode(), ctypes.byref(W), ctypes.byref(H), P(out)) == 0
+    want = px[..., 0].astype(np.uint32) | (px[..., 1].astype(np.uint32) << 8) | (px[..., 2].astype(np.uint32) << 16)
+    np.testing.assert_array_equal(out, want)
+    bad = tmp_path / "bad.ppm"
+    bad.write_bytes(b"P5\n7 5\n255\n" + px.tobytes())
+    assert lib().dmsc_load_ppm(str(bad).encode(), ctypes.byref(W), ctypes.byref(H), None) != 0
+
+
+STAGE_API_PROGRAM = r"""
+#include <cstdio>
+#include <stdexcept>
+#include "Stencil.h"
+#include "combine.h"
+#include "convert.h"
+#include "globalPb.h"
+#include "intervening.h"
+#include "kmeans.h"
+#include "lanczos.h"
+#include "localcues.h"
+#include "nonmax.h"
+#include "spectralPb.h"
+#include "stencilMVM.h"
+#include "texton.h"
+int main() {
+  Stencil s(5, 481, 321, findPitchInFloats(481 * 321));
+  int offs[441], diag[441];
+  s.copyOffsets(offs);
+  s.copyDiagonalOffsets(diag);
+  long lin = 0;
+  for (unsigned i = 0; i < s.getStencilArea(); i++) lin += offs[i];
+  // last diagonal is (dy, dx) = (5, 0); entry (row, row + 5w) sits on it
+  int idx = s.matrixIndex(1000, 1000 + 5 * 481);
+  std::printf("area %u pitch %d last %ld index %d\n", s.getStencilArea(), s.getMatrixPitchInFloats(), lin, idx);
+  try {
+    int* t = nullptr;
+    findTextons(481, 321, nullptr, &t, 1);
+  } catch (const std::runtime_error& e) {
+    std::printf("error: %s\n", e.what());
+    return 0;
+  }
+  return 1;
+}
+"""
+
+
+def test_reference_stage_api_compiles_links_and_reports_errors(tmp_path):
+    """The reference-named C++ headers build with the host compiler alone and
+    link against libdmsc.so; without a device a stage call throws."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    src = tmp_path / "use_api.cpp"
+    src.write_text(STAGE_API_PROGRAM)
+    exe = tmp_path / "use_api"
+    libdir = os.path.join(ROOT, "damascene_amd")
+    cmd = ["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I",
+           "/opt/rocm/include", str(src), "-o", str(exe), "-L", libdir, "-ldmsc", f"-Wl,-rpath,{libdir}"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pitch = dm.pitch_in_floats(481 * 321)
+    assert f"area 81 pitch {pitch} last {5 * 481} index {pitch * 80 + 1000}" in r.stdout
+    assert "error:" in r.stdout

@@ -1,17 +1,26 @@
 """gPb throughput on one MI355X at 481x321 (synthetic Voronoi/grating image).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--check]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 A step is one full gPb pass (RGB8 resident in HBM -> oriented gPb) over one
-481x321 image.  `value` is images/s of the mixed-precision path; the plain
-FP32 path (the reference algorithm) is timed beside it.  The roofline object
-covers the stencil SpMV (the kernel inside every Lanczos step), timed with
-HIP events on the library's stream; the CPU baseline is the oracle's FP32
-reference-order pipeline on one image on the host cores.
+481x321 image.  `value` is images/s of the mixed-precision path (FP64-
+accumulated eigensolver converged to residual tol*1e-3); the plain FP32 path
+(the reference algorithm) is timed beside it, and both report their errors
+against the FP64 ground truth in tests/golden/truth_481x321.npz (eigenvalue
+relative error, subspace angle of eigenvectors 1..8, gPb max-abs error,
+nonmax mismatches against the oracle).
+
+roofline: the stencil SpMV (81 diagonals, 332 algorithmic bytes per row),
+average launch duration measured with HIP events on the library's stream
+after a sweep of its launch shapes; `traffic` is the HBM bytes per launch
+from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed under profiles/.
+cpu_baseline: the oracle's reference-order pipeline on one image on the host
+cores (OpenMP).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -20,19 +29,21 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-W, H, CELLS, SEED = 481, 321, 40, 20261202
+W, H, CELLS, SEED, NVEC, TOL = 481, 321, 40, 20261202, 9, 1e-3
 HBM_PEAK_GBS = 8000.0
 SPMV_BYTES_PER_ROW = 81 * 4 + 4 + 4  # 81 matrix entries + x + y (algorithmic)
+SPMV_SHAPES = {0: "k_spmv_full", 1: "k_spmv<1>", 2: "k_spmv<2>", 4: "k_spmv<4>"}
+TRUTH = os.path.join(ROOT, "tests", "golden", "truth_481x321.npz")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_spmv_full_481x321_pmc.txt")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--spmv-reps", type=int, default=200)
-    p.add_argument("--check", action="store_true", help="also report errors vs the FP64 oracle")
     return p.parse_args()
 
 
@@ -46,39 +57,80 @@ def time_pipeline(ctx, d_rgb, precision, steps, warmup):
     for _ in range(steps):
         out = ctx.gpb(d_rgb, precision=precision)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    return (t1 - t0) / steps, out
+    return (time.perf_counter() - t0) / steps, out
 
 
-def spmv_roofline(ctx, out, reps):
-    """Average duration of the stencil SpMV on this image's L_sym, HIP events
-    on the library stream (the stream every dmsc call is enqueued on)."""
+def spmv_sweep(ctx, nmax, reps):
+    """Average duration of each SpMV launch shape (HIP events on the library
+    stream, the stream every dmsc call is enqueued on); leaves the fastest
+    shape selected and returns the roofline object for it."""
     import torch
 
-    import damascene_amd as dm
+    from damascene_amd._lib import check, lib
 
     n = W * H
-    m = ctx.intervene(out["nmax"])
+    m = ctx.intervene(nmax)
     ctx.convert_matrix(W, H, m)
     x = torch.rand(n, device=ctx.device)
     y = torch.empty(n, device=ctx.device)
-    for _ in range(5):
-        y = ctx.spmv(W, H, m, x)
-    stream = ctx.stream
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        ctx.spmv(W, H, m, x)
-    e1.record(stream)
-    e1.synchronize()
-    avg_s = e0.elapsed_time(e1) / 1e3 / reps
-    algo_bytes = SPMV_BYTES_PER_ROW * n
-    achieved = algo_bytes / avg_s / 1e9
-    return {
-        "kernel": "k_spmv (stencil SpMV, 81 diagonals)",
+    spmv = lib().dmsc_stencil_spmv
+    args = (ctx.handle, W, H, m.data_ptr(), m.shape[1], x.data_ptr(), y.data_ptr())
+    times = {}
+    for shape in SPMV_SHAPES:
+        check(lib().dmsc_tune_set(b"spmv_rows_per_lane", shape))
+        for _ in range(5):
+            check(spmv(*args))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        for _ in range(reps):  # launches are queued faster than they run
+            spmv(*args)
+        e1.record(ctx.stream)
+        e1.synchronize()
+        times[shape] = e0.elapsed_time(e1) / 1e3 / reps
+    best = min(times, key=times.get)
+    check(lib().dmsc_tune_set(b"spmv_rows_per_lane", best))
+    avg_s = times[best]
+    algo = SPMV_BYTES_PER_ROW * n
+    achieved = algo / avg_s / 1e9
+    roof = {
+        "kernel": SPMV_SHAPES[best] + " (stencil SpMV, 81 diagonals)",
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "avg_us": round(avg_s * 1e6, 3), "bytes_per_launch": algo_bytes,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(SPMV_SHAPES[best]),
+        "avg_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": algo,
+        "shapes_us": {SPMV_SHAPES[k]: round(v * 1e6, 3) for k, v in times.items()},
+    }
+    return roof
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    text = open(PMC_SUMMARY).read()
+    m = re.search(r"^traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+(\d+)", text, re.M)
+    return int(m.group(1)) if m else None
+
+
+def errors_vs_truth(out, truth):
+    """Eigenvalue relative error, largest principal angle between the spans of
+    u_1..u_8 (u = D^1/2 v, with the D^1/2 direction projected out, so the
+    per-vector min-max map does not matter) and gPb max-abs error."""
+    vals = np.asarray(out["eigvals"], np.float64)
+    tv = truth["eigvals"][:NVEC]
+    sq = truth["sqrt_d"].astype(np.float64)
+    u0 = sq / np.linalg.norm(sq)
+    G = out["eigvecs"].cpu().numpy().reshape(NVEC, -1)[1:].astype(np.float64) * sq[None, :]
+    G -= np.outer(G @ u0, u0)
+    R = truth["u"].astype(np.float64)
+    R -= np.outer(R @ u0, u0)
+    qa, _ = np.linalg.qr(G.T)
+    qb, _ = np.linalg.qr(R.T)
+    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
+    return {
+        "eigval_max_rel": float(np.max(np.abs(vals[1:] - tv[1:]) / tv[1:])),
+        "subspace_angle": float(np.arccos(np.clip(s.min(), -1, 1))),
+        "gpb_max_abs": float(np.abs(out["gpb"].cpu().numpy() - truth["gpb"]).max()),
     }
 
 
@@ -87,11 +139,11 @@ def cpu_baseline(rgb):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
-    r = O.pipeline(rgb)
+    r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
     dt = time.perf_counter() - t0
     return {
         "value": round(1.0 / dt, 4), "unit": "images/s", "cores": threads, "kind": "port",
-        "sample": f"one {W}x{H} image, full plain-FP32 gPb in reference order (oracle), {dt:.2f} s",
+        "sample": f"one {W}x{H} image, whole plain-FP32 gPb in reference order (oracle, OpenMP), {dt:.2f} s",
         "stage_ms": [round(v, 1) for v in r["stage_ms"]],
     }, r
 
@@ -112,7 +164,8 @@ def main():
     t_mixed, out_mixed = time_pipeline(ctx, d_rgb, dm.PREC_MIXED, a.steps, a.warmup)
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
-    roof = spmv_roofline(ctx, out_plain, a.spmv_reps)
+    roof = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
+    torch.cuda.synchronize()
     res = {
         "metric": "gPb images/s @481x321 (mixed vs plain FP32, 1 GPU); SpMV GB/s; err vs FP64",
         "value": round(1.0 / t_mixed, 3),
@@ -124,10 +177,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (mixed: f64 Lanczos inner products)",
+        "dtype": "f32 (mixed: f64 Lanczos accumulation)",
         "data": "synthetic (Voronoi cells + gratings + noise, seed 20261202)",
         "config": {"workload": f"gpb-{W}x{H}-single-image", "width": W, "height": H, "cells": CELLS,
-                   "n_eig": 9, "tol": 1e-3, "textons": 32},
+                   "n_eig": NVEC, "tol": TOL, "textons": 32},
         "plain_fp32": {"images_per_s": round(1.0 / t_plain, 3), "ms_per_image": round(t_plain * 1e3, 3),
                        "stage_ms": [round(v, 3) for v in stage_plain["stage_ms"]],
                        "lanczos_iterations": stage_plain["lanczos_iterations"]},
@@ -138,18 +191,19 @@ def main():
         "stage_names": ["textons", "localcues", "combine+nonmax", "intervene", "eigensolve", "spectralPb+gPb"],
         "roofline": roof,
     }
-    ref = None
+    if os.path.exists(TRUTH):
+        truth = np.load(TRUTH)
+        res["plain_fp32"]["err_vs_fp64"] = errors_vs_truth(out_plain, truth)
+        res["mixed"]["err_vs_fp64"] = errors_vs_truth(out_mixed, truth)
     if not a.no_cpu:
         res["cpu_baseline"], ref = cpu_baseline(rgb)
-    if ref is not None:
-        n = W * H
-        res["parity_vs_oracle"] = {
-            "textons_mismatch": int(np.count_nonzero(out_plain["textons"].cpu().numpy() != ref["textons"])),
-            "nonmax_mismatch": int(np.count_nonzero(
-                out_plain["nmax"].cpu().numpy().view(np.uint32) != ref["nmax"].view(np.uint32))),
-            "gpb_maxabs_plain": float(np.abs(out_plain["gpb"].cpu().numpy() - ref["gpb"]).max()),
-            "gpb_maxabs_mixed": float(np.abs(out_mixed["gpb"].cpu().numpy() - ref["gpb"]).max()),
-        }
+        for name, out in (("plain_fp32", out_plain), ("mixed", out_mixed)):
+            res[name]["textons_mismatch_vs_oracle"] = int(np.count_nonzero(out["textons"].cpu().numpy()
+                                                                           != ref["textons"]))
+            res[name]["nonmax_mismatch_vs_oracle"] = int(np.count_nonzero(
+                out["nmax"].cpu().numpy().view(np.uint32) != ref["nmax"].view(np.uint32)))
+        res["plain_fp32"]["gpb_max_abs_vs_oracle_plain"] = float(
+            np.abs(out_plain["gpb"].cpu().numpy() - ref["gpb"]).max())
     print(json.dumps(res))
 
 

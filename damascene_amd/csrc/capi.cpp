@@ -162,6 +162,24 @@ int dmsc_gamma_lut(float gamma, float* lut) {
   return DMSC_OK;
 }
 
+int dmsc_tune_set(const char* key, int value) {
+  DMSC_ARG(key, "null key");
+  if (!std::strcmp(key, "spmv_rows_per_lane")) {
+    DMSC_ARG(value == 0 || value == 1 || value == 2 || value == 4, "spmv_rows_per_lane must be 0, 1, 2 or 4");
+    spmv_rows_per_lane = value;
+    return DMSC_OK;
+  }
+  return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
+}
+int dmsc_tune_get(const char* key, int* value) {
+  DMSC_ARG(key && value, "null argument");
+  if (!std::strcmp(key, "spmv_rows_per_lane")) {
+    *value = spmv_rows_per_lane;
+    return DMSC_OK;
+  }
+  return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
+}
+
 // convert.cu:88-95 (cutLoadPPM4ub packing: R in the low byte).
 int dmsc_load_ppm(const char* path, int* w, int* h, uint32_t* rgb) {
   DMSC_ARG(path && w && h, "null argument");

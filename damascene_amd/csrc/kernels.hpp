@@ -83,9 +83,8 @@ __device__ __forceinline__ void spmv_rows(float (&acc)[RPL], const float* __rest
   }
 }
 
-// Launch shapes chosen by the tuner (rows per lane: 1, 2 or 4).
+// SpMV launch shape chosen by the tuner (0: k_spmv_full; 1, 2, 4 rows per lane).
 extern int spmv_rows_per_lane;
-extern int lanczos_rows_per_lane;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,

@@ -43,8 +43,6 @@
 
 namespace dmsc {
 
-int lanczos_rows_per_lane = 1;  // kept for the tuner interface; K1 runs one row per lane
-
 namespace {
 
 constexpr int kGraphSteps = 25;   // steps per captured graph; periods are multiples of it
@@ -64,8 +62,7 @@ constexpr int kMaxGraphs = 16;
 
 // Device-side solver state.
 struct LzState {
-  int k;                 // index of the next step (read by K1, advanced by K2)
-  int k2;                // step index handed from K1 to K2
+  int k;                 // first step of the chunk being replayed
   float* V;              // basis: q_j at V + j * vstride
   int nscreened;         // screened Ritz values found
   int fail;              // 1: fewer than nvec screened values, 2: inverse iteration failed
@@ -105,45 +102,45 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
-// Z: type of the three-term vector, alpha and beta (float: plain, double:
-// mixed); inner products always accumulate in double.
-template <typename Z>
+// Step index of a kernel inside a captured chunk: the host sets st->k to
+// the chunk's first step before each replay and node t of the chunk carries
+// t as an argument, so no kernel updates the index while others read it.
+
+// ---- plain path: FP32 vectors, alpha and beta; two kernels per step.
 struct LzBufs {
   const float* A;
   size_t pitch;
   int n;
   size_t vstride;
-  Z* z0;          // z of even steps
-  Z* z1;          // z of odd steps (holds q_0 before step 0)
+  float* z0;      // z of even steps
+  float* z1;      // z of odd steps (holds q_0 before step 0)
   double* part;   // [nb1] q.z partials of K1's blocks
   double* part2;  // [nb2] |z|^2 partials of K2's blocks
-  Z* alpha;       // [kMaxIter + 2]
-  Z* beta;
+  float* alpha;   // [kMaxIter + 2]
+  float* beta;
   int nb1, nb2;
   LzState* st;
 };
 
-template <typename Z>
-__global__ __launch_bounds__(kK1) void k_lz1(LzBufs<Z> b, StencilOffsets so) {
+__global__ __launch_bounds__(kK1) void k_lz1(LzBufs b, StencilOffsets so, int t) {
   __shared__ double red[kK1 / kWave];
-  const int k = b.st->k;
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.st->k2 = k;
+  const int k = b.st->k + t;
   float* V = b.st->V;
-  const Z* zp = (k & 1) ? b.z0 : b.z1;
-  Z* zc = (k & 1) ? b.z1 : b.z0;
-  Z inv = 1, nbeta = 0;
+  const float* zp = (k & 1) ? b.z0 : b.z1;
+  float* zc = (k & 1) ? b.z1 : b.z0;
+  float inv = 1, nbeta = 0;
   if (k > 0) {
-    const Z beta = (Z)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
+    const float beta = (float)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
     if (blockIdx.x == 0 && threadIdx.x == 0) b.beta[k - 1] = beta;
-    inv = (Z)1 / beta;
+    inv = 1.0f / beta;
     nbeta = -beta;
   }
-  // q_k[c] = z_{k-1}[c] / beta_{k-1} rounded to FP32: the value the basis
+  // q_k[c] = z_{k-1}[c] * (1 / beta_{k-1}) in FP32: the value the basis
   // stores (step 0 reads q_0 from z1 with beta 1)
   const int row = blockIdx.x * kK1 + threadIdx.x;
   double dot = 0;
   if (row < b.n) {
-    const float qv = (float)(zp[row] * inv);
+    const float qv = zp[row] * inv;
     V[(size_t)k * b.vstride + row] = qv;
     float a[kStencilArea], x[kStencilArea];
 #pragma unroll
@@ -152,36 +149,26 @@ __global__ __launch_bounds__(kK1) void k_lz1(LzBufs<Z> b, StencilOffsets so) {
     for (int d = 0; d < kStencilArea; d++) {
       const int col = row + so.off[d];
       const bool in = (unsigned)col < (unsigned)b.n;
-      const float xv = (float)(zp[in ? col : 0] * inv);
+      const float xv = zp[in ? col : 0] * inv;
       x[d] = in ? xv : 0.0f;  // fma(a, 0, acc) == acc: the reference's bounds test
     }
-    Z zj;
-    if constexpr (sizeof(Z) == 4) {
-      float acc = 0.0f;  // diagonal order, as stencilMVM.cu:127-180
+    float acc = 0.0f;  // diagonal order, as stencilMVM.cu:127-180
 #pragma unroll
-      for (int d = 0; d < kStencilArea; d++) acc = fmaf(a[d], x[d], acc);
-      zj = acc;
-    } else {
-      double acc = 0.0;  // FP64 accumulation of exact FP32 products
-#pragma unroll
-      for (int d = 0; d < kStencilArea; d++) acc = fma((double)a[d], (double)x[d], acc);
-      zj = acc;
-    }
-    if (k > 0) zj = fma_t(nbeta, (Z)V[(size_t)(k - 1) * b.vstride + row], zj);
-    zc[row] = zj;
-    dot = (double)qv * (double)zj;  // exact product
+    for (int d = 0; d < kStencilArea; d++) acc = fmaf(a[d], x[d], acc);
+    if (k > 0) acc = fmaf(nbeta, V[(size_t)(k - 1) * b.vstride + row], acc);
+    zc[row] = acc;
+    dot = (double)qv * (double)acc;  // exact product
   }
   const double s = block_sum<kK1>(dot, red);
   if (threadIdx.x == 0) b.part[blockIdx.x] = s;
 }
 
-template <typename Z>
-__global__ __launch_bounds__(kK2) void k_lz2(LzBufs<Z> b) {
+__global__ __launch_bounds__(kK2) void k_lz2(LzBufs b, int t) {
   __shared__ double red[kK2 / kWave];
-  const int k = b.st->k2;
+  const int k = b.st->k + t;
   const float* qk = b.st->V + (size_t)k * b.vstride;
-  Z* zc = (k & 1) ? b.z1 : b.z0;
-  const Z alpha = (Z)sum_partials<kK2>(b.part, b.nb1, red);
+  float* zc = (k & 1) ? b.z1 : b.z0;
+  const float alpha = (float)sum_partials<kK2>(b.part, b.nb1, red);
   if (blockIdx.x == 0 && threadIdx.x == 0) b.alpha[k] = alpha;
   double nrm = 0;
   const int base = blockIdx.x * kK2 * kK2Rows + threadIdx.x;
@@ -189,28 +176,122 @@ __global__ __launch_bounds__(kK2) void k_lz2(LzBufs<Z> b) {
   for (int j = 0; j < kK2Rows; j++) {
     const int i = base + j * kK2;
     if (i < b.n) {
-      const Z zj = fma_t(-alpha, (Z)qk[i], zc[i]);
+      const float zj = fmaf(-alpha, qk[i], zc[i]);
       zc[i] = zj;
       nrm = fma((double)zj, (double)zj, nrm);
     }
   }
   const double s = block_sum<kK2>(nrm, red);
+  if (threadIdx.x == 0) b.part2[blockIdx.x] = s;
+}
+
+// beta of the last step m-1 (needed by the check's residuals).
+__global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs b, int m) {
+  __shared__ double red[kK1 / kWave];
+  const float beta = (float)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
   if (threadIdx.x == 0) {
-    b.part2[blockIdx.x] = s;
-    // K2's blocks read k from k2, so block 0 may advance k for the next K1
-    if (blockIdx.x == 0) b.st->k = k + 1;
+    b.beta[m - 1] = beta;
+    b.st->m = m;
   }
 }
 
-// beta of the step just finished (needed by the check's residuals).
-template <typename Z>
-__global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs<Z> b) {
+// ---- mixed path: FP32 matrix and basis, FP64 accumulation and vectors; one
+// kernel per step.  Kernel k keeps z'_k = A q_k - beta_{k-1} q_{k-1} (before
+// the alpha correction) and partial sums of q_k.z'_k, |z'_k|^2 and |q_k|^2.
+// Kernel k+1 forms alpha_k = sum q.z', and, since z_k = z'_k - alpha_k q_k,
+//   beta_k^2 = |z_k|^2 = sum z'^2 - 2 alpha_k sum q.z' + alpha_k^2 sum q^2
+//            = sum z'^2 + alpha_k^2 (sum q^2 - 2),
+// which in FP64 loses at most eps64 |z'|^2 / beta^2 (< 1e-12 here), then
+// gathers q_{k+1}[c] = (z'_k[c] - alpha_k q_k[c]) / beta_k on the fly.  The
+// basis keeps a zero row before q_0 so step 0 needs no special case.
+struct LzmBufs {
+  const float* A;
+  size_t pitch;
+  int n;
+  size_t vstride;
+  double* z0;  // z' of even steps
+  double* z1;  // z' of odd steps (holds q_0 before step 0)
+  double* p1;  // [nb] q.z' partials
+  double* p2;  // [nb] |z'|^2 partials
+  double* p3;  // [nb] |q|^2 partials
+  double* alpha;
+  double* beta;
+  int nb;
+  LzState* st;
+};
+
+// alpha and beta of step k-1 from the partials of kernel k-1 (all blocks
+// reduce in the same order).
+__device__ __forceinline__ void lzm_coeffs(const LzmBufs& b, double* red, double& alpha, double& beta) {
+  double v1 = 0, v2 = 0, v3 = 0;
+  for (int i = threadIdx.x; i < b.nb; i += kK1) {
+    v1 += b.p1[i];
+    v2 += b.p2[i];
+    v3 += b.p3[i];
+  }
+  alpha = block_sum<kK1>(v1, red);
+  const double s2 = block_sum<kK1>(v2, red), s3 = block_sum<kK1>(v3, red);
+  beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
+}
+
+__global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t) {
   __shared__ double red[kK1 / kWave];
-  const int k = b.st->k - 1;
-  const Z beta = (Z)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
+  const int k = b.st->k + t;
+  float* V = b.st->V;  // row k at V + k * vstride; row -1 is zero
+  const double* zp = (k & 1) ? b.z0 : b.z1;
+  double* zc = (k & 1) ? b.z1 : b.z0;
+  double alpha, beta;
+  lzm_coeffs(b, red, alpha, beta);  // step k-1 (alpha 0, beta 1 before step 0)
+  if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    b.alpha[k - 1] = alpha;
+    b.beta[k - 1] = beta;
+  }
+  const double inv = 1.0 / beta;
+  const float* qm = V + (size_t)(k - 1) * b.vstride;  // q_{k-1}
+  auto q_at = [&](int c) -> float { return (float)(fma(-alpha, (double)qm[c], zp[c]) * inv); };
+  const int row = blockIdx.x * kK1 + threadIdx.x;
+  double d1 = 0, d2 = 0, d3 = 0;
+  if (row < b.n) {
+    const float qv = q_at(row);
+    V[(size_t)k * b.vstride + row] = qv;
+    float a[kStencilArea], x[kStencilArea];
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) a[d] = b.A[(size_t)d * b.pitch + row];
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) {
+      const int col = row + so.off[d];
+      const bool in = (unsigned)col < (unsigned)b.n;
+      const float xv = q_at(in ? col : 0);
+      x[d] = in ? xv : 0.0f;
+    }
+    double acc = 0.0;  // FP64 accumulation of exact FP32 products
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) acc = fma((double)a[d], (double)x[d], acc);
+    const double zj = fma(-beta, (double)qm[row], acc);
+    zc[row] = zj;
+    d1 = (double)qv * zj;
+    d2 = zj * zj;
+    d3 = (double)qv * (double)qv;
+  }
+  d1 = block_sum<kK1>(d1, red);
+  d2 = block_sum<kK1>(d2, red);
+  d3 = block_sum<kK1>(d3, red);
   if (threadIdx.x == 0) {
-    b.beta[k] = beta;
-    b.st->m = k + 1;
+    b.p1[blockIdx.x] = d1;
+    b.p2[blockIdx.x] = d2;
+    b.p3[blockIdx.x] = d3;
+  }
+}
+
+// alpha and beta of the last step m-1, for the check.
+__global__ __launch_bounds__(kK1) void k_lzm_finish(LzmBufs b, int m) {
+  __shared__ double red[kK1 / kWave];
+  double alpha, beta;
+  lzm_coeffs(b, red, alpha, beta);
+  if (threadIdx.x == 0) {
+    b.alpha[m - 1] = alpha;
+    b.beta[m - 1] = beta;
+    b.st->m = m;
   }
 }
 
@@ -465,12 +546,13 @@ __global__ void k_normalize(int n, int nvec, const float* __restrict__ fin, floa
   }
 }
 
+// Basis row -1 (zeros), q_0's source in z1, and the step-(-1) partials of
+// the mixed path (alpha 0, |z'|^2 1, |q|^2 0 -> beta 1).
 template <typename Z>
-__global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n) {
+__global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n, double* p1, double* p2, double* p3, int nb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     st->k = 0;
-    st->k2 = 0;
     st->V = V;
     st->nscreened = 0;
     st->fail = 0;
@@ -480,13 +562,25 @@ __global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n) {
     V[i] = q0;
     z1[i] = (Z)q0;
   }
+  if (p1 && i < nb) {
+    p1[i] = 0.0;
+    p2[i] = i == 0 ? 1.0 : 0.0;
+    p3[i] = 0.0;
+  }
+}
+
+__global__ void k_zero_row(float* row, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) row[i] = 0.0f;
 }
 
 __global__ void k_set_basis(LzState* st, float* V) { st->V = V; }
+__global__ void k_set_step(LzState* st, int k) { st->k = k; }
 
 // ---------------------------------------------------------------- host
-// Captured check periods, per context; the kernels' arguments are baked into
-// the graph, so the key holds every pointer and size they depend on.
+// Captured chunks of kGraphSteps steps, per context; the kernels' arguments
+// are baked into the graph, so the key holds every pointer and size they
+// depend on.
 struct GraphKey {
   const void* A;
   const void* st;
@@ -514,26 +608,37 @@ std::map<const Ctx*, std::unique_ptr<GraphCache>>& graph_caches() {
   return g;
 }
 
-template <typename Z>
-void enqueue_steps(hipStream_t s, const LzBufs<Z>& b, const StencilOffsets& so, int steps) {
-  for (int i = 0; i < steps; i++) {
-    hipLaunchKernelGGL(k_lz1<Z>, dim3(b.nb1), dim3(kK1), 0, s, b, so);
-    hipLaunchKernelGGL(k_lz2<Z>, dim3(b.nb2), dim3(kK2), 0, s, b);
+struct Steps {
+  bool mixed;
+  LzBufs plain;
+  LzmBufs mix;
+  StencilOffsets so;
+};
+
+void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
+  for (int t = 0; t < steps; t++) {
+    if (p.mixed) {
+      hipLaunchKernelGGL(k_lzm, dim3(p.mix.nb), dim3(kK1), 0, s, p.mix, p.so, t);
+    } else {
+      hipLaunchKernelGGL(k_lz1, dim3(p.plain.nb1), dim3(kK1), 0, s, p.plain, p.so, t);
+      hipLaunchKernelGGL(k_lz2, dim3(p.plain.nb2), dim3(kK2), 0, s, p.plain, t);
+    }
   }
 }
 
-// `steps` Lanczos steps on ctx->stream through a cached HIP graph.
-template <typename Z>
-int run_steps(Ctx* ctx, const LzBufs<Z>& b, const StencilOffsets& so, int w, int steps) {
+// Steps first .. first+steps-1 on ctx->stream through a cached HIP graph.
+int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps) {
   auto& gc = graph_caches()[ctx];
   if (!gc) gc.reset(new GraphCache());
   if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
-  const GraphKey key{b.A, b.st, (int)b.pitch, b.n, w, (int)(sizeof(Z) == 8), steps};
+  const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
+                     (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
+                     steps};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
     DMSC_HIP(hipStreamBeginCapture(gc->cap, hipStreamCaptureModeThreadLocal));
-    enqueue_steps<Z>(gc->cap, b, so, steps);
+    enqueue_steps(gc->cap, p, steps);
     const hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(gc->cap, &g);
@@ -545,18 +650,23 @@ int run_steps(Ctx* ctx, const LzBufs<Z>& b, const StencilOffsets& so, int w, int
     DMSC_HIP(ie);
     it = gc->execs.emplace(key, exec).first;
   }
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, ctx->stream, st, first);
+  DMSC_HIP(hipGetLastError());
   DMSC_HIP(hipGraphLaunch(it->second, ctx->stream));
   return DMSC_OK;
 }
 
-template <typename Z>
-int launch_ritz(hipStream_t s, int nvec, int n, const LzState* st, size_t vstride, int m, const double* S,
-                float* out) {
+int launch_ritz(hipStream_t s, bool mixed, int nvec, int n, const LzState* st, size_t vstride, int m,
+                const double* S, float* out) {
   const dim3 g(div_up(n, 256)), b(256);
-  constexpr bool S32 = sizeof(Z) == 4;
   switch (nvec) {
-#define RITZ_CASE(NV) \
-  case NV: hipLaunchKernelGGL((k_ritz<NV, S32>), g, b, 0, s, n, st, vstride, m, S, out); break;
+#define RITZ_CASE(NV)                                                                                   \
+  case NV:                                                                                              \
+    if (mixed)                                                                                          \
+      hipLaunchKernelGGL((k_ritz<NV, false>), g, b, 0, s, n, st, vstride, m, S, out);                   \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_ritz<NV, true>), g, b, 0, s, n, st, vstride, m, S, out);                    \
+    break;
     RITZ_CASE(1) RITZ_CASE(2) RITZ_CASE(3) RITZ_CASE(4) RITZ_CASE(5) RITZ_CASE(6) RITZ_CASE(7) RITZ_CASE(8)
     RITZ_CASE(9) RITZ_CASE(10) RITZ_CASE(11) RITZ_CASE(12) RITZ_CASE(13) RITZ_CASE(14) RITZ_CASE(15)
     RITZ_CASE(16) RITZ_CASE(17) RITZ_CASE(18) RITZ_CASE(19) RITZ_CASE(20) RITZ_CASE(21) RITZ_CASE(22)
@@ -571,7 +681,7 @@ int launch_ritz(hipStream_t s, int nvec, int n, const LzState* st, size_t vstrid
 // Mixed path check schedule: the largest residual of the wanted Ritz pairs
 // falls roughly geometrically with the step count, so the next check goes
 // where the trend of the last two checks reaches the tolerance (scaled by
-// 0.7), at least kGraphSteps and at most m/4 steps ahead, in whole graphs.
+// 0.7), at least kGraphSteps and at most m/4 steps ahead, in whole chunks.
 int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, double& prev_res) {
   int steps = std::max(kGraphSteps, m / 4);
   if (hs.fail == 0) {
@@ -588,17 +698,16 @@ int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, dou
   return (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
 }
 
-template <typename Z>
 int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool mixed, float* h_eigvals,
           float* d_eigvecs, int* iterations) {
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const size_t vstride = (size_t)pitch_floats(n);
   const int nb1 = div_up(n, kK1), nb2 = div_up(n, kK2 * kK2Rows);
-  const StencilOffsets so = make_stencil_offsets(w);
   int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
   const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
   const int nblk = std::min(div_up(n, 256), 1024);
+  const size_t zb = mixed ? sizeof(double) : sizeof(float);
 
   // ---- solver state and small vectors (arena 2); offsets depend on n only
   size_t off = 0;
@@ -608,10 +717,11 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     return o;
   };
   const size_t o_st = carve(sizeof(LzState));
-  const size_t o_z0 = carve(sizeof(Z) * vstride), o_z1 = carve(sizeof(Z) * vstride);
+  const size_t o_z0 = carve(zb * vstride), o_z1 = carve(zb * vstride);
   const size_t o_rs = carve(sizeof(float) * vstride);
-  const size_t o_p = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb2);
-  const size_t o_al = carve(sizeof(Z) * (kMaxIter + 2)), o_be = carve(sizeof(Z) * (kMaxIter + 2));
+  const size_t o_p1 = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb1),
+               o_p3 = carve(sizeof(double) * nb1);
+  const size_t o_al = carve(zb * (kMaxIter + 2)), o_be = carve(zb * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
   const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);
   const size_t o_S = carve(sizeof(double) * kMaxEig * (kMaxIter + 2));
@@ -627,26 +737,19 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   double* wv = (double*)(B + o_w);
   double* S = (double*)(B + o_S);
 
-  // ---- basis (arena 3), grown by whole check periods
+  // ---- basis (arena 3): a zero row, then q_0, q_1, ...; grown by whole periods
   void* vb;
-  DMSC_TRY(ctx->scratch(3, vstride * sizeof(float) * (size_t)(period + 2), &vb));
-  size_t cap = ctx->arena_bytes[3] / (vstride * sizeof(float));
-  float* V = (float*)vb;
+  DMSC_TRY(ctx->scratch(3, vstride * sizeof(float) * (size_t)(period + 3), &vb));
+  size_t cap = ctx->arena_bytes[3] / (vstride * sizeof(float)) - 1;  // rows after the zero row
+  float* V = (float*)vb + vstride;
 
-  LzBufs<Z> bf;
-  bf.A = A;
-  bf.pitch = (size_t)pitch;
-  bf.n = n;
-  bf.vstride = vstride;
-  bf.z0 = (Z*)(B + o_z0);
-  bf.z1 = (Z*)(B + o_z1);
-  bf.part = (double*)(B + o_p);
-  bf.part2 = (double*)(B + o_p2);
-  bf.alpha = (Z*)(B + o_al);
-  bf.beta = (Z*)(B + o_be);
-  bf.nb1 = nb1;
-  bf.nb2 = nb2;
-  bf.st = st;
+  Steps p;
+  p.mixed = mixed;
+  p.so = make_stencil_offsets(w);
+  p.plain = LzBufs{A, (size_t)pitch, n, vstride, (float*)(B + o_z0), (float*)(B + o_z1), (double*)(B + o_p1),
+                   (double*)(B + o_p2), (float*)(B + o_al), (float*)(B + o_be), nb1, nb2, st};
+  p.mix = LzmBufs{A, (size_t)pitch, n, vstride, (double*)(B + o_z0), (double*)(B + o_z1), (double*)(B + o_p1),
+                  (double*)(B + o_p2), (double*)(B + o_p3), (double*)(B + o_al), (double*)(B + o_be), nb1, st};
 
   static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
   if (!invit_lds) {
@@ -663,7 +766,13 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   }
   DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
   const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
-  hipLaunchKernelGGL(k_init<Z>, dim3(div_up(n, 256)), dim3(256), 0, s, st, V, bf.z1, q0, n);
+  const dim3 gi(div_up(std::max(n, nb1), 256));
+  hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
+  if (mixed)
+    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.mix.z1, q0, n, p.mix.p1, p.mix.p2, p.mix.p3, nb1);
+  else
+    hipLaunchKernelGGL(k_init<float>, gi, dim3(256), 0, s, st, V, p.plain.z1, q0, n, (double*)nullptr,
+                       (double*)nullptr, (double*)nullptr, 0);
   DMSC_HIP(hipGetLastError());
 
   int eigCheck = nvec + kCullumExtra;
@@ -673,20 +782,28 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   bool converged = false;
   while (done < kMaxIter) {
     const int steps = std::min(period, kMaxIter - done);
-    if ((size_t)done + steps + 2 > cap) {  // grow the basis, keeping q_0..q_{done-1}
+    if ((size_t)done + steps + 2 > cap) {  // grow the basis, keeping the zero row and q_0..q_{done-1}
       const size_t ncap = std::min<size_t>(std::max(cap * 2, (size_t)done + steps + 2), kMaxIter + 2);
       void* nbuf;
-      DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * ncap));
-      DMSC_HIP(hipMemcpyAsync(nbuf, V, vstride * sizeof(float) * done, hipMemcpyDeviceToDevice, s));
-      DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * ncap));  // synchronises before freeing
-      V = (float*)nbuf;
+      DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * (ncap + 1)));
+      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * (done + 1), hipMemcpyDeviceToDevice, s));
+      DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * (ncap + 1)));  // synchronises before freeing
+      V = (float*)nbuf + vstride;
       cap = ncap;
       hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
     }
-    for (int t = 0; t < steps; t += kGraphSteps) DMSC_TRY(run_steps<Z>(ctx, bf, so, w, std::min(kGraphSteps, steps - t)));
+    for (int t = 0; t < steps; t += kGraphSteps)
+      DMSC_TRY(run_steps(ctx, p, st, w, done + t, std::min(kGraphSteps, steps - t)));
     done += steps;
-    hipLaunchKernelGGL(k_lz_beta<Z>, dim3(1), dim3(kK1), 0, s, bf);
-    hipLaunchKernelGGL(k_widen<Z>, dim3(div_up(done, 256)), dim3(256), 0, s, bf.alpha, bf.beta, ad, bd, done);
+    if (mixed) {
+      hipLaunchKernelGGL(k_lzm_finish, dim3(1), dim3(kK1), 0, s, p.mix, done);
+      hipLaunchKernelGGL(k_widen<double>, dim3(div_up(done, 256)), dim3(256), 0, s, p.mix.alpha, p.mix.beta, ad, bd,
+                         done);
+    } else {
+      hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kK1), 0, s, p.plain, done);
+      hipLaunchKernelGGL(k_widen<float>, dim3(div_up(done, 256)), dim3(256), 0, s, p.plain.alpha, p.plain.beta, ad,
+                         bd, done);
+    }
     for (;;) {
       const int ncheck = std::min(eigCheck, done);
       hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, s, ad, bd, st, wv,
@@ -718,7 +835,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const int m = hs.m;
 
   // Ritz vectors, D^-1/2 scaling (stencilMVM.cu:182-198), min-max normalisation
-  DMSC_TRY(launch_ritz<Z>(s, nvec, n, st, vstride, m, S, d_eigvecs));
+  DMSC_TRY(launch_ritz(s, mixed, nvec, n, st, vstride, m, S, d_eigvecs));
   const dim3 g(div_up(n, 256));
   hipLaunchKernelGGL(k_scale_rs, g, dim3(256), 0, s, n, nvec, rs, d_eigvecs);
   float* mm = (float*)(B + o_mm);
@@ -738,9 +855,7 @@ int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float 
                    float* h_eigvals, float* d_eigvecs, int* iterations) {
   if (nvec < 1 || nvec > kMaxEig) return fail(DMSC_ERR_ARG, "nvec must be in [1, %d]", kMaxEig);
   if (pitch % 4) return fail(DMSC_ERR_ARG, "matrix pitch must be a multiple of 4 floats");
-  if (precision == DMSC_PREC_MIXED)
-    return solve<double>(ctx, w, h, A, pitch, nvec, tol, true, h_eigvals, d_eigvecs, iterations);
-  return solve<float>(ctx, w, h, A, pitch, nvec, tol, false, h_eigvals, d_eigvecs, iterations);
+  return solve(ctx, w, h, A, pitch, nvec, tol, precision == DMSC_PREC_MIXED, h_eigvals, d_eigvecs, iterations);
 }
 
 void release_eigensolver(const Ctx* ctx) { graph_caches().erase(ctx); }

@@ -91,9 +91,33 @@ __global__ __launch_bounds__(64) void k_spmv(int n, StencilOffsets so, const flo
     if (row + j < n) y[row + j] = acc[j];
 }
 
+// y = A x with one row per lane in 256-thread blocks: all 81 matrix loads and
+// all 81 gathers of a lane are issued before the multiply-add chain, so each
+// wave has one memory round trip instead of one per group of diagonals.
+__global__ __launch_bounds__(256) void k_spmv_full(int n, StencilOffsets so, const float* __restrict__ A,
+                                                   size_t pitch, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= n) return;
+  float a[kStencilArea], xv[kStencilArea];
+#pragma unroll
+  for (int d = 0; d < kStencilArea; d++) a[d] = A[(size_t)d * pitch + row];
+#pragma unroll
+  for (int d = 0; d < kStencilArea; d++) {
+    const int col = row + so.off[d];
+    const bool in = (unsigned)col < (unsigned)n;
+    const float v = x[in ? col : 0];
+    xv[d] = in ? v : 0.0f;  // fma(a, 0, acc) == acc: the reference's bounds test
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int d = 0; d < kStencilArea; d++) acc = fmaf(a[d], xv[d], acc);
+  y[row] = acc;
+}
+
 }  // namespace
 
-int spmv_rows_per_lane = 4;
+int spmv_rows_per_lane = 0;  // 0: k_spmv_full; 1, 2, 4: k_spmv<RPL>
 
 int launch_spmv(hipStream_t s, int w, int h, const float* A, int pitch, const float* x, float* y) {
   const int n = w * h;
@@ -106,8 +130,11 @@ int launch_spmv(hipStream_t s, int w, int h, const float* A, int pitch, const fl
     case 2:
       hipLaunchKernelGGL(k_spmv<2>, dim3(div_up(n, 128)), dim3(64), 0, s, n, so, A, (size_t)pitch, x, y);
       break;
-    default:
+    case 4:
       hipLaunchKernelGGL(k_spmv<4>, dim3(div_up(n, 256)), dim3(64), 0, s, n, so, A, (size_t)pitch, x, y);
+      break;
+    default:
+      hipLaunchKernelGGL(k_spmv_full, dim3(div_up(n, 256)), dim3(256), 0, s, n, so, A, (size_t)pitch, x, y);
       break;
   }
   DMSC_HIP(hipGetLastError());

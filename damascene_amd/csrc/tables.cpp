@@ -329,13 +329,20 @@ int rotation_tables(int width, int height, float thetaPi, int* nw, int* nh, int*
 }
 
 // rotate.cu:480-614: half-disc as an axis-aligned rectangle in the rotated
-// frame; eight corner offsets A..H (linear, rotated image).
+// frame; eight corner offsets A..H (linear, rotated image).  The tables
+// cover the mirrored frame (w + 2 kBorder by h + 2 kBorder) while the
+// reference centres the disc at (w/2, h/2); for images under about 50 px
+// that puts its corner lookups outside the tables, so they are clamped to
+// the table range (no effect at larger sizes).
 void halfdisc_offsets(int w, int h, float thetaPi, int radius, const int* bX, const int* bY,
                       const int* gX, const int* gY, int rw, int* rect) {
   const float theta = (float)(thetaPi * kPiL);
   const float er = (float)((-4.0 + 4.0 * (double)(float)radius * (double)std::sqrt((float)kPiL)) / 8.0);
   const float s = std::sin(theta), c = std::cos(theta);
+  const int bw = w + 2 * kBorder, bh = h + 2 * kBorder;
   auto rot = [&](int x, int y, int& rx, int& ry) {
+    x = std::min(std::max(x, 0), bw - 1);
+    y = std::min(std::max(y, 0), bh - 1);
     rx = gX[y] + bX[x];
     ry = gY[y] + bY[x];
   };

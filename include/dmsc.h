@@ -163,6 +163,16 @@ int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* d_matrix, in
 int dmsc_spectral_pb(dmsc_ctx* ctx, const float* h_eigvals, const float* d_eigvecs, int w,
                      int h, int nvec, float* d_spb, int pitch);
 
+/* ------------------------------------------------------------------ tuning */
+/* Launch-shape knobs read by the kernels at launch (process-wide; no
+ * reference counterpart: they back the project's tuner).  Keys:
+ *   "spmv_rows_per_lane": 0 = one row per lane in 256-thread blocks with all
+ *                         81 loads in flight (default); 1, 2, 4 = one wave
+ *                         per 64*RPL rows.
+ * Returns DMSC_ERR_ARG for an unknown key or value. */
+int dmsc_tune_set(const char* key, int value);
+int dmsc_tune_get(const char* key, int* value);
+
 /* ------------------------------------------------------------------ whole pipeline */
 typedef struct {
   int n_eig;          /* number of eigenvectors (2..25), default 9 */

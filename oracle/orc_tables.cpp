@@ -6,6 +6,7 @@
 // Float/double semantics follow the reference sources as compiled by a
 // C++11 host compiler: float overloads of sin/cos/exp/round/ceil for float
 // arguments, double pow(float,int), no contraction (-ffp-contract=off).
+#include <algorithm>
 #include "oracle.h"
 #include <cmath>
 #include <cstdlib>
@@ -397,7 +398,12 @@ void orc_halfdisc_offsets(int width, int height, float thetaPi, int radius, cons
   int x3 = (int)std::round(er * s + er * c);
   int y3 = (int)std::round(er * s - er * c);
   int xc = width / 2, yc = height / 2;
-  int x2c = x2 + xc, y2c = y2 + yc, x3c = x3 + xc, y3c = y3 + yc;
+  // The tables cover the mirrored frame (border 30); clamp the lookups into
+  // it (images under ~50 px would otherwise read outside, as the reference does).
+  const int bw = width + 60, bh = height + 60;
+  auto cx = [&](int v) { return std::min(std::max(v, 0), bw - 1); };
+  auto cy = [&](int v) { return std::min(std::max(v, 0), bh - 1); };
+  int x2c = cx(x2 + xc), y2c = cy(y2 + yc), x3c = cx(x3 + xc), y3c = cy(y3 + yc);
   int x2rc = gX[y2c] + bX[x2c], y2rc = gY[y2c] + bY[x2c];
   int x3rc = gX[y3c] + bX[x3c], y3rc = gY[y3c] + bY[x3c];
   int xcr = gX[yc] + bX[xc], ycr = gY[yc] + bY[xc];

@@ -198,6 +198,22 @@ class Context:
         check(lib().dmsc_calc_gpb(self.handle, w * h, pitch, 8, _p(cg), _p(spb), _p(nmax), _p(all_), _p(g)))
         return all_, g
 
+    def postprocess(self, gpb, nmax):
+        """skeleton.cu PostProcess: returns (sigmoid(gPb), thinned sigmoid(gPb),
+        thinning iterations); the input gPb is left unchanged."""
+        h, w = nmax.shape
+        g = gpb.reshape(h, w).clone()
+        thin = self.empty(h, w)
+        it = ctypes.c_int()
+        check(lib().dmsc_postprocess(self.handle, w, h, _p(g), _p(nmax), _p(thin), ctypes.byref(it)))
+        return g, thin, it.value
+
+    def normalize_gpb_all(self, gpb_all, npix):
+        """skeleton.cu NormalizeGpbAll on a copy of the [8][pitch] planes."""
+        out = gpb_all.clone()
+        check(lib().dmsc_normalize_gpb_all(self.handle, npix, out.shape[0], out.shape[1], _p(out)))
+        return out
+
     def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False):
         """Whole pipeline on a device RGB8 image (int32/uint32 [h, w] tensor)."""
         h, w = rgb.shape

@@ -46,6 +46,11 @@ SIGNATURES = {
     "dmsc_stencil": (c_int, [c_int, c_int, P, P]),
     "dmsc_gamma_lut": (c_int, [c_float, P]),
     "dmsc_load_ppm": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), P]),
+    "dmsc_postprocess": (c_int, [P, c_int, c_int, P, P, P, ctypes.POINTER(c_int)]),
+    "dmsc_normalize_gpb_all": (c_int, [P, c_int, c_int, c_int, P]),
+    "dmsc_write_pb": (c_int, [ctypes.c_char_p, c_int, c_int, P]),
+    "dmsc_write_gpb_all": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, P]),
+    "dmsc_write_pgm": (c_int, [ctypes.c_char_p, c_int, c_int, P]),
     "dmsc_tune_set": (c_int, [ctypes.c_char_p, c_int]),
     "dmsc_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int)]),
     "dmsc_rgb_to_grey": (c_int, [P, c_int, c_int, P, P]),

@@ -162,6 +162,18 @@ int dmsc_gamma_lut(float gamma, float* lut) {
   return DMSC_OK;
 }
 
+int dmsc_postprocess(dmsc_ctx* ctx, int w, int h, float* gpb, const float* nmax, float* gpb_thin,
+                     int* iterations) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 0 && h > 0 && (long)w * h < (1L << 28) && gpb && nmax && gpb_thin, "bad arguments");
+  return run_postprocess(ctx, w, h, gpb, nmax, gpb_thin, iterations);
+}
+int dmsc_normalize_gpb_all(dmsc_ctx* ctx, int npix, int norient, int pitch, float* all) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(npix > 0 && norient > 0 && pitch >= npix && all, "bad arguments");
+  return launch_normalize_gpb_all(ctx->stream, npix, norient, pitch, all);
+}
+
 int dmsc_tune_set(const char* key, int value) {
   DMSC_ARG(key, "null key");
   if (!std::strcmp(key, "spmv_rows_per_lane")) {

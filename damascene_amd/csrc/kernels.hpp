@@ -91,6 +91,10 @@ int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float 
                    float* h_eigvals, float* d_eigvecs, int* iterations);
 void release_eigensolver(const Ctx* ctx);  // drops the context's captured graphs
 
+// ------------------------------------------------------------- post-processing
+int run_postprocess(Ctx* ctx, int w, int h, float* gpb, const float* nmax, float* gpb_thin, int* iterations);
+int launch_normalize_gpb_all(hipStream_t s, int npix, int norient, int pitch, float* all);
+
 // ------------------------------------------------------------- spectral Pb
 int launch_spectral_pb(hipStream_t s, const float* h_eigvals, const float* d_eigvecs, int w, int h, int nvec,
                        float* spb, int pitch);

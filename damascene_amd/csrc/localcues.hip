@@ -136,56 +136,26 @@ struct BlurK {
 // NB bins, KR blur radius (0 = no blur).  Output plane of channel ch:
 // [scale][slot] with slot o+4 for the top/bottom split (sense A) and slot o
 // for the left/right split (sense B) (gradient.cu:240-241).
+// chi^2 of one split of the half-disc pair from the packed corner counts:
+// a = c[i0] - c[i1] + c[i2] - c[i3], b = c[j0] - c[j1] + c[j2] - c[j3] per
+// bin (uint16 modulo arithmetic recovers the exact counts), optional bin
+// blur, normalisation to sum 8 and 0.5 * sum (a-b)^2/(a+b) with the
+// reference's 32-slot tree sums (rotate.cu:352-472).
 template <int NB, int KR>
-__global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad0, const int* __restrict__ tabs,
-                                                   const uint16_t* __restrict__ I, BlurK bk,
-                                                   float* __restrict__ grads, size_t plane) {
-  const int n = g.w * g.h;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int scale = blockIdx.y >> 1, sense = blockIdx.y & 1;
-  const int ch = ch0 + (blockIdx.z >> 2), o = blockIdx.z & 3, f = ch * 4 + o;
-  if (p >= n) return;
-  const int y = p / g.w, x = p - y * g.w;
-  const int* t = tabs + g.tab_off[o];
-  const int ix = x + kBorder, iy = y + kBorder;
-  const int rx = t[2 * g.bw + iy] + t[ix];
-  const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
-  const long long pix = (long long)ry * g.nw[o] + rx;
-  const int* rc = g.rect[o][rad0 + scale];
-  // sense A: a = A-C+E-D, b = H-F+D-E ; sense B: a = A-F+G-B, b = H-C+B-G
-  const int k1 = sense ? 5 : 2, k2 = sense ? 6 : 4, k3 = sense ? 1 : 3;
-  const int m1 = sense ? 2 : 5, m2 = sense ? 1 : 3, m3 = sense ? 6 : 4;
-  const uint16_t* base = I + g.int_off[f];
-  const uint4* cA = reinterpret_cast<const uint4*>(base + (size_t)(rc[0] + pix) * 32);
-  const uint4* cB = reinterpret_cast<const uint4*>(base + (size_t)(rc[k1] + pix) * 32);
-  const uint4* cC = reinterpret_cast<const uint4*>(base + (size_t)(rc[k2] + pix) * 32);
-  const uint4* cD = reinterpret_cast<const uint4*>(base + (size_t)(rc[k3] + pix) * 32);
-  const uint4* dA = reinterpret_cast<const uint4*>(base + (size_t)(rc[7] + pix) * 32);
-  const uint4* dB = reinterpret_cast<const uint4*>(base + (size_t)(rc[m1] + pix) * 32);
-  const uint4* dC = reinterpret_cast<const uint4*>(base + (size_t)(rc[m2] + pix) * 32);
-  const uint4* dD = reinterpret_cast<const uint4*>(base + (size_t)(rc[m3] + pix) * 32);
+__device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0, int i1, int i2, int i3, int j0,
+                                            int j1, int j2, int j3, const BlurK& bk) {
   float a[32], b[32];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {  // 8 bins per 16-byte load
-    if (q * 8 >= NB) {
-#pragma unroll
-      for (int j = 0; j < 8; j++) { a[q * 8 + j] = 0.f; b[q * 8 + j] = 0.f; }
-      continue;
-    }
-    const uint4 va = cA[q], vb = cB[q], vc = cC[q], vd = cD[q];
-    const uint4 wa = dA[q], wb = dB[q], wc = dC[q], wd = dD[q];
-    const uint32_t pa[4] = {va.x, va.y, va.z, va.w}, pb[4] = {vb.x, vb.y, vb.z, vb.w};
-    const uint32_t pc[4] = {vc.x, vc.y, vc.z, vc.w}, pd[4] = {vd.x, vd.y, vd.z, vd.w};
-    const uint32_t qa[4] = {wa.x, wa.y, wa.z, wa.w}, qb[4] = {wb.x, wb.y, wb.z, wb.w};
-    const uint32_t qc[4] = {wc.x, wc.y, wc.z, wc.w}, qd[4] = {wd.x, wd.y, wd.z, wd.w};
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int sh = (j & 1) * 16, e = j >> 1;
-      const uint32_t s0 = (pa[e] >> sh) - (pb[e] >> sh) + (pc[e] >> sh) - (pd[e] >> sh);
-      const uint32_t s1 = (qa[e] >> sh) - (qb[e] >> sh) + (qc[e] >> sh) - (qd[e] >> sh);
-      const bool live = q * 8 + j < NB;
-      a[q * 8 + j] = live ? (float)(s0 & 0xffffu) : 0.f;
-      b[q * 8 + j] = live ? (float)(s1 & 0xffffu) : 0.f;
+  for (int j = 0; j < 32; j++) {
+    const int e = j >> 1, sh = (j & 1) * 16;
+    if (j < NB) {
+      const uint32_t s0 = (pk[i0][e] >> sh) - (pk[i1][e] >> sh) + (pk[i2][e] >> sh) - (pk[i3][e] >> sh);
+      const uint32_t s1 = (pk[j0][e] >> sh) - (pk[j1][e] >> sh) + (pk[j2][e] >> sh) - (pk[j3][e] >> sh);
+      a[j] = (float)(s0 & 0xffffu);
+      b[j] = (float)(s1 & 0xffffu);
+    } else {
+      a[j] = 0.f;
+      b[j] = 0.f;
     }
   }
   if (KR > 0) {  // rotate.cu:352-370: zero-padded correlation over bins
@@ -208,7 +178,10 @@ __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad
       cb[i] = sb;
     }
 #pragma unroll
-    for (int i = 0; i < NB; i++) { a[i] = ca[i]; b[i] = cb[i]; }
+    for (int i = 0; i < NB; i++) {
+      a[i] = ca[i];
+      b[i] = cb[i];
+    }
   }
   float t32[32];
 #pragma unroll
@@ -230,59 +203,129 @@ __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad
     }
     t32[i] = chi;
   }
-  const float out = 0.5f * dev::tree32(t32);
-  const int slot = sense ? o : o + 4;
-  grads[(size_t)((ch * 3 + scale) * 8 + slot) * plane + p] = out;
+  return 0.5f * dev::tree32(t32);
 }
 
-// Savitzky-Golay taps of one (radius, orientation) in the reference's
-// order (v outer, u inner), zero taps dropped.
-struct Tap {
-  short du, dv;
-  float c;
-};
+// Both splits of one (pixel, scale, orientation): the eight corners A..H
+// are loaded once (uint16 counts, 8 bins per 16-byte load) and feed
+//   sense A (top/bottom, slot o+4): a = A-C+E-D, b = H-F+D-E
+//   sense B (left/right, slot o):   a = A-F+G-B, b = H-C+B-G
+// (rotate.cu:400-445; gradient.cu:240-241).
+template <int NB, int KR>
+__global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad0, const int* __restrict__ tabs,
+                                                   const uint16_t* __restrict__ I, BlurK bk,
+                                                   float* __restrict__ grads, size_t plane) {
+  const int n = g.w * g.h;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int scale = blockIdx.y;
+  const int ch = ch0 + (blockIdx.z >> 2), o = blockIdx.z & 3, f = ch * 4 + o;
+  if (p >= n) return;
+  const int y = p / g.w, x = p - y * g.w;
+  const int* t = tabs + g.tab_off[o];
+  const int ix = x + kBorder, iy = y + kBorder;
+  const int rx = t[2 * g.bw + iy] + t[ix];
+  const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
+  const long long pix = (long long)ry * g.nw[o] + rx;
+  const int* rc = g.rect[o][rad0 + scale];
+  const uint16_t* base = I + g.int_off[f];
+  constexpr int NQ = (NB + 7) / 8;
+  uint32_t pk[8][16];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint4* src = reinterpret_cast<const uint4*>(base + (size_t)(rc[c] + pix) * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (q < NQ) {
+        const uint4 v = src[q];
+        pk[c][4 * q] = v.x;
+        pk[c][4 * q + 1] = v.y;
+        pk[c][4 * q + 2] = v.z;
+        pk[c][4 * q + 3] = v.w;
+      } else {
+        pk[c][4 * q] = pk[c][4 * q + 1] = pk[c][4 * q + 2] = pk[c][4 * q + 3] = 0u;
+      }
+    }
+  }
+  // corner indices: A0 B1 C2 D3 E4 F5 G6 H7
+  const float top = chi2_split<NB, KR>(pk, 0, 2, 4, 3, 7, 5, 3, 4, bk);
+  const float left = chi2_split<NB, KR>(pk, 0, 5, 6, 1, 7, 2, 1, 6, bk);
+  float* dst = grads + (size_t)((ch * 3 + scale) * 8) * plane + p;
+  dst[(size_t)(o + 4) * plane] = top;
+  dst[(size_t)o * plane] = left;
+}
 
-constexpr int PTX = 64, PTY = 16;
+// Parabola smoothing (parabola.cu:75-124 / parabola_kernel.cu:1-30) of one
+// scale of a channel group, plane z = (channel - ch0) * 8 + orientation.
+// Each thread computes QX horizontally adjacent outputs.  For every filter
+// row v (the reference's outer loop, ascending) it loads the QX + 2R tile
+// values that row needs into registers once, then runs the fully unrolled u
+// loop (inner, ascending) over the dense (2R+1)-wide row.  Taps outside the
+// ellipse have coefficient 0 and fma(x, 0, acc) == acc for finite x (acc
+// starts at +0 and can never become -0), so the sums equal the reference's
+// zero-skipping loop bit for bit; rows with no non-zero tap are skipped.
+constexpr int QX = 8, QCOLS = 16, QTX = QX * QCOLS, QTY = 16;
 
-// parabola.cu:75-124 / parabola_kernel.cu:1-30 for one scale of a channel
-// group: plane z = (channel - ch0) * 8 + orientation.
-__global__ __launch_bounds__(256) void k_parabola(int w, int h, int r, int scale, int ch0,
-                                                  const float* __restrict__ grads, size_t gplane,
-                                                  const Tap* __restrict__ taps, int tap_stride,
-                                                  const int* __restrict__ ntaps, CueOut out, int pitch) {
-  extern __shared__ float tile[];
-  const int LW = PTX + 2 * r, LH = PTY + 2 * r;
+template <int R>
+constexpr int q_lw() {
+  return (QTX + 2 * R + 3) / 4 * 4;  // tile row length, float4-aligned
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_parabola(int w, int h, int scale, int ch0, const float* __restrict__ grads,
+                                                  size_t gplane, const float* __restrict__ coef,
+                                                  const int2* __restrict__ vrange, CueOut out, int pitch) {
+  extern __shared__ float4 tile4[];
+  float* tile = reinterpret_cast<float*>(tile4);
+  constexpr int LW = q_lw<R>(), LH = QTY + 2 * R, D = 2 * R + 1;
+  constexpr int SEG = (QX + 2 * R + 3) / 4 * 4;
   const int ch = ch0 + blockIdx.z / 8, o = blockIdx.z % 8;
   const float* src = grads + (size_t)((ch * 3 + scale) * 8 + o) * gplane;
-  const int x0 = blockIdx.x * PTX, y0 = blockIdx.y * PTY;
+  const int x0 = blockIdx.x * QTX, y0 = blockIdx.y * QTY;
   for (int i = threadIdx.x; i < LW * LH; i += 256) {
     const int ly = i / LW, lx = i - ly * LW;
     // tile columns past the image only feed discarded outputs: clamp into
     // the mirrored range [0, w + 2*border) before reflecting
-    const int gx = dev::mirror(dev::clampi(x0 + lx - r + kBorder, 0, w + 2 * kBorder - 1), kBorder, w);
-    const int gy = dev::mirror(dev::clampi(y0 + ly - r + kBorder, 0, h + 2 * kBorder - 1), kBorder, h);
+    const int gx = dev::mirror(dev::clampi(x0 + lx - R + kBorder, 0, w + 2 * kBorder - 1), kBorder, w);
+    const int gy = dev::mirror(dev::clampi(y0 + ly - R + kBorder, 0, h + 2 * kBorder - 1), kBorder, h);
     tile[i] = src[(size_t)gy * w + gx];
   }
   __syncthreads();
-  const int tx = threadIdx.x & 63;
-  const Tap* tp = taps + (size_t)o * tap_stride;
-  const int nt = ntaps[o];
-  float* dst = out.p[ch] + (size_t)(scale * 8 + o) * pitch;
-  for (int ty = threadIdx.x >> 6; ty < PTY; ty += 4) {
-    float acc = 0.0f;
-    const float* ctr = tile + (size_t)(ty + r) * LW + tx + r;
-    // nt is a multiple of 8 (lists are padded with zero taps, which leave the
-    // sum unchanged), so the loop has no remainder iterations.
-    for (int k = 0; k < nt; k += 8) {
+  const int tx = (threadIdx.x % QCOLS) * QX, ty = threadIdx.x / QCOLS;
+  const float* f = coef + (size_t)o * D * D;
+  const int2 vr = vrange[o];
+  float acc[QX];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const Tap tk = tp[k + j];
-        acc = fmaf(ctr[tk.dv * LW + tk.du], tk.c, acc);
-      }
+  for (int j = 0; j < QX; j++) acc[j] = 0.0f;
+  for (int v = vr.x; v <= vr.y; v++) {
+    const float4* rowp = reinterpret_cast<const float4*>(tile + (ty + R + v) * LW + tx);
+    float seg[SEG];
+#pragma unroll
+    for (int q = 0; q < SEG / 4; q++) {
+      const float4 t = rowp[q];
+      seg[4 * q] = t.x;
+      seg[4 * q + 1] = t.y;
+      seg[4 * q + 2] = t.z;
+      seg[4 * q + 3] = t.w;
     }
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx < w && gy < h) dst[(size_t)gy * w + gx] = acc;
+    const float* fr = f + (v + R) * D;
+#pragma unroll
+    for (int u = 0; u < D; u++) {
+      const float c = fr[u];
+#pragma unroll
+      for (int j = 0; j < QX; j++) acc[j] = fmaf(seg[j + u], c, acc[j]);
+    }
   }
+  float* dst = out.p[ch] + (size_t)(scale * 8 + o) * pitch;
+  const int gy = y0 + ty;
+  if (gy < h)
+#pragma unroll
+    for (int j = 0; j < QX; j++)
+      if (x0 + tx + j < w) dst[(size_t)gy * w + x0 + tx + j] = acc[j];
+}
+
+template <int R>
+size_t parabola_lds() {
+  return sizeof(float) * (size_t)q_lw<R>() * (QTY + 2 * R) + 16;
 }
 
 // ---------------------------------------------------------------- host plan
@@ -293,15 +336,15 @@ struct CuePlan {
   size_t lab_bytes = 0, int_elems = 0;
   int tap_stride = 0;
   int* d_tabs = nullptr;
-  Tap* d_taps[4] = {nullptr, nullptr, nullptr, nullptr};
-  int* d_ntaps[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* d_coef[4] = {nullptr, nullptr, nullptr, nullptr};  // [8][2r+1][2r+1] per radius
+  int2* d_vrange[4] = {nullptr, nullptr, nullptr, nullptr};  // rows holding non-zero taps
   BlurK blur_bg, blur_cg;
   int kr_bg = 0, kr_cg = 0;
   ~CuePlan() {
     if (d_tabs) (void)hipFree(d_tabs);
     for (int i = 0; i < 4; i++) {
-      if (d_taps[i]) (void)hipFree(d_taps[i]);
-      if (d_ntaps[i]) (void)hipFree(d_ntaps[i]);
+      if (d_coef[i]) (void)hipFree(d_coef[i]);
+      if (d_vrange[i]) (void)hipFree(d_vrange[i]);
     }
   }
 };
@@ -368,26 +411,21 @@ int build_plan(hipStream_t s, int w, int h, CuePlan& P) {
     const int r = kRadii[ri], len = 2 * r + 1;
     std::vector<float> filt((size_t)8 * len * len);
     parabola_filters(r, filt.data());
-    const size_t stride = (size_t)len * len + 8;
-    std::vector<Tap> all((size_t)8 * stride, Tap{0, 0, 0.f});
-    std::vector<int> cnt(8);
+    std::vector<int2> vr(8);
     for (int o = 0; o < 8; o++) {
-      int k = 0;
+      int v0 = r + 1, v1 = -r - 1;
       for (int v = -r; v <= r; v++)
-        for (int u = -r; u <= r; u++) {
-          const float c = filt[(size_t)o * len * len + (v + r) * len + (u + r)];
-          if (c != 0.0f) all[(size_t)o * stride + k++] = Tap{(short)u, (short)v, c};
-        }
-      // pad to a multiple of 8 with zero taps at the centre: fma(x, 0, acc) == acc
-      // for finite x because acc is never -0 (it starts at +0 and an exact
-      // cancellation rounds to +0)
-      while (k % 8) all[(size_t)o * stride + k++] = Tap{0, 0, 0.f};
-      cnt[o] = k;
+        for (int u = -r; u <= r; u++)
+          if (filt[(size_t)o * len * len + (v + r) * len + (u + r)] != 0.0f) {
+            v0 = std::min(v0, v);
+            v1 = std::max(v1, v);
+          }
+      vr[o] = make_int2(v0, v1);  // empty range (v0 > v1) if every tap is zero
     }
-    DMSC_HIP(hipMalloc(&P.d_taps[ri], sizeof(Tap) * all.size()));
-    DMSC_HIP(hipMemcpy(P.d_taps[ri], all.data(), sizeof(Tap) * all.size(), hipMemcpyHostToDevice));
-    DMSC_HIP(hipMalloc(&P.d_ntaps[ri], sizeof(int) * 8));
-    DMSC_HIP(hipMemcpy(P.d_ntaps[ri], cnt.data(), sizeof(int) * 8, hipMemcpyHostToDevice));
+    DMSC_HIP(hipMalloc(&P.d_coef[ri], sizeof(float) * filt.size()));
+    DMSC_HIP(hipMemcpy(P.d_coef[ri], filt.data(), sizeof(float) * filt.size(), hipMemcpyHostToDevice));
+    DMSC_HIP(hipMalloc(&P.d_vrange[ri], sizeof(int2) * 8));
+    DMSC_HIP(hipMemcpy(P.d_vrange[ri], vr.data(), sizeof(int2) * 8, hipMemcpyHostToDevice));
   }
   // gradient.cu:233: blur sigma = (int)(sigma * nbins) -> Bg 2, Cg 1
   memset(&P.blur_bg, 0, sizeof(BlurK));
@@ -460,21 +498,28 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
   memset(&none, 0, sizeof(none));
   // Bg: channel 0, radii {3,5,10}; Cga/Cgb: channels 1-2, radii {5,10,20};
   // Tg: channel 3, radii {5,10,20}.
-  hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 6, 4), dim3(256), 0, s, g, 0, 0,
+  hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 0, 0,
                      P->d_tabs, I, P->blur_bg, grads, plane);
-  hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 6, 8), dim3(256), 0, s, g, 1, 1,
+  hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 3, 8), dim3(256), 0, s, g, 1, 1,
                      P->d_tabs, I, P->blur_cg, grads, plane);
-  hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 6, 4), dim3(256), 0, s, g, 3, 1,
+  hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
                      P->d_tabs, I, none, grads, plane);
   CueOut out{{bg, cga, cgb, tg}};
   for (int scale = 0; scale < 3; scale++) {
     for (int grp = 0; grp < 2; grp++) {
-      const int ri = scale + grp, r = kRadii[ri];
+      const int ri = scale + grp;
       const int ch0 = grp == 0 ? 0 : 1, nch = grp == 0 ? 1 : 3;
-      const size_t lds = sizeof(float) * (PTX + 2 * r) * (PTY + 2 * r);
-      hipLaunchKernelGGL(k_parabola, dim3(div_up(w, PTX), div_up(h, PTY), nch * 8), dim3(256), lds, s, w,
-                         h, r, scale, ch0, grads, plane, P->d_taps[ri], (2 * r + 1) * (2 * r + 1) + 8,
-                         P->d_ntaps[ri], out, pitch);
+      const dim3 grid(div_up(w, QTX), div_up(h, QTY), nch * 8);
+      switch (kRadii[ri]) {
+#define PARABOLA_CASE(R)                                                                                   \
+  case R:                                                                                                  \
+    hipLaunchKernelGGL(k_parabola<R>, grid, dim3(256), parabola_lds<R>(), s, w, h, scale, ch0, grads,       \
+                       plane, P->d_coef[ri], P->d_vrange[ri], out, pitch);                                \
+    break;
+        PARABOLA_CASE(3) PARABOLA_CASE(5) PARABOLA_CASE(10) PARABOLA_CASE(20)
+#undef PARABOLA_CASE
+        default: return fail(DMSC_ERR_INTERNAL, "no parabola kernel for radius %d", kRadii[ri]);
+      }
     }
   }
   DMSC_HIP(hipGetLastError());

@@ -29,6 +29,7 @@
 #include "lanczos.h"
 #include "localcues.h"
 #include "nonmax.h"
+#include "skeleton.h"
 #include "spectralPb.h"
 #include "stencilMVM.h"
 #include "texton.h"
@@ -296,4 +297,16 @@ void StartCalcGPb(int p_nPixels, int p_nMatrixPitch, int p_nOrient, float* devCo
   check(dmsc_calc_gpb(default_ctx(), p_nPixels, p_nMatrixPitch, p_nOrient, devCombinedGradient, devspb, devmpb,
                       devGpball, devResult),
         "StartCalcGPb");
+}
+
+// ------------------------------------------------------------------ post-processing
+void PostProcess(int width, int height, unsigned int matrixPitchInFloats, float* devGPb, float* devMPb,
+                 float* devGPb_thin) {
+  if ((int)matrixPitchInFloats != width) throw std::runtime_error("PostProcess: pitch must equal width");
+  check(dmsc_postprocess(default_ctx(), width, height, devGPb, devMPb, devGPb_thin, nullptr), "PostProcess");
+}
+
+void NormalizeGpbAll(int p_nPixels, int p_nOrient, int matrixPitchInFloats, float* devGpball) {
+  check(dmsc_normalize_gpb_all(default_ctx(), p_nPixels, p_nOrient, matrixPitchInFloats, devGpball),
+        "NormalizeGpbAll");
 }

@@ -163,6 +163,27 @@ int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* d_matrix, in
 int dmsc_spectral_pb(dmsc_ctx* ctx, const float* h_eigvals, const float* d_eigvecs, int w,
                      int h, int nvec, float* d_spb, int pitch);
 
+/* ------------------------------------------------------------------ post-processing */
+/* skeleton.cu:172-187 PostProcess: thin mask = nmax > 0.05 thinned by the
+ * eight look-up tables to a fixed point; d_gpb_thin = sigmoid(gPb on the
+ * mask); d_gpb is replaced by sigmoid(gPb) (in place, as the reference).
+ * *iterations (may be NULL) = thinning iterations run. */
+int dmsc_postprocess(dmsc_ctx* ctx, int w, int h, float* d_gpb, const float* d_nmax, float* d_gpb_thin,
+                     int* iterations);
+/* skeleton.cu:247-252 NormalizeGpbAll: sigmoid of the oriented gPb planes,
+ * in place. */
+int dmsc_normalize_gpb_all(dmsc_ctx* ctx, int npix, int norient, int pitch, float* d_gpb_all);
+
+/* ------------------------------------------------------------------ result files (host) */
+/* damascene.cu:80-89 writeFile (.pb): int width, int height, float[w*h]. */
+int dmsc_write_pb(const char* path, int w, int h, const float* h_data);
+/* damascene.cu:109-128 writeArray of GpbAll.ary: int 3, int {8, w, h}, then
+ * the planes of orientations {3,2,1,0,7,6,5,4} (damascene.cu:501-510), each
+ * transposed; h_all is the host copy of the [8][pitch] gPbAll planes. */
+int dmsc_write_gpb_all(const char* path, int w, int h, int pitch, const float* h_all);
+/* 8-bit binary PGM of a float map in [0, 1] (Pb.pgm). */
+int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
+
 /* ------------------------------------------------------------------ tuning */
 /* Launch-shape knobs read by the kernels at launch (process-wide; no
  * reference counterpart: they back the project's tuner).  Keys:

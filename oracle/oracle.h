@@ -105,6 +105,14 @@ void orc_spectral_pb(int w, int h, int nvec, const float* eigvals, const float* 
 void orc_gpb(int w, int h, const float* combinedG, const float* spb, const float* nmax,
              float* gpbAll, float* gpb);
 
+/* postprocess/skeleton.cu: one look-up-table pass (lut 0..7) over a 0/1
+ * image, returns the number of cleared pixels; the whole PostProcess
+ * (gpb replaced by its sigmoid, thin = sigmoid of gPb on the thinned mask);
+ * NormalizeGpbAll in place. */
+int  orc_skeleton_pass(int w, int h, int lut, const float* in, float* out);
+int  orc_postprocess(int w, int h, float* gpb, const float* nmax, float* thin);
+void orc_normalize_gpb_all(int npix, int norient, int pitch, float* all);
+
 /* whole pipeline (plain FP32 reference order); returns 0 on success */
 int  orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol,
                       int* textons, float* mpb8, float* cg8, float* nmax, float* eigvals,

@@ -745,3 +745,79 @@ void orc_gpb(int w, int h, const float* cg, const float* spb, const float* nmax,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- postprocess
+// postprocess/skeleton.cu:104-129: neighbourhood codes marked by each table.
+static const int kSkelLut[8][8] = {
+    {89, 91, 217, 219, -1, -1, -1, -1},   {152, 153, 216, 217, 408, 409, 472, 473},
+    {23, 31, 55, 63, -1, -1, -1, -1},     {26, 27, 30, 31, 90, 91, 94, 95},
+    {464, 472, 496, 504, -1, -1, -1, -1}, {50, 51, 54, 55, 306, 307, 310, 311},
+    {308, 310, 436, 438, -1, -1, -1, -1}, {176, 180, 240, 244, 432, 436, 496, 500}};
+
+// skeleton.cu:8-54 applyLut (reads a snapshot, clears where the table fires)
+// with skeleton.cu:189-212 Nhood3's code: weights[r][c] = 2^(r + 3c) for
+// the neighbour at (row + r - 1, col + c - 1).
+extern "C" int orc_skeleton_pass(int w, int h, int lut, const float* in, float* out) {
+  static const int weights[3][3] = {{1, 8, 64}, {2, 16, 128}, {4, 32, 256}};
+  bool table[512] = {false};
+  for (int k = 0; k < 8; k++)
+    if (kSkelLut[lut][k] >= 0) table[kSkelLut[lut][k]] = true;
+  int removed = 0;
+  for (int row = 0; row < h; row++)
+    for (int col = 0; col < w; col++) {
+      const int minRow = row == 0 ? 1 : 0, maxRow = row == h - 1 ? 1 : 2;
+      const int minCol = col == 0 ? 1 : 0, maxCol = col == w - 1 ? 1 : 2;
+      int code = 0;
+      for (int r = minRow; r <= maxRow; r++)
+        for (int c = minCol; c <= maxCol; c++)
+          if (in[(row + r - 1) * w + (col + c - 1)] != 0) code += weights[r][c];
+      float v = in[row * w + col];
+      if (table[code]) {
+        if (v != 0) removed++;
+        v = 0;
+      }
+      out[row * w + col] = v;
+    }
+  return removed;
+}
+
+// skeleton.cu:84-99 normalizeSigmoid in FP32.
+static float skel_sigmoid(float val) {
+  val = (val <= 0.0f) ? 0.0f : ((val * 1.2f > 1.0f) ? 1.0f : 1.2f * val);
+  val = 1.0f / (1.0f + std::exp(2.6433f - 10.7998f * val));
+  return (val - 0.0667f) / 0.9333f;
+}
+
+// skeleton.cu:172-187 PostProcess; returns the thinning iterations.
+extern "C" int orc_postprocess(int w, int h, float* gpb, const float* nmax, float* thin) {
+  const int n = w * h;
+  std::vector<float> a(n), b(n);
+  for (int p = 0; p < n; p++) a[p] = nmax[p] > 0.05f ? 1.0f : 0.0f;
+  int it = 0, changed;
+  do {
+    changed = 0;
+    for (int l = 0; l < 8; l++) {
+      changed += orc_skeleton_pass(w, h, l, a.data(), b.data());
+      a.swap(b);
+    }
+    it++;
+  } while (changed);
+  for (int p = 0; p < n; p++) {
+    const float t = a[p] != 0 ? gpb[p] : 0.0f;
+    gpb[p] = skel_sigmoid(gpb[p]);
+    thin[p] = skel_sigmoid(t);
+  }
+  return it;
+}
+
+// skeleton.cu:227-245 SigmoidGpbAll.
+extern "C" void orc_normalize_gpb_all(int npix, int norient, int pitch, float* all) {
+  for (int o = 0; o < norient; o++)
+    for (int p = 0; p < npix; p++) {
+      float v = all[(size_t)o * pitch + p];
+      v = (v * 1.2f > 1.0f) ? 1.0f : v * 1.2f;
+      if (v < 0.0f) v = 0.0f;
+      v = 1.0f / (1.0f + std::exp(2.6433f - 10.7998f * v));
+      all[(size_t)o * pitch + p] = (v - 0.0667f) / 0.9333f;
+    }
+}

@@ -275,3 +275,25 @@ def pipeline(rgb, nvec=9, tol=1e-3):
     r["rc"] = rc
     r["stage_ms"] = ms
     return r
+
+
+def skeleton_pass(img, lut):
+    h, w = img.shape
+    out = f32(h, w)
+    removed = lib().orc_skeleton_pass(w, h, lut, P(np.ascontiguousarray(img, np.float32)), P(out))
+    return out, removed
+
+
+def postprocess(gpb, nmax):
+    """Returns (sigmoid(gPb), thinned sigmoid(gPb), iterations)."""
+    h, w = nmax.shape
+    g = np.array(gpb, np.float32, copy=True).reshape(h, w)
+    thin = f32(h, w)
+    it = lib().orc_postprocess(w, h, P(g), P(np.ascontiguousarray(nmax, np.float32)), P(thin))
+    return g, thin, it
+
+
+def normalize_gpb_all(all_):
+    a = np.array(all_, np.float32, copy=True)
+    lib().orc_normalize_gpb_all(a.shape[-1] if a.ndim == 2 else a[0].size, a.shape[0], a[0].size, P(a))
+    return a

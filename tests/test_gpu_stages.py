@@ -80,6 +80,15 @@ def test_kmeans_labels(ctx, responses):
     np.testing.assert_array_equal(host(labels), ref)
 
 
+def test_kmeans_labels_mixed(ctx, responses):
+    """BF16 MFMA distances with the certified-margin test and FP32 re-check
+    of near ties: the same labels and iteration count as the exact path."""
+    labels, it = ctx.kmeans(responses, precision=dm.PREC_MIXED)
+    ref, rit = O.kmeans(host(responses))
+    assert it == rit, f"iterations gpu {it} oracle {rit}"
+    np.testing.assert_array_equal(host(labels), ref)
+
+
 @pytest.fixture(scope="module")
 def textons(ctx, colour):
     lab, _ = ctx.find_textons(colour[1])
@@ -162,3 +171,26 @@ def test_spectral_pb_and_gpb(ctx, combined):
     ra, rg = O.gpb(host(cg)[:, :n].reshape(8, H, W).copy(), ref, host(nm))
     assert_bitexact(host(all_)[:, :n].reshape(8, H, W), ra, "gPbAll")
     assert_bitexact(host(g), rg, "gPb")
+
+
+def test_postprocess_and_normalize(ctx, combined):
+    """skeleton.cu PostProcess and NormalizeGpbAll against the oracle: the
+    thinned mask and iteration count exactly, values within 1e-6 (device
+    expf in the sigmoid)."""
+    (mpb, cg), _ = combined
+    n = W * H
+    nm = ctx.non_max_suppression(W, H, mpb)
+    rng = np.random.default_rng(11)
+    gpb = torch.from_numpy(rng.random((H, W), dtype=np.float32) * 1.2).cuda()
+    g, thin, it = ctx.postprocess(gpb, nm)
+    rg, rthin, rit = O.postprocess(host(gpb), host(nm))
+    assert it == rit, f"iterations gpu {it} oracle {rit}"
+    off = np.float32((1 / (1 + np.exp(np.float32(2.6433))) - np.float32(0.0667)) / np.float32(0.9333))
+    np.testing.assert_array_equal(np.abs(host(thin) - off) > 1e-5, np.abs(rthin - off) > 1e-5)
+    np.testing.assert_allclose(host(thin), rthin, atol=1e-6, rtol=0)
+    np.testing.assert_allclose(host(g), rg, atol=1e-6, rtol=0)
+    all_ = torch.from_numpy(np.pad(rng.random((8, n), dtype=np.float32) * 1.5 - 0.2,
+                                   ((0, 0), (0, dm.pitch_in_floats(n) - n)))).cuda()
+    got = ctx.normalize_gpb_all(all_, n)
+    ref = O.normalize_gpb_all(host(all_)[:, :n].copy())
+    np.testing.assert_allclose(host(got)[:, :n], ref, atol=1e-6, rtol=0)

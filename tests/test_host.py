@@ -182,6 +182,7 @@ STAGE_API_PROGRAM = r"""
 #include "lanczos.h"
 #include "localcues.h"
 #include "nonmax.h"
+#include "skeleton.h"
 #include "spectralPb.h"
 #include "stencilMVM.h"
 #include "texton.h"
@@ -228,3 +229,35 @@ def test_reference_stage_api_compiles_links_and_reports_errors(tmp_path):
     pitch = dm.pitch_in_floats(481 * 321)
     assert f"area 81 pitch {pitch} last {5 * 481} index {pitch * 80 + 1000}" in r.stdout
     assert "error:" in r.stdout
+
+
+def test_result_file_formats(tmp_path):
+    """damascene.cu writeFile (.pb) and writeArray (GpbAll.ary: orientation
+    order {3,2,1,0,7,6,5,4}, planes transposed), and the 8-bit PGM."""
+    w, h = 5, 3
+    rng = np.random.default_rng(9)
+    g = rng.random((h, w), dtype=np.float32)
+    pb = tmp_path / "x.pb"
+    assert lib().dmsc_write_pb(str(pb).encode(), w, h, P(g)) == 0
+    raw = pb.read_bytes()
+    assert np.frombuffer(raw[:8], np.int32).tolist() == [w, h]
+    np.testing.assert_array_equal(np.frombuffer(raw[8:], np.float32).reshape(h, w), g)
+    pitch = 64
+    planes = np.zeros((8, pitch), np.float32)
+    planes[:, :w * h] = rng.random((8, w * h), dtype=np.float32)
+    ary = tmp_path / "GpbAll.ary"
+    assert lib().dmsc_write_gpb_all(str(ary).encode(), w, h, pitch, P(planes)) == 0
+    raw = ary.read_bytes()
+    hdr = np.frombuffer(raw[:16], np.int32)
+    assert hdr.tolist() == [3, 8, w, h]
+    data = np.frombuffer(raw[16:], np.float32).reshape(8, w, h)
+    for i, o in enumerate([3, 2, 1, 0, 7, 6, 5, 4]):
+        np.testing.assert_array_equal(data[i], planes[o, :w * h].reshape(h, w).T)
+    pgm = tmp_path / "Pb.pgm"
+    g[0, 0], g[0, 1] = 1.5, -0.2
+    assert lib().dmsc_write_pgm(str(pgm).encode(), w, h, P(g)) == 0
+    raw = pgm.read_bytes()
+    head = b"P5\n5 3\n255\n"
+    assert raw.startswith(head)
+    px = np.frombuffer(raw[len(head):], np.uint8).reshape(h, w)
+    np.testing.assert_array_equal(px, (np.clip(g, 0, 1) * np.float32(255)).astype(np.uint8))

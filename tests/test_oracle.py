@@ -376,3 +376,72 @@ def test_spectral_pb_direct_convolution():
                     r += K[o, dy, dx] * p[dy:dy + h, dx:dx + w]
             want[o] += np.abs(r) / math.sqrt(vals[i])
     np.testing.assert_allclose(out, want, rtol=2e-5, atol=1e-5)
+
+
+# --------------------------------------------------------------- post-processing
+def test_skeleton_driver_case():
+    """postprocess/postprocessmain.cu:3-68: on the 3x3 image with the top row
+    and the centre set, the first table marks only the centre."""
+    img = np.zeros((3, 3), np.float32)
+    img[0, :] = 1
+    img[1, 1] = 1
+    out, removed = O.skeleton_pass(img, 0)
+    assert removed == 1
+    want = img.copy()
+    want[1, 1] = 0
+    np.testing.assert_array_equal(out, want)
+
+
+def thin_fixed_point(mask):
+    """Runs the eight passes on a 0/1 image until nothing changes."""
+    a = mask.astype(np.float32)
+    while True:
+        total = 0
+        for lut in range(8):
+            a, r = O.skeleton_pass(a, lut)
+            total += r
+        if total == 0:
+            return a
+
+
+def test_skeleton_invariants():
+    rng = np.random.default_rng(5)
+    h, w = 40, 50
+    # thick bars, a disc and noise
+    m = np.zeros((h, w), np.float32)
+    m[5:8, 3:45] = 1
+    m[10:35, 20:23] = 1
+    yy, xx = np.mgrid[:h, :w]
+    m[(yy - 28) ** 2 + (xx - 38) ** 2 <= 16] = 1
+    m[rng.random((h, w)) < 0.05] = 1
+    t = thin_fixed_point(m)
+    assert ((t != 0) <= (m != 0)).all()  # only removes pixels
+    # a fixed point: one more round removes nothing
+    for lut in range(8):
+        t2, r = O.skeleton_pass(t, lut)
+        assert r == 0
+    # the 3-px bars become 1 px wide along their length
+    assert (t[4:9, 6:42].sum(0) == 1).all()
+    assert (t[12:33, 19:24].sum(1) == 1).all()
+    # 1-px lines are already thin
+    line = np.zeros((9, 20), np.float32)
+    line[4, 2:18] = 1
+    line[1:8, 10] = 1
+    np.testing.assert_array_equal(thin_fixed_point(line), line)
+
+
+def test_postprocess_sigmoid_closed_form():
+    nm = np.zeros((6, 7), np.float32)
+    nm[2, 1:6] = 0.5  # a thin ridge above the 0.05 threshold: kept
+    g = np.linspace(-0.2, 1.1, 42, dtype=np.float32).reshape(6, 7)
+
+    def sig(v):
+        v = np.where(v <= 0, 0, np.where(v * 1.2 > 1, 1, 1.2 * v))
+        v = 1 / (1 + np.exp(2.6433 - 10.7998 * v))
+        return (v - 0.0667) / 0.9333
+
+    rg, thin, it = O.postprocess(g, nm)
+    np.testing.assert_allclose(rg, sig(g.astype(np.float64)), atol=2e-6)
+    np.testing.assert_allclose(thin, sig(np.where(nm > 0.05, g, 0).astype(np.float64)), atol=2e-6)
+    all_ = np.stack([g.ravel(), -g.ravel()])
+    np.testing.assert_allclose(O.normalize_gpb_all(all_), sig(all_.astype(np.float64)), atol=2e-6)

@@ -44,7 +44,86 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--spmv-reps", type=int, default=200)
+    p.add_argument("--pmc-workload", action="store_true",
+                   help="run only the workload of the rocprofv3 counter passes: one 256 MiB device copy "
+                        "(calibration) and 20 launches of the selected SpMV shape, then exit")
+    p.add_argument("--spmv-shape", type=int, default=0, choices=sorted(SPMV_SHAPES),
+                   help="SpMV launch shape for --pmc-workload (0 = k_spmv_full, else rows per lane)")
+    p.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                   help="summarise two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE passes of "
+                        "--pmc-workload) into HBM bytes per SpMV launch, calibrated on the 256 MiB copy")
     return p.parse_args()
+
+
+def pmc_summarize(fetch_dir, write_dir, shape):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the SpMV launches, each
+    scaled by what the same counter reads for the 256 MiB copy (expected
+    262,144 KiB read and 262,144 KiB written)."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        vals = {}
+        for f in files:
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == counter:
+                    key = (r["Kernel_Name"], r.get("Dispatch_Id", ""))
+                    vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+        return vals
+
+    kname = SPMV_SHAPES[shape].split("<")[0]
+    out = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        v = per_kernel(d, counter)
+        copies = [x for (k, _), x in v.items() if "copy" in k.lower()]
+        spmv = [x for (k, _), x in v.items() if kname in k and "spmv" in k]
+        if not copies or not spmv:
+            raise SystemExit(f"{counter}: copy kernel or {kname} not found in {d}")
+        cal = max(copies)
+        out[counter] = dict(copy_kib=cal, scale=262144.0 / cal, spmv_kib=sum(spmv) / len(spmv), launches=len(spmv))
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    traffic = (f["spmv_kib"] * f["scale"] + w["spmv_kib"] * w["scale"]) * 1024
+    algo = SPMV_BYTES_PER_ROW * W * H
+    lines = [
+        f"calibration copy (256 MiB): FETCH_SIZE {f['copy_kib']:.0f} KiB (scale {f['scale']:.4f}), "
+        f"WRITE_SIZE {w['copy_kib']:.0f} KiB (scale {w['scale']:.4f})",
+        f"{SPMV_SHAPES[shape]} ({f['launches']} launches): FETCH_SIZE {f['spmv_kib']:.1f} KiB, "
+        f"WRITE_SIZE {w['spmv_kib']:.1f} KiB per launch",
+        f"algorithmic bytes per launch {algo}",
+        f"traffic_bytes_per_launch {SPMV_SHAPES[shape]} {traffic:.0f}",
+        f"traffic / algorithmic = {traffic / algo:.3f}",
+    ]
+    print("\n".join(lines))
+
+
+def pmc_workload(shape):
+    """Counter-pass workload: a 256 MiB copy whose FETCH_SIZE/WRITE_SIZE
+    calibrate the counters (256 MiB read + 256 MiB written), then 20 stencil
+    SpMVs on the 481x321 L_sym with the given launch shape."""
+    import torch
+
+    import damascene_amd as dm
+    from damascene_amd._lib import check, lib
+
+    ctx = dm.Context(0)
+    out = ctx.gpb(ctx.upload(dm.generate_image(W, H, CELLS, SEED)), precision=dm.PREC_FP32)
+    m = ctx.intervene(out["nmax"])
+    ctx.convert_matrix(W, H, m)
+    a = torch.rand(64 << 20, device=ctx.device)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    x = torch.rand(W * H, device=ctx.device)
+    y = torch.empty_like(x)
+    check(lib().dmsc_tune_set(b"spmv_rows_per_lane", shape))
+    for _ in range(20):
+        check(lib().dmsc_stencil_spmv(ctx.handle, W, H, m.data_ptr(), m.shape[1], x.data_ptr(), y.data_ptr()))
+    torch.cuda.synchronize()
+    print(f"pmc workload done: copy 256 MiB, 20 x {SPMV_SHAPES[shape]}", flush=True)
 
 
 def time_pipeline(ctx, d_rgb, precision, steps, warmup):
@@ -153,6 +232,12 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+    if a.pmc_workload:
+        pmc_workload(a.spmv_shape)
+        return
+    if a.pmc_summarize:
+        pmc_summarize(*a.pmc_summarize, a.spmv_shape)
+        return
     import torch
 
     import damascene_amd as dm

@@ -21,6 +21,7 @@ __all__ = [
 PREC_FP32 = 0
 PREC_MIXED = 1
 TEXTON32 = 1
+TEXTON64 = 2
 
 
 def _p(t):
@@ -114,23 +115,23 @@ class Context:
         check(lib().dmsc_texton_responses(self.handle, w, h, _p(grey), _p(out)))
         return out
 
-    def kmeans(self, resp, precision=PREC_FP32, max_iter=15):
+    def kmeans(self, resp, precision=PREC_FP32, max_iter=15, k=32):
         _, h, w = resp.shape
         labels = self.empty(h, w, dtype=self._torch.int32)
         ch, it = ctypes.c_int(), ctypes.c_int()
-        check(lib().dmsc_kmeans(self.handle, w, h, 32, 34, _p(resp), _p(labels), max_iter, 0, precision,
+        check(lib().dmsc_kmeans(self.handle, w, h, k, 34, _p(resp), _p(labels), max_iter, 0, precision,
                                 ctypes.byref(ch), ctypes.byref(it)))
         return labels, it.value
 
-    def find_textons(self, grey, precision=PREC_FP32):
+    def find_textons(self, grey, precision=PREC_FP32, texton_choice=TEXTON32):
         h, w = grey.shape
         labels = self.empty(h, w, dtype=self._torch.int32)
         it = ctypes.c_int()
-        check(lib().dmsc_find_textons(self.handle, w, h, _p(grey), _p(labels), TEXTON32, precision,
+        check(lib().dmsc_find_textons(self.handle, w, h, _p(grey), _p(labels), texton_choice, precision,
                                       ctypes.byref(it)))
         return labels, it.value
 
-    def local_cues(self, L, a, b, textons, raw=False):
+    def local_cues(self, L, a, b, textons, raw=False, texton_choice=TEXTON32):
         """Returns (bg, cga, cgb, tg) as [24, N] views of pitched stacks (and raw [4,3,8,N])."""
         h, w = L.shape
         n = w * h
@@ -139,18 +140,18 @@ class Context:
         if raw:
             r = self.empty(4, 3, 8, n)
             check(lib().dmsc_local_cues_raw(self.handle, w, h, _p(L), _p(a), _p(b), _p(textons),
-                                            *[_p(c) for c in cues], pitch, _p(r)))
+                                            *[_p(c) for c in cues], pitch, _p(r), texton_choice))
             return [c[:, :n] for c in cues], r
         check(lib().dmsc_local_cues(self.handle, w, h, _p(L), _p(a), _p(b), _p(textons),
-                                    *[_p(c) for c in cues], pitch, TEXTON32))
+                                    *[_p(c) for c in cues], pitch, texton_choice))
         return [c[:, :n] for c in cues]
 
-    def combine(self, w, h, bg, cga, cgb, tg):
+    def combine(self, w, h, bg, cga, cgb, tg, texton_choice=TEXTON32):
         """Cue stacks given as pitched [24, pitch] tensors; returns pitched (mpb, combinedG)."""
         pitch = bg.shape[1]
         mpb, cg = self.zeros(8, pitch), self.zeros(8, pitch)
         check(lib().dmsc_combine(self.handle, w, h, pitch, _p(bg), _p(cga), _p(cgb), _p(tg), _p(mpb), _p(cg),
-                                 TEXTON32))
+                                 texton_choice))
         return mpb, cg
 
     def non_max_suppression(self, w, h, pb):
@@ -214,7 +215,7 @@ class Context:
         check(lib().dmsc_normalize_gpb_all(self.handle, npix, out.shape[0], out.shape[1], _p(out)))
         return out
 
-    def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False):
+    def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False, texton_choice=TEXTON32):
         """Whole pipeline on a device RGB8 image (int32/uint32 [h, w] tensor)."""
         h, w = rgb.shape
         n = w * h
@@ -231,7 +232,7 @@ class Context:
         o.d_nmax, o.d_eigvecs = out["nmax"].data_ptr(), out["eigvecs"].data_ptr()
         o.d_spb, o.d_gpb_all, o.d_gpb = out["spb"].data_ptr(), out["gpb_all"].data_ptr(), out["gpb"].data_ptr()
         o.pitch = pitch
-        p = GpbParams(n_eig, tol, TEXTON32, precision)
+        p = GpbParams(n_eig, tol, texton_choice, precision)
         check(lib().dmsc_gpb(self.handle, w, h, _p(rgb), ctypes.byref(p), ctypes.byref(o), 1 if timing else 0))
         out["eigvals"] = np.array(o.h_eigvals[:n_eig], np.float32)
         out["kmeans_iterations"] = o.kmeans_iterations

@@ -64,7 +64,7 @@ SIGNATURES = {
     "dmsc_kmeans": (c_int, [P, c_int, c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P]),
     "dmsc_find_textons": (c_int, [P, c_int, c_int, P, P, c_int, c_int, P]),
     "dmsc_local_cues": (c_int, [P, c_int, c_int, P, P, P, P, P, P, P, P, c_int, c_int]),
-    "dmsc_local_cues_raw": (c_int, [P, c_int, c_int, P, P, P, P, P, P, P, P, c_int, P]),
+    "dmsc_local_cues_raw": (c_int, [P, c_int, c_int, P, P, P, P, P, P, P, P, c_int, P, c_int]),
     "dmsc_combine": (c_int, [P, c_int, c_int, c_int, P, P, P, P, P, P, c_int]),
     "dmsc_nonmax": (c_int, [P, c_int, c_int, P, c_int, P]),
     "dmsc_calc_gpb": (c_int, [P, c_int, c_int, c_int, P, P, P, P, P]),

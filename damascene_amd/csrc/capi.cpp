@@ -74,6 +74,9 @@ int Ctx::texton_coeffs(const float** out) {
 
 using namespace dmsc;
 
+// texton_choice 1 -> 32 textons, 2 -> 64 (texton.cu:65-67)
+static int texton_count(int texton_choice) { return texton_choice == DMSC_TEXTON64 ? 64 : 32; }
+
 #define CHECK_CTX(ctx) DMSC_ARG((ctx) != nullptr, "null context")
 #define CHECK_SIZE(w, h) \
   DMSC_ARG((w) >= kMinSide && (h) >= kMinSide && (long)(w) * (h) < (1L << 28), "image size %dx%d out of range", (w), (h))
@@ -288,7 +291,7 @@ int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* r
                 int conv_thresh, int precision, int* changes, int* iterations) {
   CHECK_CTX(ctx);
   DMSC_ARG(w > 0 && h > 0 && resp && labels, "bad arguments");
-  DMSC_ARG(k == kTextons && nfilters == kNFilters, "kmeans supports k=32 over 34 filters");
+  DMSC_ARG((k == 32 || k == 64) && nfilters == kNFilters, "kmeans supports k=32 or 64 over 34 filters");
   DMSC_ARG(max_iter >= 1 && max_iter <= 1000, "max_iter out of range");
   DMSC_ARG(conv_thresh == 0, "only conv_thresh=0 (the reference default) is supported");
   DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
@@ -299,7 +302,7 @@ int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* r
   DMSC_TRY(ctx->scratch(2, st + sizeof(float) * (size_t)pitch_floats(n) + mixed_bytes, &scr));
   float* pdot = (float*)((char*)scr + st);
   if (precision == DMSC_PREC_MIXED) {
-    DMSC_TRY(launch_kmeans_mixed(ctx->stream, w, h, resp, labels, pdot, scr, max_iter,
+    DMSC_TRY(launch_kmeans_mixed(ctx->stream, w, h, k, resp, labels, pdot, scr, max_iter,
                                  (char*)scr + st + sizeof(float) * (size_t)pitch_floats(n)));
   } else {
     DMSC_TRY(launch_kmeans(ctx->stream, w, h, k, nfilters, resp, labels, pdot, scr, max_iter));
@@ -312,13 +315,15 @@ int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* grey, int* texto
   CHECK_CTX(ctx);
   CHECK_SIZE(w, h);
   DMSC_ARG(grey && textons, "bad arguments");
-  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  DMSC_ARG(texton_choice == DMSC_TEXTON32 || texton_choice == DMSC_TEXTON64,
+           "texton_choice must be 1 (32 textons) or 2 (64 textons)");
   const int n = w * h;
   void* r;
   DMSC_TRY(ctx->scratch(0, sizeof(float) * (size_t)kNFilters * n, &r));
   DMSC_TRY(dmsc_texton_responses(ctx, w, h, grey, (float*)r));
   int changes;
-  return dmsc_kmeans(ctx, w, h, kTextons, kNFilters, (float*)r, textons, 15, 0, precision, &changes, iterations);
+  return dmsc_kmeans(ctx, w, h, texton_count(texton_choice), kNFilters, (float*)r, textons, 15, 0, precision,
+                     &changes, iterations);
 }
 
 // ----------------------------------------------------------------- cues
@@ -328,18 +333,23 @@ int dmsc_local_cues(dmsc_ctx* ctx, int w, int h, const float* L, const float* a,
   CHECK_SIZE(w, h);
   DMSC_ARG(L && a && b && tex && bg && cga && cgb && tg, "null pointer");
   DMSC_ARG(cue_pitch >= w * h, "cue pitch smaller than the image");
-  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
-  return launch_local_cues(ctx, w, h, L, a, b, tex, bg, cga, cgb, tg, cue_pitch, nullptr);
+  DMSC_ARG(texton_choice == DMSC_TEXTON32 || texton_choice == DMSC_TEXTON64,
+           "texton_choice must be 1 (32 textons) or 2 (64 textons)");
+  return launch_local_cues(ctx, w, h, L, a, b, tex, texton_count(texton_choice), bg, cga, cgb, tg, cue_pitch,
+                           nullptr);
 }
 
 int dmsc_local_cues_raw(dmsc_ctx* ctx, int w, int h, const float* L, const float* a, const float* b,
                         const int* tex, float* bg, float* cga, float* cgb, float* tg, int cue_pitch,
-                        float* raw) {
+                        float* raw, int texton_choice) {
   CHECK_CTX(ctx);
   CHECK_SIZE(w, h);
   DMSC_ARG(L && a && b && tex && bg && cga && cgb && tg && raw, "null pointer");
   DMSC_ARG(cue_pitch >= w * h, "cue pitch smaller than the image");
-  return launch_local_cues(ctx, w, h, L, a, b, tex, bg, cga, cgb, tg, cue_pitch, raw);
+  DMSC_ARG(texton_choice == DMSC_TEXTON32 || texton_choice == DMSC_TEXTON64,
+           "texton_choice must be 1 (32 textons) or 2 (64 textons)");
+  return launch_local_cues(ctx, w, h, L, a, b, tex, texton_count(texton_choice), bg, cga, cgb, tg, cue_pitch,
+                           raw);
 }
 
 // ----------------------------------------------------------------- combine / nonmax / gPb
@@ -347,8 +357,9 @@ int dmsc_combine(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* bg, co
                  const float* tg, float* mpb, float* cg, int texton_choice) {
   CHECK_CTX(ctx);
   DMSC_ARG(w > 0 && h > 0 && cue_pitch >= w * h && bg && cga && cgb && tg && mpb && cg, "bad arguments");
-  DMSC_ARG(texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
-  return launch_combine(ctx->stream, w * h, cue_pitch, bg, cga, cgb, tg, mpb, cg);
+  DMSC_ARG(texton_choice == DMSC_TEXTON32 || texton_choice == DMSC_TEXTON64,
+           "texton_choice must be 1 (32 textons) or 2 (64 textons)");
+  return launch_combine(ctx->stream, w * h, cue_pitch, bg, cga, cgb, tg, texton_count(texton_choice), mpb, cg);
 }
 
 int dmsc_nonmax(dmsc_ctx* ctx, int w, int h, const float* pb, int pitch, float* nmax) {
@@ -426,7 +437,9 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   dmsc_gpb_params p;
   if (prm) p = *prm; else dmsc_gpb_default_params(&p);
   DMSC_ARG(p.n_eig >= 2 && p.n_eig <= 25, "n_eig must be in [2, 25]");
-  DMSC_ARG(p.texton_choice == DMSC_TEXTON32, "only texton_choice 1 (32 textons) is supported");
+  DMSC_ARG(p.texton_choice == DMSC_TEXTON32 || p.texton_choice == DMSC_TEXTON64,
+           "texton_choice must be 1 (32 textons) or 2 (64 textons)");
+  const int ntex = texton_count(p.texton_choice);
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const int pitch = out->pitch ? out->pitch : pitch_floats(n);
@@ -477,19 +490,19 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
     DMSC_TRY(ctx->scratch(2, st + sizeof(float) * P + mixed_bytes, &scr));
     float* pdot = (float*)((char*)scr + st);
     if (p.precision == DMSC_PREC_MIXED && kmeans_mixed_work_bytes(n) > 0)
-      DMSC_TRY(launch_kmeans_mixed(s, w, h, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
+      DMSC_TRY(launch_kmeans_mixed(s, w, h, ntex, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
     else
-      DMSC_TRY(launch_kmeans(s, w, h, kTextons, kNFilters, resp, tex, pdot, scr, 15));
+      DMSC_TRY(launch_kmeans(s, w, h, ntex, kNFilters, resp, tex, pdot, scr, 15));
     int ch = 0;
     DMSC_TRY(kmeans_read_state(s, scr, &ch, &out->kmeans_iterations));
   }
   if (timing) DMSC_HIP(hipEventRecord(ev[1], s));
   // damascene.cu:352 localCues
-  DMSC_TRY(launch_local_cues(ctx, w, h, L, A, Bc, tex, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, pitch,
-                             nullptr));
+  DMSC_TRY(launch_local_cues(ctx, w, h, L, A, Bc, tex, ntex, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P,
+                             pitch, nullptr));
   if (timing) DMSC_HIP(hipEventRecord(ev[2], s));
   // damascene.cu:388-404 combine + nonMaxSuppression
-  DMSC_TRY(launch_combine(s, n, pitch, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, mpb, cg));
+  DMSC_TRY(launch_combine(s, n, pitch, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, ntex, mpb, cg));
   DMSC_TRY(launch_nonmax(s, w, h, mpb, pitch, spb, (int*)all, nmax));  // spb/all as scratch here
   if (timing) DMSC_HIP(hipEventRecord(ev[3], s));
   // damascene.cu:417-420 intervene

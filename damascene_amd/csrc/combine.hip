@@ -112,8 +112,14 @@ __global__ void k_calc_gpb(int n, int pitch, int norient, const float* __restric
 
 }  // namespace
 
-// combine.cu:16-21 (32-texton tables).
-static CombineW weights32() {
+// combine.cu:16-21: the 32-texton tables (lines 17, 21) and the 64-texton
+// ones (lines 16, 20), which differ in the three Tg entries.
+static CombineW combine_weights(int ntex) {
+  if (ntex == 64)
+    return CombineW{{0.0123f, 0.0110f, 0.0117f, 0.0169f, 0.0176f, 0.0198f, 0.0138f, 0.0138f, 0.0145f,
+                     0.0104f, 0.0105f, 0.0115f},
+                    {0.f, 0.f, 0.0028f, 0.0041f, 0.0042f, 0.0047f, 0.0033f, 0.0033f, 0.0035f, 0.0025f,
+                     0.0025f, 0.0137f}};
   return CombineW{{0.0123f, 0.0110f, 0.0117f, 0.0169f, 0.0176f, 0.0198f, 0.0138f, 0.0138f, 0.0145f,
                    0.0144f, 0.0149f, 0.0158f},
                   {0.f, 0.f, 0.0028f, 0.0041f, 0.0042f, 0.0047f, 0.0033f, 0.0033f, 0.0035f, 0.0035f,
@@ -121,9 +127,10 @@ static CombineW weights32() {
 }
 
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga,
-                   const float* cgb, const float* tg, float* mpb, float* cg) {
+                   const float* cgb, const float* tg, int ntex, float* mpb, float* cg) {
+  if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "combine supports 32 or 64 textons");
   hipLaunchKernelGGL(k_combine, dim3(div_up(n, 256), 8), dim3(256), 0, s, n, pitch, bg, cga, cgb, tg,
-                     weights32(), mpb, cg);
+                     combine_weights(ntex), mpb, cg);
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
 }

@@ -21,19 +21,19 @@ int launch_texton_conv(hipStream_t s, int w, int h, const float* grey, const flo
 size_t kmeans_state_bytes();
 int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp, int* labels, float* pdot,
                   void* state, int max_iter);
-int launch_kmeans_mixed(hipStream_t s, int w, int h, const float* resp, int* labels, float* pdot, void* state,
-                        int max_iter, void* work);
+int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, int* labels, float* pdot,
+                        void* state, int max_iter, void* work);
 size_t kmeans_mixed_work_bytes(int n);
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations);
 
 // ------------------------------------------------------------- local cues
 size_t local_cues_scratch_bytes(int w, int h);
 int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B, const int* tex,
-                      float* bg, float* cga, float* cgb, float* tg, int pitch, float* grads_out);
+                      int ntex, float* bg, float* cga, float* cgb, float* tg, int pitch, float* grads_out);
 
 // ------------------------------------------------------------- combine / nonmax / gPb
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga, const float* cgb,
-                   const float* tg, float* mpb, float* cg);
+                   const float* tg, int ntex, float* mpb, float* cg);
 int launch_nonmax(hipStream_t s, int w, int h, const float* pb, int pitch, float* mx, int* ori, float* nmax);
 int launch_calc_gpb(hipStream_t s, int n, int pitch, int norient, const float* cg, const float* spb,
                     const float* mpb, float* all, float* gpb);

@@ -203,7 +203,10 @@ __global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs b, int m) {
 //            = sum z'^2 + alpha_k^2 (sum q^2 - 2),
 // which in FP64 loses at most eps64 |z'|^2 / beta^2 (< 1e-12 here), then
 // gathers q_{k+1}[c] = (z'_k[c] - alpha_k q_k[c]) / beta_k on the fly.  The
-// basis keeps a zero row before q_0 so step 0 needs no special case.
+// basis keeps a zero row before q_0 so step 0 needs no special case.  The
+// partials are double-buffered by step parity: kernel k reads buffer
+// (k-1)&1 in every block while its finished blocks already write buffer
+// k&1, so no block can overwrite a partial another block has yet to read.
 struct LzmBufs {
   const float* A;
   size_t pitch;
@@ -211,23 +214,24 @@ struct LzmBufs {
   size_t vstride;
   double* z0;  // z' of even steps
   double* z1;  // z' of odd steps (holds q_0 before step 0)
-  double* p1;  // [nb] q.z' partials
-  double* p2;  // [nb] |z'|^2 partials
-  double* p3;  // [nb] |q|^2 partials
+  double* p1;  // [2][nb] q.z' partials, buffer = step parity
+  double* p2;  // [2][nb] |z'|^2 partials
+  double* p3;  // [2][nb] |q|^2 partials
   double* alpha;
   double* beta;
   int nb;
   LzState* st;
 };
 
-// alpha and beta of step k-1 from the partials of kernel k-1 (all blocks
-// reduce in the same order).
-__device__ __forceinline__ void lzm_coeffs(const LzmBufs& b, double* red, double& alpha, double& beta) {
+// alpha and beta of step k-1 from the partials kernel k-1 left in buffer
+// (k-1)&1 (all blocks reduce in the same order).
+__device__ __forceinline__ void lzm_coeffs(const LzmBufs& b, int k, double* red, double& alpha, double& beta) {
+  const size_t o = (size_t)((k - 1) & 1) * b.nb;
   double v1 = 0, v2 = 0, v3 = 0;
   for (int i = threadIdx.x; i < b.nb; i += kK1) {
-    v1 += b.p1[i];
-    v2 += b.p2[i];
-    v3 += b.p3[i];
+    v1 += b.p1[o + i];
+    v2 += b.p2[o + i];
+    v3 += b.p3[o + i];
   }
   alpha = block_sum<kK1>(v1, red);
   const double s2 = block_sum<kK1>(v2, red), s3 = block_sum<kK1>(v3, red);
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t
   const double* zp = (k & 1) ? b.z0 : b.z1;
   double* zc = (k & 1) ? b.z1 : b.z0;
   double alpha, beta;
-  lzm_coeffs(b, red, alpha, beta);  // step k-1 (alpha 0, beta 1 before step 0)
+  lzm_coeffs(b, k, red, alpha, beta);  // step k-1 (alpha 0, beta 1 before step 0)
   if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
     b.alpha[k - 1] = alpha;
     b.beta[k - 1] = beta;
@@ -277,9 +281,10 @@ __global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t
   d2 = block_sum<kK1>(d2, red);
   d3 = block_sum<kK1>(d3, red);
   if (threadIdx.x == 0) {
-    b.p1[blockIdx.x] = d1;
-    b.p2[blockIdx.x] = d2;
-    b.p3[blockIdx.x] = d3;
+    const size_t o = (size_t)(k & 1) * b.nb + blockIdx.x;
+    b.p1[o] = d1;
+    b.p2[o] = d2;
+    b.p3[o] = d3;
   }
 }
 
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t
 __global__ __launch_bounds__(kK1) void k_lzm_finish(LzmBufs b, int m) {
   __shared__ double red[kK1 / kWave];
   double alpha, beta;
-  lzm_coeffs(b, red, alpha, beta);
+  lzm_coeffs(b, m, red, alpha, beta);
   if (threadIdx.x == 0) {
     b.alpha[m - 1] = alpha;
     b.beta[m - 1] = beta;
@@ -562,10 +567,10 @@ __global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n, double* p1
     V[i] = q0;
     z1[i] = (Z)q0;
   }
-  if (p1 && i < nb) {
-    p1[i] = 0.0;
-    p2[i] = i == 0 ? 1.0 : 0.0;
-    p3[i] = 0.0;
+  if (p1 && i < nb) {  // the step -1 buffer (parity 1): alpha 0, beta 1
+    p1[nb + i] = 0.0;
+    p2[nb + i] = i == 0 ? 1.0 : 0.0;
+    p3[nb + i] = 0.0;
   }
 }
 
@@ -719,8 +724,8 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const size_t o_st = carve(sizeof(LzState));
   const size_t o_z0 = carve(zb * vstride), o_z1 = carve(zb * vstride);
   const size_t o_rs = carve(sizeof(float) * vstride);
-  const size_t o_p1 = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb1),
-               o_p3 = carve(sizeof(double) * nb1);
+  const size_t o_p1 = carve(sizeof(double) * 2 * nb1), o_p2 = carve(sizeof(double) * 2 * nb1),
+               o_p3 = carve(sizeof(double) * 2 * nb1);
   const size_t o_al = carve(zb * (kMaxIter + 2)), o_be = carve(zb * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
   const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);

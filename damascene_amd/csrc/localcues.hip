@@ -3,7 +3,7 @@
 //
 // All 4 channels x 4 rotations (16 frames) are processed together so every
 // launch has enough waves to fill 256 CUs:
-//   1. labels:    quantised L/a/b (25 bins) or textons (32), mirrored (border
+//   1. labels:    quantised L/a/b (25 bins) or textons (32/64), mirrored (border
 //                 30) and scattered through the Bresenham rotation tables into
 //                 uint8 rotated frames (255 = outside the rotated image);
 //   2. integrals: per-bin 2-D inclusive counts, row pass then column pass,
@@ -13,12 +13,14 @@
 //                 the bytes of int32;
 //   3. gradients: per pixel, scale and split: 4+4 corner gathers per bin,
 //                 optional bin blur, normalisation to sum 8 and chi-squared
-//                 with the reference's 32-slot tree sums;
+//                 with the reference's 32-slot tree sums (64 textons: two
+//                 lanes per pixel, one 32-bin half each);
 //   4. parabola:  Savitzky-Golay smoothing of each mirrored gradient plane
 //                 over the filter's non-zero taps (v outer, u inner, fma).
 #include <cstring>
 #include <map>
 #include <memory>
+#include <tuple>
 
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
@@ -38,7 +40,12 @@ struct FrameGeom {
   long long int_off[kFrames];  // uint16 element offset of frame f (integral buffer)
   int tab_off[4];              // offset of orientation o's tables: bX[bw] bY[bw] gX[bh] gY[bh]
   int rect[4][4][8];           // [orientation][radius index][corner A..H]
+  int tbp;                     // bins per integral pixel of the texture frames (32 or 64)
 };
+
+// Bins stored per integral-image pixel of frame f: 32 for the L/a/b frames
+// (25 used), 32 or 64 for the texton frames 12..15.
+__device__ __forceinline__ int bin_pitch(const FrameGeom& g, int f) { return f >= 12 ? g.tbp : 32; }
 
 struct CueOut {
   float* p[4];  // Bg, Cga, Cgb, Tg; each [24][pitch]
@@ -73,40 +80,43 @@ __global__ void k_cue_labels(FrameGeom g, const float* __restrict__ L, const flo
 }
 
 // gradient.cu:66-87: per-bin inclusive prefix counts along each row of a
-// rotated frame.  Block = one row; thread = (bin, one of 8 row segments).
-__global__ __launch_bounds__(256) void k_integral_rows(FrameGeom g, const uint8_t* __restrict__ frames,
+// rotated frame with BP bins per pixel.  Block = one row; thread = (bin,
+// one of 256/BP row segments).  Frames f0 + blockIdx.y.
+template <int BP>
+__global__ __launch_bounds__(256) void k_integral_rows(FrameGeom g, int f0, const uint8_t* __restrict__ frames,
                                                        uint16_t* __restrict__ I) {
+  constexpr int NSEG = 256 / BP;
   extern __shared__ uint8_t row[];
-  __shared__ uint32_t segtot[8][32];
-  const int f = blockIdx.y, o = f & 3, y = blockIdx.x;
+  __shared__ uint32_t segtot[NSEG][BP];
+  const int f = f0 + blockIdx.y, o = f & 3, y = blockIdx.x;
   const int nw = g.nw[o];
   if (y >= g.nh[o]) return;
   const uint8_t* src = frames + g.lab_off[f] + (size_t)y * nw;
   for (int i = threadIdx.x; i < nw; i += 256) row[i] = src[i];
   __syncthreads();
-  const int bin = threadIdx.x & 31, seg = threadIdx.x >> 5;
-  const int len = (nw + 7) / 8, x0 = seg * len, x1 = min(nw, x0 + len);
+  const int bin = threadIdx.x % BP, seg = threadIdx.x / BP;
+  const int len = (nw + NSEG - 1) / NSEG, x0 = seg * len, x1 = min(nw, x0 + len);
   uint32_t c = 0;
   for (int x = x0; x < x1; x++) c += (row[x] == bin);
   segtot[seg][bin] = c;
   __syncthreads();
   uint32_t run = 0;
   for (int s = 0; s < seg; s++) run += segtot[s][bin];
-  uint16_t* dst = I + g.int_off[f] + (size_t)y * nw * 32;
+  uint16_t* dst = I + g.int_off[f] + (size_t)y * nw * BP;
   for (int x = x0; x < x1; x++) {
     run += (row[x] == bin);
-    dst[(size_t)x * 32 + bin] = (uint16_t)run;
+    dst[(size_t)x * BP + bin] = (uint16_t)run;
   }
 }
 
 // gradient.cu:89-103: column pass, two bins per thread, modulo 2^16.
 __global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, uint16_t* __restrict__ I) {
   const int f = blockIdx.y, o = f & 3;
-  const int nw = g.nw[o], nh = g.nh[o];
+  const int nw = g.nw[o], nh = g.nh[o], pairs = bin_pitch(g, f) / 2;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (x, bin pair)
-  if (t >= nw * 16) return;
+  if (t >= nw * pairs) return;
   uint32_t* col = reinterpret_cast<uint32_t*>(I + g.int_off[f]) + t;
-  const size_t stride = (size_t)nw * 16;
+  const size_t stride = (size_t)nw * pairs;
   uint32_t lo = 0, hi = 0;
   int y = 0;
   for (; y + 8 <= nh; y += 8) {
@@ -141,9 +151,14 @@ struct BlurK {
 // bin (uint16 modulo arithmetic recovers the exact counts), optional bin
 // blur, normalisation to sum 8 and 0.5 * sum (a-b)^2/(a+b) with the
 // reference's 32-slot tree sums (rotate.cu:352-472).
-template <int NB, int KR>
+// PAIR: the pixel's 64 bins are split over this lane (32 bins in pk) and
+// its neighbour lane (lane ^ 1); each 32-slot tree sum is completed with the
+// partner's as lo + hi (rotate.cu:384-389 adds slot 32 to slot 0), and IEEE
+// addition being commutative both lanes hold the same total.
+template <int NB, int KR, bool PAIR = false>
 __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0, int i1, int i2, int i3, int j0,
                                             int j1, int j2, int j3, const BlurK& bk) {
+  static_assert(!PAIR || (NB == 32 && KR == 0), "paired halves hold 32 unblurred bins");
   float a[32], b[32];
 #pragma unroll
   for (int j = 0; j < 32; j++) {
@@ -186,11 +201,13 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
   float t32[32];
 #pragma unroll
   for (int i = 0; i < 32; i++) t32[i] = a[i];
-  const float sa = dev::tree32(t32);
+  float sa = dev::tree32(t32);
+  if (PAIR) sa = sa + __shfl_xor(sa, 1, 64);
   const float na = (sa != 0.f) ? 8.0f / sa : 0.f;
 #pragma unroll
   for (int i = 0; i < 32; i++) t32[i] = b[i];
-  const float sb = dev::tree32(t32);
+  float sb = dev::tree32(t32);
+  if (PAIR) sb = sb + __shfl_xor(sb, 1, 64);
   const float nb = (sb != 0.f) ? 8.0f / sb : 0.f;
 #pragma unroll
   for (int i = 0; i < 32; i++) {
@@ -203,7 +220,9 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
     }
     t32[i] = chi;
   }
-  return 0.5f * dev::tree32(t32);
+  float c = dev::tree32(t32);
+  if (PAIR) c = c + __shfl_xor(c, 1, 64);
+  return 0.5f * c;
 }
 
 // Both splits of one (pixel, scale, orientation): the eight corners A..H
@@ -252,6 +271,51 @@ __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad
   float* dst = grads + (size_t)((ch * 3 + scale) * 8) * plane + p;
   dst[(size_t)(o + 4) * plane] = top;
   dst[(size_t)o * plane] = left;
+}
+
+// The texture channel with 64 textons (rotate.cu:780-782,
+// computeGradient<64, 64, false, *>): lanes 2p and 2p+1 share pixel p,
+// holding bins 0..31 and 32..63 of every corner (adjacent 64-byte halves
+// of the pixel's 128-byte integral record).  Output as k_gradients.
+__global__ __launch_bounds__(256) void k_gradients64(FrameGeom g, int rad0, const int* __restrict__ tabs,
+                                                     const uint16_t* __restrict__ I, float* __restrict__ grads,
+                                                     size_t plane) {
+  const int n = g.w * g.h;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int half = gt & 1;
+  const bool live = (gt >> 1) < n;
+  const int p = live ? (gt >> 1) : n - 1;  // both lanes of a pair agree; all lanes reach the shuffles
+  const int scale = blockIdx.y;
+  const int ch = 3, o = blockIdx.z, f = ch * 4 + o;
+  const int y = p / g.w, x = p - y * g.w;
+  const int* t = tabs + g.tab_off[o];
+  const int ix = x + kBorder, iy = y + kBorder;
+  const int rx = t[2 * g.bw + iy] + t[ix];
+  const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
+  const long long pix = (long long)ry * g.nw[o] + rx;
+  const int* rc = g.rect[o][rad0 + scale];
+  const uint16_t* base = I + g.int_off[f] + 32 * half;
+  uint32_t pk[8][16];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint4* src = reinterpret_cast<const uint4*>(base + (size_t)(rc[c] + pix) * 64);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = src[q];
+      pk[c][4 * q] = v.x;
+      pk[c][4 * q + 1] = v.y;
+      pk[c][4 * q + 2] = v.z;
+      pk[c][4 * q + 3] = v.w;
+    }
+  }
+  BlurK none{};
+  const float top = chi2_split<32, 0, true>(pk, 0, 2, 4, 3, 7, 5, 3, 4, none);
+  const float left = chi2_split<32, 0, true>(pk, 0, 5, 6, 1, 7, 2, 1, 6, none);
+  if (live && half == 0) {
+    float* dst = grads + (size_t)((ch * 3 + scale) * 8) * plane + p;
+    dst[(size_t)(o + 4) * plane] = top;
+    dst[(size_t)o * plane] = left;
+  }
 }
 
 // Parabola smoothing (parabola.cu:75-124 / parabola_kernel.cu:1-30) of one
@@ -349,10 +413,11 @@ struct CuePlan {
   }
 };
 
-int build_plan(hipStream_t s, int w, int h, CuePlan& P) {
+int build_plan(hipStream_t s, int w, int h, int tbp, CuePlan& P) {
   FrameGeom& g = P.g;
   memset(&g, 0, sizeof(g));
   g.w = w; g.h = h; g.bw = w + 2 * kBorder; g.bh = h + 2 * kBorder;
+  g.tbp = tbp;
   std::vector<int> tabs;
   std::vector<int> bX(g.bw), bY(g.bw), gX(g.bh), gY(g.bh);
   for (int o = 0; o < 4; o++) {
@@ -401,7 +466,7 @@ int build_plan(hipStream_t s, int w, int h, CuePlan& P) {
     g.lab_off[f] = (long long)lo;
     g.int_off[f] = (long long)io;
     lo += ((size_t)g.nw[o] * g.nh[o] + 255) & ~(size_t)255;
-    io += (size_t)g.nw[o] * g.nh[o] * 32;
+    io += (size_t)g.nw[o] * g.nh[o] * (f >= 12 ? tbp : 32);
   }
   P.lab_bytes = lo;
   P.int_elems = io;
@@ -440,20 +505,20 @@ int build_plan(hipStream_t s, int w, int h, CuePlan& P) {
 }
 
 struct PlanCache {
-  std::map<std::pair<int, int>, std::unique_ptr<CuePlan>> plans;
+  std::map<std::tuple<int, int, int>, std::unique_ptr<CuePlan>> plans;
 };
 
 }  // namespace
 
-// Cached per (w, h) on the context.
-static int get_plan(Ctx* ctx, int w, int h, CuePlan** out) {
+// Cached per (w, h, texture bin pitch) on the context.
+static int get_plan(Ctx* ctx, int w, int h, int tbp, CuePlan** out) {
   static std::map<Ctx*, PlanCache> caches;  // keyed by context; cleared by ctx teardown
   PlanCache& pc = caches[ctx];
-  auto key = std::make_pair(w, h);
+  auto key = std::make_tuple(w, h, tbp);
   auto it = pc.plans.find(key);
   if (it == pc.plans.end()) {
     std::unique_ptr<CuePlan> p(new CuePlan());
-    DMSC_TRY(build_plan(ctx->stream, w, h, *p));
+    DMSC_TRY(build_plan(ctx->stream, w, h, tbp, *p));
     it = pc.plans.emplace(key, std::move(p)).first;
   }
   *out = it->second.get();
@@ -465,18 +530,20 @@ size_t local_cues_scratch_bytes(int w, int h) {
   // generous bound: rotated frames are at most (bw+bh)^2
   const size_t rot = (size_t)(bw + bh) * (bw + bh);
   const size_t lab = 16 * (rot + 256);
-  const size_t ints = 16 * rot * 32 * sizeof(uint16_t);
+  const size_t ints = (12 * 32 + 4 * 64) * rot * sizeof(uint16_t);  // texture frames at up to 64 bins
   const size_t grads = (size_t)4 * 24 * w * h * sizeof(float);
   return lab + ints + grads + 4096;
 }
 
 // localcues.cu:257-300 localCues for all four cues.
 int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B,
-                      const int* tex, float* bg, float* cga, float* cgb, float* tg, int pitch,
+                      const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg, int pitch,
                       float* grads_out) {
   hipStream_t s = ctx->stream;
+  if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "local cues support 32 or 64 textons");
+  const int tbp = ntex;
   CuePlan* P;
-  DMSC_TRY(get_plan(ctx, w, h, &P));
+  DMSC_TRY(get_plan(ctx, w, h, tbp, &P));
   const FrameGeom& g = P->g;
   const int n = w * h;
   void* scratch;
@@ -491,8 +558,13 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
                      tex, P->d_tabs, frames);
   int maxh = 0, maxw = 0;
   for (int o = 0; o < 4; o++) { maxh = std::max(maxh, g.nh[o]); maxw = std::max(maxw, g.nw[o]); }
-  hipLaunchKernelGGL(k_integral_rows, dim3(maxh, kFrames), dim3(256), (size_t)maxw, s, g, frames, I);
-  hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 16, 256), kFrames), dim3(256), 0, s, g, I);
+  if (tbp == 32) {
+    hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, kFrames), dim3(256), (size_t)maxw, s, g, 0, frames, I);
+  } else {
+    hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 12), dim3(256), (size_t)maxw, s, g, 0, frames, I);
+    hipLaunchKernelGGL(k_integral_rows<64>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+  }
+  hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * tbp / 2, 256), kFrames), dim3(256), 0, s, g, I);
   const size_t plane = (size_t)n;
   BlurK none;
   memset(&none, 0, sizeof(none));
@@ -502,8 +574,12 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
                      P->d_tabs, I, P->blur_bg, grads, plane);
   hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 3, 8), dim3(256), 0, s, g, 1, 1,
                      P->d_tabs, I, P->blur_cg, grads, plane);
-  hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
-                     P->d_tabs, I, none, grads, plane);
+  if (ntex == 64)
+    hipLaunchKernelGGL(k_gradients64, dim3(div_up(2 * n, 256), 3, 4), dim3(256), 0, s, g, 1, P->d_tabs, I, grads,
+                       plane);
+  else
+    hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
+                       P->d_tabs, I, none, grads, plane);
   CueOut out{{bg, cga, cgb, tg}};
   for (int scale = 0; scale < 3; scale++) {
     for (int grp = 0; grp < 2; grp++) {

@@ -9,9 +9,9 @@
 // loads).
 //
 // k-means: per iteration one accumulation kernel (int32 wrap-around centroid
-// sums via LDS atomics, exact and order independent) and one assignment
-// kernel (centroids rebuilt per block, 32 distances per pixel in the
-// reference order, argmin with strict '<').  A device-side done flag lets
+// sums via LDS atomics, exact and order independent), one centroid kernel
+// and one assignment kernel (centroids staged per block, 32 or 64 distances
+// per pixel in the reference order, argmin with strict '<').  A device-side done flag lets
 // all iterations be enqueued without host round trips.
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
@@ -70,20 +70,26 @@ __global__ void k_zero(float* p, size_t n) {
 }
 
 // ------------------------------------------------------------------ k-means
+// Cluster count K is 32 (texton_choice 1) or 64 (texton_choice 2); the
+// state is sized for 64 and indexed with stride K.
+constexpr int kMaxK = 64;
+
 struct KmState {
-  uint32_t mass[2][kNFilters * 32];  // [buffer][filter][cluster], int32 wrap-around sums
-  uint32_t count[2][32];
+  uint32_t mass[2][kNFilters * kMaxK];  // [buffer][filter][cluster], int32 wrap-around sums
+  uint32_t count[2][kMaxK];
   int changes[2];
   unsigned int done;        // set when an iteration changed no label
   unsigned int iterations;  // iterations executed
   unsigned int rechecks;    // mixed path: pixels re-labelled by the exact FP32 distances
-  float cent[kNFilters][32];  // centroids of the current iteration (FP32)
-  float cdot[32];             // |c|^2 in the reference order
-  alignas(16) uint16_t centb[32][32];  // mixed path: BF16 centroids, filters 0..31, [cluster][filter]
+  float cent[kNFilters][kMaxK];  // centroids of the current iteration (FP32), [filter][cluster]
+  float cdot[kMaxK];             // |c|^2 in the reference order
+  alignas(16) uint16_t centb[kMaxK][32];  // mixed path: BF16 centroids, filters 0..31, [cluster][filter]
 };
 
-// kmeans.cu:14-35 assignInitialClusters (6x6 grid capped at 31) and
+// kmeans.cu:14-35 assignInitialClusters (6x6 grid capped at 31) for K = 32,
+// kmeans.cu:37-55 assignInitialClusters_64 (8x8 grid) for K = 64, and
 // kmeans.cu:296-307 makeSelfDots over the float responses.
+template <int K>
 __global__ void k_km_init(int w, int h, int nf, const float* __restrict__ resp, int* __restrict__ labels,
                           float* __restrict__ pdot, KmState* st) {
   const int n = w * h;
@@ -95,13 +101,14 @@ __global__ void k_km_init(int w, int h, int nf, const float* __restrict__ resp, 
     st->rechecks = 0;
   }
   if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < kNFilters * 32; i += blockDim.x) { st->mass[0][i] = 0; st->mass[1][i] = 0; }
-    if (threadIdx.x < 32) { st->count[0][threadIdx.x] = 0; st->count[1][threadIdx.x] = 0; }
+    for (int i = threadIdx.x; i < kNFilters * K; i += blockDim.x) { st->mass[0][i] = 0; st->mass[1][i] = 0; }
+    if (threadIdx.x < K) { st->count[0][threadIdx.x] = 0; st->count[1][threadIdx.x] = 0; }
   }
   if (p >= n) return;
   const int y = p / w, x = p - y * w;
-  int c = (y / ((h - 1) / 6 + 1)) * 6 + x / ((w - 1) / 6 + 1);
-  labels[p] = c >= 32 ? 31 : c;
+  constexpr int G = K == 64 ? 8 : 6;
+  const int c = (y / ((h - 1) / G + 1)) * G + x / ((w - 1) / G + 1);
+  labels[p] = (K == 32 && c >= 32) ? 31 : c;
   float d = 0.0f;
   for (int f = 0; f < nf; f++) {
     const float r = resp[(size_t)f * n + p];
@@ -123,29 +130,31 @@ __device__ __forceinline__ bool km_stopped(KmState* st, int it) {
   return false;
 }
 
-// kmeans.cu:57-103 findCentroids: int32 sums of (int)(1e5 * response)
-// (INTCONFACTOR, kmeans.h:16), wrap-around arithmetic (order independent).
+// kmeans.cu:57-103 findCentroids / 106-153 findCentroids_64: int32 sums of
+// (int)(1e5 * response) (INTCONFACTOR, kmeans.h:16), wrap-around
+// arithmetic (order independent).
+template <int K>
 __global__ __launch_bounds__(256) void k_km_accum(int n, int nf, const float* __restrict__ resp,
                                                   const int* __restrict__ labels, KmState* st, int it) {
   if (km_stopped(st, it)) return;
-  __shared__ uint32_t smass[kNFilters * 32];
-  __shared__ uint32_t scount[32];
-  for (int i = threadIdx.x; i < nf * 32; i += 256) smass[i] = 0;
-  if (threadIdx.x < 32) scount[threadIdx.x] = 0;
+  __shared__ uint32_t smass[kNFilters * K];
+  __shared__ uint32_t scount[K];
+  for (int i = threadIdx.x; i < nf * K; i += 256) smass[i] = 0;
+  if (threadIdx.x < K) scount[threadIdx.x] = 0;
   __syncthreads();
   const int buf = it & 1;
   for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
     const int c = labels[p];
-    if (c < 0 || c >= 32) continue;  // unassigned pixel (all distances >= 1e7)
+    if (c < 0 || c >= K) continue;  // unassigned pixel (all distances >= 1e7)
 #pragma unroll
     for (int f = 0; f < kNFilters; f++)
-      atomicAdd(&smass[f * 32 + c], (uint32_t)(int)(100000.0f * resp[(size_t)f * n + p]));
+      atomicAdd(&smass[f * K + c], (uint32_t)(int)(100000.0f * resp[(size_t)f * n + p]));
     atomicAdd(&scount[c], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nf * 32; i += 256)
+  for (int i = threadIdx.x; i < nf * K; i += 256)
     if (smass[i]) atomicAdd(&st->mass[buf][i], smass[i]);
-  if (threadIdx.x < 32 && scount[threadIdx.x]) atomicAdd(&st->count[buf][threadIdx.x], scount[threadIdx.x]);
+  if (threadIdx.x < K && scount[threadIdx.x]) atomicAdd(&st->count[buf][threadIdx.x], scount[threadIdx.x]);
 }
 
 __device__ __forceinline__ uint16_t to_bf16(float x) {  // round to nearest even
@@ -155,44 +164,61 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {  // round to nearest even
   return (uint16_t)(u >> 16);
 }
 
-// kmeans.cu:268-276 finishCentroids (an empty cluster gives NaN, which never
-// wins a comparison) and makeSelfDots for the centroids; one block.  Also
-// clears the buffers the next iteration accumulates into.
+// kmeans.cu:268-286 finishCentroids(_64) (an empty cluster gives NaN, which
+// never wins a comparison) and makeSelfDots for the centroids; one block.
+// Also clears the buffers the next iteration accumulates into.
+template <int K>
 __global__ __launch_bounds__(256) void k_km_centroids(int nf, KmState* st, int it) {
   if (st->done) return;
   const int buf = it & 1;
-  for (int i = threadIdx.x; i < kNFilters * 32; i += 256) {
-    const int f = i / 32, c = i - f * 32;
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) {
+    const int f = i / K, c = i - f * K;
     st->cent[f][c] = f < nf ? (float)(int)st->mass[buf][i] / (100000.0f * (float)st->count[buf][c]) : 0.0f;
   }
   __syncthreads();
-  if (threadIdx.x < 32) {
+  if (threadIdx.x < K) {
     const int c = threadIdx.x;
     float d = 0.0f;
     for (int f = 0; f < nf; f++) d = fmaf(st->cent[f][c], st->cent[f][c], d);
     st->cdot[c] = d;
   }
-  for (int i = threadIdx.x; i < 32 * 32; i += 256) {
+  for (int i = threadIdx.x; i < K * 32; i += 256) {
     const int c = i / 32, f = i - c * 32;
     st->centb[c][f] = to_bf16(st->cent[f][c]);
   }
-  for (int i = threadIdx.x; i < kNFilters * 32; i += 256) st->mass[buf ^ 1][i] = 0;
-  if (threadIdx.x < 32) st->count[buf ^ 1][threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) st->mass[buf ^ 1][i] = 0;
+  if (threadIdx.x < K) st->count[buf ^ 1][threadIdx.x] = 0;
   if (threadIdx.x == 0) st->changes[buf] = 0;
+}
+
+// Centroids and their norms staged in LDS by every assignment block.
+template <int K>
+struct KmCent {
+  float cent[kNFilters][K];
+  float cdot[K];
+};
+
+template <int K>
+__device__ __forceinline__ void km_stage(KmCent<K>& sc, const KmState* st) {
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) {
+    const int f = i / K, c = i - f * K;
+    sc.cent[f][c] = st->cent[f][c];
+  }
+  if (threadIdx.x < K) sc.cdot[threadIdx.x] = st->cdot[threadIdx.x];
 }
 
 // Exact FP32 label of one pixel: kmeans.cu:296-338 makeDots + SGEMM(alpha
 // -2, beta 1) + 447-472 findDiffLabels: dist_c = fma(-2, x.c, |x|^2 +
 // |c|^2), x.c accumulated over the filters in order, strict '<' from 1e7.
-__device__ __forceinline__ int km_exact_label(const float (&x)[kNFilters], float pd, const float (*cent)[32],
-                                              const float* cdot) {
+template <int K>
+__device__ __forceinline__ int km_exact_label(const float (&x)[kNFilters], float pd, const KmCent<K>& sc) {
   float best = 10000000.0f;
   int bl = -1;
-  for (int c = 0; c < 32; c++) {
+  for (int c = 0; c < K; c++) {
     float acc = 0.0f;
 #pragma unroll
-    for (int f = 0; f < kNFilters; f++) acc = fmaf(x[f], cent[f][c], acc);
-    const float d = fmaf(-2.0f, acc, pd + cdot[c]);
+    for (int f = 0; f < kNFilters; f++) acc = fmaf(x[f], sc.cent[f][c], acc);
+    const float d = fmaf(-2.0f, acc, pd + sc.cdot[c]);
     if (d < best) {
       best = d;
       bl = c;
@@ -201,15 +227,14 @@ __device__ __forceinline__ int km_exact_label(const float (&x)[kNFilters], float
   return bl;
 }
 
+template <int K>
 __global__ __launch_bounds__(256) void k_km_assign(int n, const float* __restrict__ resp,
                                                    const float* __restrict__ pdot, int* __restrict__ labels,
                                                    KmState* st, int it) {
   if (st->done) return;
-  __shared__ float cent[kNFilters][32];
-  __shared__ float cdot[32];
+  __shared__ KmCent<K> sc;
   __shared__ int schanges;
-  for (int i = threadIdx.x; i < kNFilters * 32; i += 256) (&cent[0][0])[i] = (&st->cent[0][0])[i];
-  if (threadIdx.x < 32) cdot[threadIdx.x] = st->cdot[threadIdx.x];
+  km_stage<K>(sc, st);
   if (threadIdx.x == 0) schanges = 0;
   __syncthreads();
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(256) void k_km_assign(int n, const float* __restric
     float x[kNFilters];
 #pragma unroll
     for (int f = 0; f < kNFilters; f++) x[f] = resp[(size_t)f * n + p];
-    const int bl = km_exact_label(x, pdot[p], cent, cdot);
+    const int bl = km_exact_label<K>(x, pdot[p], sc);
     changed = bl != labels[p];
     labels[p] = bl;
   }
@@ -249,40 +274,39 @@ __global__ void k_km_bf16(int n, const float* __restrict__ resp, uint16_t* __res
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Distances from v_mfma_f32_16x16x32_bf16 products (clusters as rows,
-// pixels as columns, filters 0..31 as K) plus the FP32 filter-32 term.  Per
-// (pixel, cluster) the approximation error against the exact FP32 distance
-// is below e = 2^-5 |x| |c| + 2^-20 (|x|^2 + |c|^2): BF16 rounding of both
-// operands perturbs each product by at most (2^-7 + 2^-16)|x_f c_f|, summing
-// to (2^-7 + 2^-16)|x||c| by Cauchy-Schwarz, times 2 for the -2 factor, and
-// FP32 rounding of both computations stays inside the second term.  The
-// label of the smallest approximate distance is the exact label when its
-// upper bound lies below every other cluster's lower bound (no tie is
-// possible then); any other pixel is re-labelled with the exact FP32
-// distances in the reference order, so labels are bit-exact.
-// Lane layout (checked on the device): A row l%16, B column l%16, K slice
-// 8*(l/16)..+7; result rows 4*(l/16)+i of column l%16.
+// Distances from v_mfma_f32_16x16x32_bf16 products (clusters as rows, in
+// K/16 tiles of 16, pixels as columns, filters 0..31 as K) plus the FP32
+// filter-32 term.  Per (pixel, cluster) the approximation error against the
+// exact FP32 distance is below e = 2^-5 |x| |c| + 2^-20 (|x|^2 + |c|^2):
+// BF16 rounding of both operands perturbs each product by at most (2^-7 +
+// 2^-16)|x_f c_f|, summing to (2^-7 + 2^-16)|x||c| by Cauchy-Schwarz, times
+// 2 for the -2 factor, and FP32 rounding of both computations stays inside
+// the second term.  The label of the smallest approximate distance is the
+// exact label when its upper bound lies below every other cluster's lower
+// bound (no tie is possible then); any other pixel is re-labelled with the
+// exact FP32 distances in the reference order, so labels are bit-exact.
+// Lane layout: A row l%16, B column l%16, K slice 8*(l/16)..+7; result rows
+// 4*(l/16)+i of column l%16.
+template <int K>
 __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __restrict__ resp,
                                                         const uint16_t* __restrict__ xb,
                                                         const float* __restrict__ pdot, int* __restrict__ labels,
                                                         KmState* st, int it) {
+  constexpr int NT = K / 16;  // cluster tiles
   if (st->done) return;
-  __shared__ float cent[kNFilters][32];
-  __shared__ float cdot[32];
+  __shared__ KmCent<K> sc;
   __shared__ int schanges, srecheck;
-  for (int i = threadIdx.x; i < kNFilters * 32; i += 256) (&cent[0][0])[i] = (&st->cent[0][0])[i];
-  if (threadIdx.x < 32) cdot[threadIdx.x] = st->cdot[threadIdx.x];
+  km_stage<K>(sc, st);
   if (threadIdx.x == 0) schanges = srecheck = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
   // A operands: centroid rows (clusters) 16*t + col, filters 8*grp..+7
-  bf16x8 a0, a1;
-  {
-    const uint4 v0 = *reinterpret_cast<const uint4*>(&st->centb[col][8 * grp]);
-    const uint4 v1 = *reinterpret_cast<const uint4*>(&st->centb[16 + col][8 * grp]);
-    __builtin_memcpy(&a0, &v0, 16);
-    __builtin_memcpy(&a1, &v1, 16);
+  bf16x8 a[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const uint4 v = *reinterpret_cast<const uint4*>(&st->centb[16 * t + col][8 * grp]);
+    __builtin_memcpy(&a[t], &v, 16);
   }
   int changed = 0, rech = 0;
   // each wave labels 64 pixels, 16 per MFMA column tile
@@ -295,25 +319,28 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
       const uint4 v = *reinterpret_cast<const uint4*>(xb + (size_t)pp * 32 + 8 * grp);
       __builtin_memcpy(&b, &v, 16);
     }
-    f32x4 s0 = {0, 0, 0, 0}, s1 = {0, 0, 0, 0};
-    s0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, s0, 0, 0, 0);  // clusters 0..15
-    s1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, s1, 0, 0, 0);  // clusters 16..31
+    f32x4 s[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      s[t] = f32x4{0, 0, 0, 0};
+      s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, s[t], 0, 0, 0);  // clusters 16t..16t+15
+    }
     const float pd = pdot[pp], x32 = resp[(size_t)32 * n + pp];
     const float xn = sqrtf(pd);
-    // this lane's 8 clusters, in increasing order: 4*grp+i, 16+4*grp+i
-    float dt[8];
-    int ci[8];
+    // this lane's 4*NT clusters, in increasing order: 16t + 4*grp + i
+    float dt[4 * NT];
+    int ci[4 * NT];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int c = (i < 4 ? 0 : 16) + 4 * grp + (i & 3);
-      const float sv = (i < 4 ? s0[i & 3] : s1[i & 3]) + x32 * cent[32][c];
-      dt[i] = (pd + cdot[c]) - 2.0f * sv;
+    for (int i = 0; i < 4 * NT; i++) {
+      const int c = 16 * (i >> 2) + 4 * grp + (i & 3);
+      const float sv = s[i >> 2][i & 3] + x32 * sc.cent[32][c];
+      dt[i] = (pd + sc.cdot[c]) - 2.0f * sv;
       ci[i] = c;
     }
     float best = INFINITY;
-    int bc = 64;
+    int bc = 2 * K;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 4 * NT; i++)
       if (dt[i] < best) { best = dt[i]; bc = ci[i]; }
 #pragma unroll
     for (int o = 16; o <= 32; o <<= 1) {  // the pixel's 4 lanes: smaller distance, then lower index
@@ -323,16 +350,16 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
     }
     float lo_other = INFINITY;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 4 * NT; i++)
       if (ci[i] != bc) {
-        const float e = 0.03125f * xn * sqrtf(cdot[ci[i]]) + 9.5367431640625e-7f * (pd + cdot[ci[i]]);
+        const float e = 0.03125f * xn * sqrtf(sc.cdot[ci[i]]) + 9.5367431640625e-7f * (pd + sc.cdot[ci[i]]);
         lo_other = fminf(lo_other, dt[i] - e);
       }
 #pragma unroll
     for (int o = 16; o <= 32; o <<= 1) lo_other = fminf(lo_other, __shfl_xor(lo_other, o, 64));
     bool certain = false;
-    if (bc < 32) {
-      const float e = 0.03125f * xn * sqrtf(cdot[bc]) + 9.5367431640625e-7f * (pd + cdot[bc]);
+    if (bc < K) {
+      const float e = 0.03125f * xn * sqrtf(sc.cdot[bc]) + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
       certain = best + e < lo_other && best + e < 10000000.0f;
     }
     if (grp == 0 && live) {
@@ -341,7 +368,7 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
         float x[kNFilters];
 #pragma unroll
         for (int f = 0; f < kNFilters; f++) x[f] = resp[(size_t)f * n + p];
-        bl = km_exact_label(x, pd, cent, cdot);
+        bl = km_exact_label<K>(x, pd, sc);
         rech++;
       }
       changed += bl != labels[p];
@@ -377,41 +404,49 @@ size_t kmeans_state_bytes() { return sizeof(KmState); }
 size_t kmeans_mixed_work_bytes(int n) { return (size_t)n * 32 * sizeof(uint16_t) + 256; }
 
 namespace {
-int km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels, const float* pdot, KmState* st,
-                  int max_iter, const uint16_t* xb) {
+template <int K>
+void km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels, const float* pdot, KmState* st,
+                   int max_iter, const uint16_t* xb) {
   const int blocks_acc = std::min(div_up(n, 256), 256);
   for (int it = 0; it < max_iter; it++) {
-    hipLaunchKernelGGL(k_km_accum, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
-    hipLaunchKernelGGL(k_km_centroids, dim3(1), dim3(256), 0, s, nf, st, it);
+    hipLaunchKernelGGL(k_km_accum<K>, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
+    hipLaunchKernelGGL(k_km_centroids<K>, dim3(1), dim3(256), 0, s, nf, st, it);
     if (xb)
-      hipLaunchKernelGGL(k_km_assign_mfma, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb, pdot, labels, st,
-                         it);
+      hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb, pdot, labels,
+                         st, it);
     else
-      hipLaunchKernelGGL(k_km_assign, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
+      hipLaunchKernelGGL(k_km_assign<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
   }
+}
+
+template <int K>
+int run_kmeans(hipStream_t s, int w, int h, int nf, const float* resp, int* labels, float* pdot, KmState* st,
+               int max_iter, uint16_t* xb) {
+  const int n = w * h;
+  hipLaunchKernelGGL(k_km_init<K>, dim3(div_up(n, 256)), dim3(256), 0, s, w, h, nf, resp, labels, pdot, st);
+  if (xb) hipLaunchKernelGGL(k_km_bf16, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb);
+  km_iterations<K>(s, n, nf, resp, labels, pdot, st, max_iter, xb);
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
 }
 }  // namespace
 
-int launch_kmeans_mixed(hipStream_t s, int w, int h, const float* resp, int* labels, float* pdot, void* state,
-                        int max_iter, void* work) {
-  const int n = w * h;
+int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, int* labels, float* pdot,
+                        void* state, int max_iter, void* work) {
   KmState* st = (KmState*)state;
   uint16_t* xb = (uint16_t*)work;
   if ((uintptr_t)xb % 16 || (uintptr_t)state % 16) return fail(DMSC_ERR_INTERNAL, "k-means buffers misaligned");
-  hipLaunchKernelGGL(k_km_init, dim3(div_up(n, 256)), dim3(256), 0, s, w, h, kNFilters, resp, labels, pdot, st);
-  hipLaunchKernelGGL(k_km_bf16, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb);
-  return km_iterations(s, n, kNFilters, resp, labels, pdot, st, max_iter, xb);
+  if (k == 64) return run_kmeans<64>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, xb);
+  if (k == 32) return run_kmeans<32>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, xb);
+  return fail(DMSC_ERR_ARG, "k-means supports 32 or 64 clusters");
 }
 
 int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp, int* labels,
                   float* pdot, void* state, int max_iter) {
-  (void)k;
-  const int n = w * h;
   KmState* st = (KmState*)state;
-  hipLaunchKernelGGL(k_km_init, dim3(div_up(n, 256)), dim3(256), 0, s, w, h, nf, resp, labels, pdot, st);
-  return km_iterations(s, n, nf, resp, labels, pdot, st, max_iter, nullptr);
+  if (k == 64) return run_kmeans<64>(s, w, h, nf, resp, labels, pdot, st, max_iter, nullptr);
+  if (k == 32) return run_kmeans<32>(s, w, h, nf, resp, labels, pdot, st, max_iter, nullptr);
+  return fail(DMSC_ERR_ARG, "k-means supports 32 or 64 clusters");
 }
 
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations) {

@@ -101,7 +101,7 @@ int dmsc_quantize_image(dmsc_ctx* ctx, int w, int h, int nbins, const float* d_i
 /* ------------------------------------------------------------------ textons */
 /* texton.cu:39-57: 34 filter responses [34][N] (filter 33 is zero). */
 int dmsc_texton_responses(dmsc_ctx* ctx, int w, int h, const float* d_grey, float* d_resp);
-/* kmeans.h:18 kmeans: k=32 clusters over [nfilters][N] responses; labels
+/* kmeans.h:18 kmeans: k=32 or 64 clusters over [nfilters][N] responses; labels
  * out; *changes = label changes of the last iteration; *iterations = count.
  * precision: DMSC_PREC_FP32 (VALU distances, reference order) or
  * DMSC_PREC_MIXED (BF16 MFMA distances + FP32 re-check of near ties; same
@@ -109,7 +109,7 @@ int dmsc_texton_responses(dmsc_ctx* ctx, int w, int h, const float* d_grey, floa
 int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* d_resp,
                 int* d_labels, int max_iter, int conv_thresh, int precision, int* changes,
                 int* iterations);
-/* texton.h:16 findTextons (texton_choice 1 = 32 textons). */
+/* texton.h:16 findTextons (texton_choice 1 = 32 textons, 2 = 64). */
 int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* d_grey, int* d_textons,
                       int texton_choice, int precision, int* iterations);
 
@@ -122,7 +122,7 @@ int dmsc_local_cues(dmsc_ctx* ctx, int w, int h, const float* d_L, const float* 
  * parabola smoothing (gradient.cu:240-241): d_raw [4 cues][3 scales][8][N]. */
 int dmsc_local_cues_raw(dmsc_ctx* ctx, int w, int h, const float* d_L, const float* d_a,
                         const float* d_b, const int* d_textons, float* d_bg, float* d_cga,
-                        float* d_cgb, float* d_tg, int cue_pitch, float* d_raw);
+                        float* d_cgb, float* d_tg, int cue_pitch, float* d_raw, int texton_choice);
 
 /* ------------------------------------------------------------------ combine / nonmax / gPb */
 /* combine.h:15 combine -> mPb [8][pitch], combinedG [8][pitch]. */
@@ -198,7 +198,7 @@ int dmsc_tune_get(const char* key, int* value);
 typedef struct {
   int n_eig;          /* number of eigenvectors (2..25), default 9 */
   float tol;          /* Lanczos residual tolerance, default 1e-3 */
-  int texton_choice;  /* DMSC_TEXTON32 */
+  int texton_choice;  /* DMSC_TEXTON32 (default) or DMSC_TEXTON64 */
   int precision;      /* DMSC_PREC_FP32 or DMSC_PREC_MIXED */
 } dmsc_gpb_params;
 

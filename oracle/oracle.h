@@ -60,7 +60,7 @@ void orc_quantize(int w, int h, int nbins, const float* in, int* out);
 void orc_texton_convolve(int w, int h, const float* grey, float* responses /*[34][N]*/);
 int  orc_kmeans(int w, int h, int k, int nfilters, const float* responses, int* labels,
                 int max_iter, int* iterations);
-int  orc_find_textons(int w, int h, const float* grey, int* labels, int* iterations);
+int  orc_find_textons(int w, int h, const float* grey, int ntex /*32|64*/, int* labels, int* iterations);
 
 void orc_integral_images(int W, int H, int nbins, const int* labels, int binPitch, int32_t* integrals);
 void orc_gradient_pair(int w, int h, int border, int nbins, int blur, const float* blurKernel,
@@ -73,9 +73,9 @@ void orc_channel_gradients(int w, int h, int nbins, int blur, float sigma, const
 void orc_parabola(int w, int h, int border, int radius, const float* grads8 /*[8][N]*/,
                   const float* filters /*8*(2r+1)^2*/, float* out8 /*[8][N]*/);
 void orc_local_cues(int w, int h, const float* L, const float* a, const float* b,
-                    const int* textons, float* bg, float* cga, float* cgb, float* tg);
+                    const int* textons, int ntex, float* bg, float* cga, float* cgb, float* tg);
 void orc_combine(int w, int h, const float* bg, const float* cga, const float* cgb,
-                 const float* tg, float* mpb /*[8][N]*/, float* combinedG /*[8][N]*/);
+                 const float* tg, int ntex, float* mpb /*[8][N]*/, float* combinedG /*[8][N]*/);
 void orc_nonmax(int w, int h, const float* pb8, float* nmax);
 
 void orc_intervene(int w, int h, const float* mpb, float* matrix /*[81][N]*/, float sigma);
@@ -116,7 +116,8 @@ void orc_normalize_gpb_all(int npix, int norient, int pitch, float* all);
 /* whole pipeline (plain FP32 reference order); returns 0 on success */
 int  orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol,
                       int* textons, float* mpb8, float* cg8, float* nmax, float* eigvals,
-                      float* eigvecs, float* spb8, float* gpbAll, float* gpb, double* stage_ms);
+                      float* eigvecs, float* spb8, float* gpbAll, float* gpb, double* stage_ms,
+                      int ntex);
 
 #ifdef __cplusplus
 }

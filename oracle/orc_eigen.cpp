@@ -376,7 +376,7 @@ int orc_generalized_eigensolve(int w, int h, const float* W, int nvec, float tol
 // damascene.cu:269-459 stage order for the plain FP32 path.
 int orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol, int* textons,
                      float* mpb8, float* cg8, float* nmax, float* eigvals, float* eigvecs,
-                     float* spb8, float* gpbAll, float* gpb, double* ms) {
+                     float* spb8, float* gpbAll, float* gpb, double* ms, int ntex) {
   using clk = std::chrono::steady_clock;
   const int n = w * h;
   auto t0 = clk::now();
@@ -388,14 +388,14 @@ int orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol, int
   std::vector<float> grey(n), L(n), A(n), B(n);
   orc_rgb_to_grey(w, h, rgb, grey.data());
   int it = 0;
-  orc_find_textons(w, h, grey.data(), textons, &it);
+  orc_find_textons(w, h, grey.data(), ntex, textons, &it);
   lap(0);
   orc_rgb_to_lab(w, h, 2.5f, rgb, L.data(), A.data(), B.data());
   orc_normalize_lab(w, h, L.data(), A.data(), B.data());
   std::vector<float> bg((size_t)24 * n), ca((size_t)24 * n), cb((size_t)24 * n), tg((size_t)24 * n);
-  orc_local_cues(w, h, L.data(), A.data(), B.data(), textons, bg.data(), ca.data(), cb.data(), tg.data());
+  orc_local_cues(w, h, L.data(), A.data(), B.data(), textons, ntex, bg.data(), ca.data(), cb.data(), tg.data());
   lap(1);
-  orc_combine(w, h, bg.data(), ca.data(), cb.data(), tg.data(), mpb8, cg8);
+  orc_combine(w, h, bg.data(), ca.data(), cb.data(), tg.data(), ntex, mpb8, cg8);
   orc_nonmax(w, h, mpb8, nmax);
   lap(2);
   std::vector<float> Wm((size_t)81 * n);

@@ -133,10 +133,13 @@ int orc_kmeans(int w, int h, int k, int nf, const float* resp, int* labels, int 
   const int n = w * h;
   std::vector<int> iresp((size_t)nf * n);
   std::vector<float> pdot(n);
+  // kmeans.cu:14-35 assignInitialClusters (6x6 grid, capped at 31) for
+  // 32 clusters; kmeans.cu:37-55 assignInitialClusters_64 (8x8 grid) for 64
+  const int g = (k == 64) ? 8 : 6;
   for (int y = 0; y < h; y++)
     for (int x = 0; x < w; x++) {
-      int c = (y / ((h - 1) / 6 + 1)) * 6 + x / ((w - 1) / 6 + 1);
-      if (c >= 32) c = 31;
+      int c = (y / ((h - 1) / g + 1)) * g + x / ((w - 1) / g + 1);
+      if (k != 64 && c >= 32) c = 31;
       labels[y * w + x] = c;
     }
   for (size_t i = 0; i < (size_t)nf * n; i++) iresp[i] = (int)(100000.0f * resp[i]);
@@ -187,12 +190,12 @@ int orc_kmeans(int w, int h, int k, int nf, const float* resp, int* labels, int 
   return changes;
 }
 
-// texton.cu:62-131 findTextons (32 textons).
-int orc_find_textons(int w, int h, const float* grey, int* labels, int* iterations) {
+// texton.cu:62-131 findTextons (ntex = 32 or 64 textons, texton.cu:65-67).
+int orc_find_textons(int w, int h, const float* grey, int ntex, int* labels, int* iterations) {
   const int n = w * h;
   std::vector<float> resp((size_t)34 * n);
   orc_texton_convolve(w, h, grey, resp.data());
-  return orc_kmeans(w, h, 32, 34, resp.data(), labels, 15, iterations);
+  return orc_kmeans(w, h, ntex, 34, resp.data(), labels, 15, iterations);
 }
 
 // gradient.cu:66-118 formIntegralImages (inclusive 2-D counts per bin);
@@ -219,6 +222,14 @@ static inline float tree32(float* t) {
   return t[0];
 }
 
+// Bin sum of a histogram padded to 32 or 64 slots: one 32-slot tree per
+// 32-thread warp, then slot 0 + slot 32 when there are more than 32 bins
+// (rotate.cu:384-389).
+static inline float tree_bins(float* t, int nbins) {
+  const float lo = tree32(t);
+  return nbins > 32 ? lo + tree32(t + 32) : lo;
+}
+
 // rotate.cu:263-475 computeGradient for both senses at every pixel.
 void orc_gradient_pair(int w, int h, int border, int nbins, int blur, const float* kern,
                        int kr, int rw, const int* bX, const int* bY, const int* gX,
@@ -232,8 +243,8 @@ void orc_gradient_pair(int w, int h, int border, int nbins, int blur, const floa
       int rx = gX[iy] + bX[ix], ry = gY[iy] + bY[ix];
       long pix = (long)ry * rw + rx;
       for (int sense = 0; sense < 2; sense++) {
-        float a[32], b[32];
-        for (int i = 0; i < 32; i++) { a[i] = 0.f; b[i] = 0.f; }
+        float a[64], b[64];
+        for (int i = 0; i < 64; i++) { a[i] = 0.f; b[i] = 0.f; }
         for (int bin = 0; bin < nbins; bin++) {
           int32_t c[8];
           for (int kk = 0; kk < 8; kk++) c[kk] = I[(size_t)(rect[kk] + pix) * bp + bin];
@@ -249,7 +260,7 @@ void orc_gradient_pair(int w, int h, int border, int nbins, int blur, const floa
           b[bin] = bv;
         }
         if (blur) {  // rotate.cu:352-370: zero-padded correlation over bins
-          float ca[32], cb[32];
+          float ca[64], cb[64];
           for (int bin = 0; bin < nbins; bin++) {
             float sa = 0.f, sb = 0.f;
             for (int nk = 0; nk < kl; nk++) {
@@ -264,20 +275,21 @@ void orc_gradient_pair(int w, int h, int border, int nbins, int blur, const floa
           }
           for (int bin = 0; bin < nbins; bin++) { a[bin] = ca[bin]; b[bin] = cb[bin]; }
         }
-        float t[32];
+        const int slots = nbins > 32 ? 64 : 32;
+        float t[64];
         memcpy(t, a, sizeof(t));
-        float sa = tree32(t);
+        float sa = tree_bins(t, nbins);
         float na = (sa != 0.f) ? 8.0f / sa : 0.f;
         memcpy(t, b, sizeof(t));
-        float sb = tree32(t);
+        float sb = tree_bins(t, nbins);
         float nb = (sb != 0.f) ? 8.0f / sb : 0.f;
-        for (int i = 0; i < 32; i++) {
+        for (int i = 0; i < slots; i++) {
           float ai = a[i] * na, bi = b[i] * nb;
           float s = ai + bi, chi = 0.f;
           if (s != 0.f) { float d = ai - bi; chi = d * d / s; }
           t[i] = chi;
         }
-        float out = 0.5f * tree32(t);
+        float out = 0.5f * tree_bins(t, nbins);
         (sense == 0 ? gA : gB)[y * w + x] = out;
       }
     }
@@ -357,10 +369,10 @@ void orc_parabola(int w, int h, int border, int r, const float* g8, const float*
 }
 
 // localcues.cu:173-300 localCues: Bg (L, 25 bins, blur sigma 0.1, radii
-// 3,5,10), Cga/Cgb (25 bins, blur 0.05, radii 5,10,20), Tg (32 textons, no
+// 3,5,10), Cga/Cgb (25 bins, blur 0.05, radii 5,10,20), Tg (ntex textons, no
 // blur, radii 5,10,20); parabola smoothing per scale.  Outputs [24][N].
 void orc_local_cues(int w, int h, const float* L, const float* A, const float* B,
-                    const int* textons, float* bg, float* cga, float* cgb, float* tg) {
+                    const int* textons, int ntex, float* bg, float* cga, float* cgb, float* tg) {
   const int n = w * h;
   static const int radii[4] = {3, 5, 10, 20};
   std::vector<float> grads((size_t)24 * n);
@@ -376,7 +388,7 @@ void orc_local_cues(int w, int h, const float* L, const float* A, const float* B
       orc_channel_gradients(w, h, nbins, 1, (float)(int)(sigma * (float)nbins), rad, q.data(),
                             grads.data());
     } else {
-      orc_channel_gradients(w, h, 32, 0, 0.0f, rad, textons, grads.data());
+      orc_channel_gradients(w, h, ntex, 0, 0.0f, rad, textons, grads.data());
     }
     for (int s = 0; s < 3; s++) {
       int r = rad[s], len = 2 * r + 1;
@@ -388,13 +400,21 @@ void orc_local_cues(int w, int h, const float* L, const float* A, const float* B
   }
 }
 
-// combine.cu:11-93 (32-texton tables), fused multiply-add in cue order.
+// combine.cu:11-93, fused multiply-add in cue order; the 32-texton tables
+// (combine.cu:17, 21) or the 64-texton ones (combine.cu:16, 20), which
+// differ in the three Tg entries.
 void orc_combine(int w, int h, const float* bg, const float* cga, const float* cgb,
-                 const float* tg, float* mpb, float* cgout) {
-  static const float coef[12] = {0.0123f, 0.0110f, 0.0117f, 0.0169f, 0.0176f, 0.0198f,
-                                 0.0138f, 0.0138f, 0.0145f, 0.0144f, 0.0149f, 0.0158f};
-  static const float wts[12] = {0.f,     0.f,     0.0028f, 0.0041f, 0.0042f, 0.0047f,
-                                0.0033f, 0.0033f, 0.0035f, 0.0035f, 0.0035f, 0.0188f};
+                 const float* tg, int ntex, float* mpb, float* cgout) {
+  static const float coef32[12] = {0.0123f, 0.0110f, 0.0117f, 0.0169f, 0.0176f, 0.0198f,
+                                   0.0138f, 0.0138f, 0.0145f, 0.0144f, 0.0149f, 0.0158f};
+  static const float wts32[12] = {0.f,     0.f,     0.0028f, 0.0041f, 0.0042f, 0.0047f,
+                                  0.0033f, 0.0033f, 0.0035f, 0.0035f, 0.0035f, 0.0188f};
+  static const float coef64[12] = {0.0123f, 0.0110f, 0.0117f, 0.0169f, 0.0176f, 0.0198f,
+                                   0.0138f, 0.0138f, 0.0145f, 0.0104f, 0.0105f, 0.0115f};
+  static const float wts64[12] = {0.f,     0.f,     0.0028f, 0.0041f, 0.0042f, 0.0047f,
+                                  0.0033f, 0.0033f, 0.0035f, 0.0025f, 0.0025f, 0.0137f};
+  const float* coef = ntex == 64 ? coef64 : coef32;
+  const float* wts = ntex == 64 ? wts64 : wts32;
   const int n = w * h;
   const float* cues[4] = {bg, cga, cgb, tg};
   for (int o = 0; o < 8; o++)

@@ -125,19 +125,19 @@ def texton_responses(grey):
     return out
 
 
-def kmeans(resp, max_iter=15):
+def kmeans(resp, max_iter=15, k=32):
     _, h, w = resp.shape
     labels = np.zeros((h, w), np.int32)
     it = ctypes.c_int()
-    lib().orc_kmeans(w, h, 32, 34, P(np.ascontiguousarray(resp, np.float32)), P(labels), max_iter, ctypes.byref(it))
+    lib().orc_kmeans(w, h, k, 34, P(np.ascontiguousarray(resp, np.float32)), P(labels), max_iter, ctypes.byref(it))
     return labels, it.value
 
 
-def find_textons(grey):
+def find_textons(grey, ntex=32):
     h, w = grey.shape
     labels = np.zeros((h, w), np.int32)
     it = ctypes.c_int()
-    lib().orc_find_textons(w, h, P(np.ascontiguousarray(grey, np.float32)), P(labels), ctypes.byref(it))
+    lib().orc_find_textons(w, h, P(np.ascontiguousarray(grey, np.float32)), ntex, P(labels), ctypes.byref(it))
     return labels, it.value
 
 
@@ -157,17 +157,18 @@ def channel_gradients(labels, nbins, blur, sigma, radii3):
     return out
 
 
-def local_cues(L, a, b, textons):
+def local_cues(L, a, b, textons, ntex=32):
     h, w = L.shape
     outs = [f32(24, h, w) for _ in range(4)]
-    lib().orc_local_cues(w, h, P(L), P(a), P(b), P(np.ascontiguousarray(textons, np.int32)), *[P(o) for o in outs])
+    lib().orc_local_cues(w, h, P(L), P(a), P(b), P(np.ascontiguousarray(textons, np.int32)), ntex,
+                         *[P(o) for o in outs])
     return outs
 
 
-def combine(bg, cga, cgb, tg):
+def combine(bg, cga, cgb, tg, ntex=32):
     _, h, w = bg.shape
     mpb, cg = f32(8, h, w), f32(8, h, w)
-    lib().orc_combine(w, h, P(bg), P(cga), P(cgb), P(tg), P(mpb), P(cg))
+    lib().orc_combine(w, h, P(bg), P(cga), P(cgb), P(tg), ntex, P(mpb), P(cg))
     return mpb, cg
 
 
@@ -263,7 +264,7 @@ def gpb(cg, spb, nmax):
     return all_, g
 
 
-def pipeline(rgb, nvec=9, tol=1e-3):
+def pipeline(rgb, nvec=9, tol=1e-3, ntex=32):
     """Whole plain-FP32 pipeline; returns a dict of host arrays and stage ms."""
     h, w = rgb.shape
     r = dict(textons=np.zeros((h, w), np.int32), mpb=f32(8, h, w), combined_g=f32(8, h, w), nmax=f32(h, w),
@@ -271,7 +272,7 @@ def pipeline(rgb, nvec=9, tol=1e-3):
     ms = np.zeros(6)
     rc = lib().orc_gpb_pipeline(w, h, P(np.ascontiguousarray(rgb, np.uint32)), nvec, ctypes.c_float(tol),
                                 P(r["textons"]), P(r["mpb"]), P(r["combined_g"]), P(r["nmax"]), P(r["eigvals"]),
-                                P(r["eigvecs"]), P(r["spb"]), P(r["gpb_all"]), P(r["gpb"]), P(ms))
+                                P(r["eigvecs"]), P(r["spb"]), P(r["gpb_all"]), P(r["gpb"]), P(ms), ntex)
     r["rc"] = rc
     r["stage_ms"] = ms
     return r

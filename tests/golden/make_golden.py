@@ -16,6 +16,10 @@ Fixtures (numpy .npz, no pickles):
                      sPb/spectralPb.cu:708-711 builds them (gaussian_2D,
                      filters.cpp:261-336), and Stencil offsets/diagonal maps
                      (stencilMatrixMultiply/Stencil.cpp:6-67).
+  combine_tables.npz -- the cue weights of combine/combine.cu:16-21 for 32
+                     and 64 textons (mPb coefficients, combinedG weights;
+                     the weights arrays carry a 13th entry, the sPb weight,
+                     kept as read).
 """
 import argparse
 import ctypes
@@ -42,10 +46,18 @@ def parse_c_array(text, name, dtype):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", choices=["combine"], help="write one fixture only")
     a = ap.parse_args()
     ref = a.reference
     if not os.path.isdir(ref):
         raise SystemExit(f"reference tree {ref} not present; fixtures are already committed")
+
+    text = open(os.path.join(ref, "combine", "combine.cu")).read()
+    tables = {k: parse_c_array(text, k, np.float32)
+              for k in ("hostCoefficients", "hostCoefficients_64", "hostweights", "hostweights_64")}
+    np.savez_compressed(os.path.join(HERE, "combine_tables.npz"), **tables)
+    if a.only == "combine":
+        return 0
 
     text = open(os.path.join(ref, "include", "filters.h")).read()
     coeffs = parse_c_array(text, "hCoefficients", np.float32)

@@ -115,3 +115,53 @@ def test_eigensolve_subspace(ctx, case, precision):
     else:
         assert rel.max() < 1e-3
         assert ang < 0.2
+
+
+def test_pipeline_texton64(ctx):
+    """texton_choice 2 end to end (plain path) against the oracle's 64-texton
+    pipeline: textons, mPb and nonmax exact; eigenvalues and gPb as for 32."""
+    W, H, CELLS, SEED = 128, 96, 10, 20261203
+    print(f"case: texton64 {W}x{H} cells={CELLS} seed={SEED}", flush=True)
+    rgb = dm.generate_image(W, H, CELLS, SEED)
+    ref = O.pipeline(rgb, nvec=NVEC, tol=TOL, ntex=64)
+    assert ref["rc"] == 0
+    out = ctx.gpb(ctx.upload(rgb), precision=dm.PREC_FP32, texton_choice=dm.TEXTON64)
+    check_exact_stages(out, ref, W, H)
+    assert host(out["textons"]).max() >= 32
+    rel = np.abs(out["eigvals"][1:] - ref["eigvals"][1:]) / np.abs(ref["eigvals"][1:])
+    gerr = np.abs(host(out["gpb"]) - ref["gpb"]).max()
+    print(f"texton64: eigval max rel {rel.max():.3e}, gPb max abs vs oracle {gerr:.3e}", flush=True)
+    assert rel.max() < 1e-4
+    assert gerr < 2e-3
+
+
+@pytest.mark.parametrize("precision", [dm.PREC_MIXED])
+def test_eigensolve_481x321_vs_truth(ctx, precision):
+    """The full-size workload (481x321, 604 blocks per Lanczos step) on the
+    oracle's own L_sym against the committed FP64 truth (shift-invert eigsh,
+    tests/golden/truth_481x321.npz)."""
+    import os
+    W, H, CELLS, SEED = 481, 321, 40, 20261202
+    print(f"case: eigensolve {W}x{H} precision {precision}", flush=True)
+    truth = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "truth_481x321.npz"))
+    rgb = dm.generate_image(W, H, CELLS, SEED)
+    tex, _ = O.find_textons(O.rgb_to_grey(rgb))
+    L, a, b = O.rgb_to_lab(rgb)
+    mpb, _ = O.combine(*O.local_cues(L, a, b, tex))
+    Wm = O.intervene(O.nonmax(mpb))
+    n = W * H
+    m = torch.from_numpy(np.pad(Wm, ((0, 0), (0, dm.pitch_in_floats(n) - n)))).cuda()
+    vals, vecs, it = ctx.generalized_eigensolve(W, H, m, nvec=NVEC, tol=TOL, precision=precision)
+    tv = truth["eigvals"][:NVEC]
+    rel = np.abs(vals[1:] - tv[1:]) / tv[1:]
+    sq = truth["sqrt_d"].astype(np.float64)
+    u0 = sq / np.linalg.norm(sq)
+    G = host(vecs).reshape(NVEC, n)[1:].astype(np.float64) * sq[None, :]
+    G -= np.outer(G @ u0, u0)
+    R = truth["u"].astype(np.float64)
+    R -= np.outer(R @ u0, u0)
+    ang = subspace_angle(G.T, R.T)
+    print(f"481x321 precision {precision}: iterations {it}, eigval max rel {rel.max():.3e}, "
+          f"subspace angle {ang:.3e}", flush=True)
+    assert rel.max() < 1e-4
+    assert ang < 1e-4

@@ -194,3 +194,56 @@ def test_postprocess_and_normalize(ctx, combined):
     got = ctx.normalize_gpb_all(all_, n)
     ref = O.normalize_gpb_all(host(all_)[:, :n].copy())
     np.testing.assert_allclose(host(got)[:, :n], ref, atol=1e-6, rtol=0)
+
+
+# ---------------------------------------------------------------- 64 textons
+# texton_choice 2 (texton.cu:65-67, kmeans.cu:37-55/106-153/278-286,
+# rotate.cu:780-782, combine.cu:16,20): same bar as the 32-texton stages.
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_kmeans64_labels(ctx, responses, precision):
+    prec = dm.PREC_FP32 if precision == "fp32" else dm.PREC_MIXED
+    print(f"case: kmeans k=64 {precision}", flush=True)
+    labels, it = ctx.kmeans(responses, precision=prec, k=64)
+    ref, rit = O.kmeans(host(responses), k=64)
+    assert it == rit, f"iterations gpu {it} oracle {rit}"
+    np.testing.assert_array_equal(host(labels), ref)
+    assert host(labels).max() >= 32  # the upper half of the clusters is in use
+
+
+@pytest.fixture(scope="module")
+def textons64(ctx, colour):
+    lab, _ = ctx.find_textons(colour[1], texton_choice=dm.TEXTON64)
+    return lab
+
+
+def test_find_textons64(textons64, colour):
+    ref, _ = O.find_textons(host(colour[1]), ntex=64)
+    np.testing.assert_array_equal(host(textons64), ref)
+
+
+@pytest.fixture(scope="module")
+def cues64(ctx, colour, textons64):
+    L, a, b = colour[2]
+    return ctx.local_cues(L, a, b, textons64, raw=True, texton_choice=dm.TEXTON64)
+
+
+def test_texture_gradients64(cues64, textons64):
+    (_, raw) = cues64
+    raw = host(raw).reshape(4, 3, 8, H, W)
+    ref = O.channel_gradients(host(textons64), 64, 0, 0.0, (5, 10, 20))
+    assert_bitexact(raw[3], ref, "raw chi-squared planes, 64 textons")
+
+
+def test_local_cues_and_combine64(ctx, cues64, colour, textons64):
+    (got, _) = cues64
+    L, a, b = [host(t) for t in colour[2]]
+    ref = O.local_cues(L, a, b, host(textons64), ntex=64)
+    for name, g, r in zip(("bg", "cga", "cgb", "tg"), got, ref):
+        assert_bitexact(host(g).reshape(24, H, W), r, name + " (64 textons)")
+    full = [torch.nn.functional.pad(c, (0, dm.pitch_in_floats(W * H) - W * H)).contiguous() for c in got]
+    mpb, cg = ctx.combine(W, H, *full, texton_choice=dm.TEXTON64)
+    n = W * H
+    ref_m, ref_c = O.combine(*ref, ntex=64)
+    assert_bitexact(host(mpb)[:, :n].reshape(8, H, W), ref_m, "mPb (64 textons)")
+    assert_bitexact(host(cg)[:, :n].reshape(8, H, W), ref_c, "combinedG (64 textons)")

@@ -184,18 +184,20 @@ def test_mirror_and_quantize():
 
 
 # --------------------------------------------------------------- textons
-def test_kmeans_fixed_point_invariants():
+@pytest.mark.parametrize("k", [32, 64])
+def test_kmeans_fixed_point_invariants(k):
     """Labels are the strict argmin (ties to the lowest index) of the
     distance to the truncated-integer-mean centroids of those labels."""
     w, h = 40, 30
     grey = RNG.random((h, w)).astype(np.float32)
     resp = O.texton_responses(grey)
-    labels, it = O.kmeans(resp, max_iter=100)
+    labels, it = O.kmeans(resp, max_iter=100, k=k)
     assert it < 100, "no fixed point reached"
     X = resp.reshape(34, -1).T.astype(np.float32)
     Xi = np.trunc(X * np.float32(100000)).astype(np.int64)
     lab = labels.ravel()
-    C = np.full((32, 34), np.nan, np.float32)
+    assert lab.min() >= 0 and lab.max() < k
+    C = np.full((k, 34), np.nan, np.float32)
     for c in np.unique(lab):
         s = Xi[lab == c].sum(0)
         s = ((s + 2**31) % 2**32 - 2**31).astype(np.float32)  # int32 wrap-around
@@ -208,6 +210,26 @@ def test_kmeans_fixed_point_invariants():
     assert (gap <= 1e-4 * np.maximum(1, best)).all()
     # filter 33 is never bound, so its response is identically zero
     assert not resp[33].any()
+
+
+def test_kmeans_initial_partition():
+    """kmeans.cu:14-55: the initial labels tile the image in blocks of
+    ceil(w/g) x ceil(h/g) pixels, g = 6 for 32 clusters (labels past 31
+    merged into 31) and g = 8 for 64 clusters (all 64 labels used)."""
+    resp = np.zeros((34, 48, 64), np.float32)
+    lab, it = O.kmeans(resp, max_iter=0, k=64)
+    assert it == 0
+    counts = np.bincount(lab.ravel(), minlength=64)
+    np.testing.assert_array_equal(counts, np.full(64, 8 * 6))
+    for c in range(64):  # each label is one 8 x 6 rectangle, row-major block order
+        ys, xs = np.nonzero(lab == c)
+        assert (ys.min(), xs.min()) == (6 * (c // 8), 8 * (c % 8))
+        assert ys.max() - ys.min() == 5 and xs.max() - xs.min() == 7
+    resp = np.zeros((34, 36, 60), np.float32)
+    lab, _ = O.kmeans(resp, max_iter=0, k=32)
+    counts = np.bincount(lab.ravel(), minlength=32)
+    # 6 x 6 blocks of 10 x 6 pixels; blocks 31..35 share label 31
+    np.testing.assert_array_equal(counts, [60] * 31 + [300])
 
 
 # --------------------------------------------------------------- histograms
@@ -236,6 +258,110 @@ def test_gradients_constant_image_and_chi2_bounds():
     assert g[0, :, h // 2, 2].max() < 1e-6  # far from the edge at radius 3
 
 
+def tree32_np(t):
+    """The reference's 32-slot warp tree (rotate.cu:378-383) over axis 0."""
+    t = t.copy()
+    s = 16
+    while s:
+        t[:s] = t[:s] + t[s:2 * s]
+        s //= 2
+    return t[0]
+
+
+def test_gradients64_halves():
+    """64 bins (rotate.cu:780-782): labels confined to either half of the
+    bins give the 32-bin result bit for bit, and two regions whose labels lie
+    in different halves give chi^2 = 8 at their edge."""
+    w, h = 40, 40
+    radii = (3, 5, 10)
+    lab = RNG.integers(0, 32, size=(h, w)).astype(np.int32)
+    g32 = O.channel_gradients(lab, 32, 0, 0.0, radii)
+    np.testing.assert_array_equal(O.channel_gradients(lab, 64, 0, 0.0, radii).view(np.uint32), g32.view(np.uint32))
+    np.testing.assert_array_equal(O.channel_gradients(lab + 32, 64, 0, 0.0, radii).view(np.uint32),
+                                  g32.view(np.uint32))
+    two = np.full((h, w), 3, np.int32)
+    two[:, w // 2:] = 40
+    g = O.channel_gradients(two, 64, 0, 0.0, radii).reshape(3, 8, h, w)
+    assert np.isclose(g[:, :, h // 2, w // 2 - 1:w // 2 + 1].max(), 8.0, atol=1e-5)
+
+
+def chi2_bruteforce(a, b, nbins, kern=None):
+    """rotate.cu:352-472 on two raw histograms: optional zero-padded bin blur,
+    normalisation to sum 8, 0.5 * sum (a-b)^2/(a+b); every sum a 32-slot
+    tree, plus a second over bins 32..63 when there are 64 (rotate.cu:384-389)."""
+    slots = 64 if nbins > 32 else 32
+    a = np.pad(a.astype(np.float32), (0, slots - a.size))
+    b = np.pad(b.astype(np.float32), (0, slots - b.size))
+    if kern is not None:  # float64 accumulation, compared with a tolerance
+        kr = kern.size // 2
+        pa, pb = np.pad(a[:nbins], kr), np.pad(b[:nbins], kr)
+        a[:nbins] = np.array([pa[i:i + kern.size] @ kern for i in range(nbins)], np.float32)
+        b[:nbins] = np.array([pb[i:i + kern.size] @ kern for i in range(nbins)], np.float32)
+
+    def total(t):
+        return tree32_np(t[:32]) + tree32_np(t[32:]) if slots == 64 else tree32_np(t)
+
+    sa, sb = total(a), total(b)
+    a = a * (np.float32(8) / sa if sa != 0 else np.float32(0))
+    b = b * (np.float32(8) / sb if sb != 0 else np.float32(0))
+    ssum = a + b
+    with np.errstate(invalid="ignore", divide="ignore"):
+        chi = np.where(ssum != 0, (a - b) * (a - b) / ssum, np.float32(0)).astype(np.float32)
+    return np.float32(0.5) * total(chi), chi
+
+
+@pytest.mark.parametrize("nbins,blur", [(25, 2.0), (32, 0.0), (64, 0.0)])
+def test_gradients_bruteforce_orientation0(nbins, blur):
+    """Orientation 0 (no rotation, axis-aligned half-discs): both splits at
+    every interior pixel against histograms counted directly in the label
+    image and reduced in the reference's order -- bit for bit without blur,
+    within 2e-6 with it.  For 64 bins a single 64-slot tree would give a
+    different chi^2 somewhere, so the order is pinned too."""
+    import ctypes
+    w, h = 40, 40
+    radii = (3, 5, 10)
+    lab = RNG.integers(0, nbins, size=(h, w)).astype(np.int32)
+    g = O.channel_gradients(lab, nbins, int(blur > 0), blur, radii).reshape(3, 8, h, w)
+    kern = None
+    if blur > 0:
+        k, kr = O.blur_kernel(blur)
+        kern = np.asarray(k[:2 * kr + 1], np.float64)
+    nw, nh, bx, by, gx, gy = O.rotation_tables(w + 60, h + 60, 0.0)
+    checked = differs = 0
+    for s, r in enumerate(radii):
+        rect = np.zeros(8, np.int32)
+        O.lib().orc_halfdisc_offsets(w, h, ctypes.c_float(0.0), r, O.P(bx), O.P(by), O.P(gx), O.P(gy), nw,
+                                     O.P(rect))
+        dy = np.round(rect / nw).astype(int)  # corner (dx, dy) from the offsets dy * nw + dx
+        dx = rect - dy * nw
+        x0, xc, x1, y0, ym, y1 = dx[0], dx[1], dx[2], dy[0], dy[3], dy[7]
+        for y in range(r + 2, h - r - 2):
+            for x in range(r + 2, w - r - 2):
+                win = lab[y + y0 + 1:y + y1 + 1, x + x0 + 1:x + x1 + 1]
+                cnt = lambda v: np.bincount(v.ravel(), minlength=nbins)
+                # inclusive-prefix corners: top rows (y0, ym], bottom (ym, y1];
+                # left columns (x0, xc], right (xc, x1]
+                top, bot = win[:ym - y0], win[ym - y0:]
+                left, right = win[:, :xc - x0], win[:, xc - x0:]
+                for slot, (p, q) in ((4, (top, bot)), (0, (left, right))):
+                    want, chi = chi2_bruteforce(cnt(p), cnt(q), nbins, kern)
+                    got = g[s, slot, y, x]
+                    if kern is None:
+                        assert got.view(np.uint32) == want.view(np.uint32), (s, slot, y, x, got, want)
+                    else:
+                        assert abs(got - want) <= 2e-6, (s, slot, y, x, got, want)
+                    if nbins == 64:
+                        c64 = chi.copy()
+                        k = 32
+                        while k:
+                            c64[:k] = c64[:k] + c64[k:2 * k]
+                            k //= 2
+                        differs += np.float32(0.5) * c64[0] != want
+                    checked += 1
+    assert checked > 3000
+    assert nbins != 64 or differs > 0
+
+
 def test_combine_known_answer():
     """combine/combinemain.cu:24-59: 2x2 image, cues filled with 2, 3, 4, 5."""
     n = 4
@@ -243,6 +369,21 @@ def test_combine_known_answer():
     mpb, cg = O.combine(*[c.reshape(24, 2, 2) for c in cues])
     np.testing.assert_allclose(mpb, 0.6268, atol=1e-6)
     np.testing.assert_allclose(cg, 0.2140, atol=1e-6)
+
+
+@pytest.mark.parametrize("ntex", [32, 64])
+def test_combine_tables_match_reference(ntex):
+    """Each cue plane alone picks out its entry of the reference's weight
+    tables (combine/combine.cu:16-21, read into tests/golden/combine_tables.npz)."""
+    t = gold("combine_tables.npz")
+    sfx = "_64" if ntex == 64 else ""
+    coef, wts = t["hostCoefficients" + sfx], t["hostweights" + sfx][:12]
+    for k in range(12):
+        cues = [np.zeros((24, 2, 3), np.float32) for _ in range(4)]
+        cues[k // 3][(k % 3) * 8:(k % 3) * 8 + 8] = 1.0
+        mpb, cg = O.combine(*cues, ntex=ntex)
+        np.testing.assert_array_equal(mpb, np.full((8, 2, 3), coef[k], np.float32))
+        np.testing.assert_array_equal(cg, np.full((8, 2, 3), wts[k], np.float32))
 
 
 # --------------------------------------------------------------- nonmax / gPb

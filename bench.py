@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 SPMV_BYTES_PER_ROW = 81 * 4 + 4 + 4  # 81 matrix entries + x + y (algorithmic)
 SPMV_SHAPES = {0: "k_spmv_full", 1: "k_spmv<1>", 2: "k_spmv<2>", 4: "k_spmv<4>"}
 TRUTH = os.path.join(ROOT, "tests", "golden", "truth_481x321.npz")
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_spmv_full_481x321_pmc.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "stencil_spmv_481x321_pmc.txt")
 
 
 def parse():
@@ -63,15 +63,22 @@ def pmc_summarize(fetch_dir, write_dir, shape):
     import glob
 
     def per_kernel(d, counter):
-        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
-        if not files:
-            raise SystemExit(f"no counter_collection.csv under {d}")
+        """{(kernel, dispatch): value} from rocprofv3's CSV or SQLite output."""
         vals = {}
-        for f in files:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             for r in csv.DictReader(open(f)):
                 if r["Counter_Name"] == counter:
                     key = (r["Kernel_Name"], r.get("Dispatch_Id", ""))
                     vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+        for f in glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True):
+            import sqlite3
+
+            con = sqlite3.connect(f)
+            q = "select kernel_name, dispatch_id, value from counters_collection where counter_name = ?"
+            for k, disp, v in con.execute(q, (counter,)):
+                vals[(k, disp)] = vals.get((k, disp), 0.0) + float(v)
+        if not vals:
+            raise SystemExit(f"no {counter} values under {d}")
         return vals
 
     kname = SPMV_SHAPES[shape].split("<")[0]

@@ -31,7 +31,14 @@ Ctx::~Ctx() {
   for (int i = 0; i < kArenas; i++)
     if (arena[i]) (void)hipFree(arena[i]);
   if (d_texton_coeffs) (void)hipFree(d_texton_coeffs);
+  if (h_pinned) (void)hipHostFree(h_pinned);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+int Ctx::pinned(int** out) {
+  if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 256, hipHostMallocDefault));
+  *out = h_pinned;
+  return DMSC_OK;
 }
 
 int Ctx::scratch(int slot, size_t bytes, void** ptr) {
@@ -473,6 +480,7 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   float* all = out->d_gpb_all ? out->d_gpb_all : (float*)(B0 + o_all);
   float* gpb = out->d_gpb ? out->d_gpb : (float*)(B0 + o_gpb);
 
+  int* kit = nullptr;
   hipEvent_t ev[7] = {};
   if (timing) {
     for (int i = 0; i < 7; i++) DMSC_HIP(hipEventCreate(&ev[i]));
@@ -493,8 +501,10 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
       DMSC_TRY(launch_kmeans_mixed(s, w, h, ntex, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
     else
       DMSC_TRY(launch_kmeans(s, w, h, ntex, kNFilters, resp, tex, pdot, scr, 15));
-    int ch = 0;
-    DMSC_TRY(kmeans_read_state(s, scr, &ch, &out->kmeans_iterations));
+    // the iteration count reaches the host without a stream synchronisation
+    // here: the eigensolver synchronises before it returns
+    DMSC_TRY(ctx->pinned(&kit));
+    DMSC_HIP(hipMemcpyAsync(kit, kmeans_iterations_ptr(scr), sizeof(int), hipMemcpyDeviceToHost, s));
   }
   if (timing) DMSC_HIP(hipEventRecord(ev[1], s));
   // damascene.cu:352 localCues
@@ -511,6 +521,7 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   // damascene.cu:433 generalizedEigensolve
   DMSC_TRY(run_eigensolve(ctx, w, h, mat, pitch, p.n_eig, p.tol, p.precision, out->h_eigvals, vec,
                           &out->lanczos_iterations));
+  out->kmeans_iterations = *kit;  // copied before the eigensolver's own synchronisations
   if (timing) DMSC_HIP(hipEventRecord(ev[5], s));
   // damascene.cu:441-459 spectralPb + StartCalcGPb
   DMSC_TRY(launch_spectral_pb(s, out->h_eigvals, vec, w, h, p.n_eig, spb, pitch));

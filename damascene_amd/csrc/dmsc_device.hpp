@@ -50,5 +50,17 @@ __device__ __forceinline__ float tree32(T* t) {
   return t[0];
 }
 
+// Logical block of a 1-D grid, XCD-aware.  The device deals workgroups to
+// its 8 XCDs in turn (workgroup i to XCD i % 8), and each XCD has its own
+// L2; this bijection of [0, G) gives XCD x one contiguous range of logical
+// blocks, so the neighbour gathers of its rows mostly hit its own L2.
+// Correctness never depends on the dealing order.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int i, int G) {
+  const int x = i % kXcds, j = i / kXcds;
+  const int q = G / kXcds, r = G % kXcds;
+  return x * q + (x < r ? x : r) + j;
+}
+
 }  // namespace dev
 }  // namespace dmsc

@@ -32,7 +32,9 @@ struct Ctx {
   void* arena[kArenas] = {};
   size_t arena_bytes[kArenas] = {};
   float* d_texton_coeffs = nullptr;
+  int* h_pinned = nullptr;  // small pinned host block for asynchronous read-backs
   ~Ctx();
+  int pinned(int** out);
   int scratch(int slot, size_t bytes, void** ptr);
   int adopt(int slot, void* p, size_t bytes);  // replace arena `slot` by p
   int texton_coeffs(const float** out);

@@ -25,6 +25,7 @@ int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, i
                         void* state, int max_iter, void* work);
 size_t kmeans_mixed_work_bytes(int n);
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations);
+const int* kmeans_iterations_ptr(const void* state);  // device address of the iteration count
 
 // ------------------------------------------------------------- local cues
 size_t local_cues_scratch_bytes(int w, int h);

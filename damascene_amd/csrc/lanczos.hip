@@ -59,6 +59,14 @@ constexpr size_t kInvitBytes = 7 * sizeof(double) + 1;  // LDS bytes per row of 
 constexpr size_t kInvitLds = 156 * 1024;
 constexpr size_t kBisectLdsMax = sizeof(double) * 2 * (kMaxIter + 2);  // (a, b^2) pairs of T
 constexpr int kMaxGraphs = 16;
+// Mixed path screening: every candidate Ritz pair of the check window gets
+// a vector and residual (at most kMaxCand candidates); copies closer than
+// kLumpMixed are one eigenvalue, represented by the copy with the smallest
+// residual; an isolated value within kSpurMixed (relative) of an eigenvalue
+// of T-hat is spurious.
+constexpr int kMaxCand = 128;
+constexpr double kLumpMixed = 1e-8;
+constexpr double kSpurMixed = 1e-7;
 
 // Device-side solver state.
 struct LzState {
@@ -69,6 +77,7 @@ struct LzState {
   int m;                 // order of T at the last check
   double vals[kMaxEig];  // screened Ritz values
   double res[kMaxEig];   // residuals |beta_m s_{m,j}|
+  int sel[kMaxEig];      // row of S holding the Ritz vector of vals[j]
 };
 
 template <typename T>
@@ -137,7 +146,8 @@ __global__ __launch_bounds__(kK1) void k_lz1(LzBufs b, StencilOffsets so, int t)
   }
   // q_k[c] = z_{k-1}[c] * (1 / beta_{k-1}) in FP32: the value the basis
   // stores (step 0 reads q_0 from z1 with beta 1)
-  const int row = blockIdx.x * kK1 + threadIdx.x;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int row = lb * kK1 + threadIdx.x;
   double dot = 0;
   if (row < b.n) {
     const float qv = zp[row] * inv;
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(kK1) void k_lz1(LzBufs b, StencilOffsets so, int t)
     dot = (double)qv * (double)acc;  // exact product
   }
   const double s = block_sum<kK1>(dot, red);
-  if (threadIdx.x == 0) b.part[blockIdx.x] = s;
+  if (threadIdx.x == 0) b.part[lb] = s;
 }
 
 __global__ __launch_bounds__(kK2) void k_lz2(LzBufs b, int t) {
@@ -195,69 +205,41 @@ __global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs b, int m) {
   }
 }
 
-// ---- mixed path: FP32 matrix and basis, FP64 accumulation and vectors; one
-// kernel per step.  Kernel k keeps z'_k = A q_k - beta_{k-1} q_{k-1} (before
-// the alpha correction) and partial sums of q_k.z'_k, |z'_k|^2 and |q_k|^2.
-// Kernel k+1 forms alpha_k = sum q.z', and, since z_k = z'_k - alpha_k q_k,
+// ---- mixed path: FP32 matrix and basis, FP64 accumulation; two kernels
+// per step.  K1m(k) gathers q_k from the basis (FP32), accumulates
+// z'_k = A q_k - beta_{k-1} q_{k-1} in FP64 and leaves partial sums of
+// q_k.z'_k, |z'_k|^2 and |q_k|^2.  K2m(k) forms alpha_k = sum q.z' and,
+// since z_k = z'_k - alpha_k q_k,
 //   beta_k^2 = |z_k|^2 = sum z'^2 - 2 alpha_k sum q.z' + alpha_k^2 sum q^2
 //            = sum z'^2 + alpha_k^2 (sum q^2 - 2),
 // which in FP64 loses at most eps64 |z'|^2 / beta^2 (< 1e-12 here), then
-// gathers q_{k+1}[c] = (z'_k[c] - alpha_k q_k[c]) / beta_k on the fly.  The
-// basis keeps a zero row before q_0 so step 0 needs no special case.  The
-// partials are double-buffered by step parity: kernel k reads buffer
-// (k-1)&1 in every block while its finished blocks already write buffer
-// k&1, so no block can overwrite a partial another block has yet to read.
+// writes q_{k+1} = (z'_k - alpha_k q_k) / beta_k to the basis in FP32.  The
+// basis keeps a zero row before q_0 so step 0 needs no special case.
 struct LzmBufs {
   const float* A;
   size_t pitch;
   int n;
   size_t vstride;
-  double* z0;  // z' of even steps
-  double* z1;  // z' of odd steps (holds q_0 before step 0)
-  double* p1;  // [2][nb] q.z' partials, buffer = step parity
-  double* p2;  // [2][nb] |z'|^2 partials
-  double* p3;  // [2][nb] |q|^2 partials
+  double* z;   // z'_k
+  double* p1;  // [nb1] q.z' partials
+  double* p2;  // [nb1] |z'|^2 partials
+  double* p3;  // [nb1] |q|^2 partials
   double* alpha;
   double* beta;
-  int nb;
+  int nb1, nb2;
   LzState* st;
 };
 
-// alpha and beta of step k-1 from the partials kernel k-1 left in buffer
-// (k-1)&1 (all blocks reduce in the same order).
-__device__ __forceinline__ void lzm_coeffs(const LzmBufs& b, int k, double* red, double& alpha, double& beta) {
-  const size_t o = (size_t)((k - 1) & 1) * b.nb;
-  double v1 = 0, v2 = 0, v3 = 0;
-  for (int i = threadIdx.x; i < b.nb; i += kK1) {
-    v1 += b.p1[o + i];
-    v2 += b.p2[o + i];
-    v3 += b.p3[o + i];
-  }
-  alpha = block_sum<kK1>(v1, red);
-  const double s2 = block_sum<kK1>(v2, red), s3 = block_sum<kK1>(v3, red);
-  beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
-}
-
-__global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t) {
+__global__ __launch_bounds__(kK1) void k_lzm1(LzmBufs b, StencilOffsets so, int t) {
   __shared__ double red[kK1 / kWave];
   const int k = b.st->k + t;
-  float* V = b.st->V;  // row k at V + k * vstride; row -1 is zero
-  const double* zp = (k & 1) ? b.z0 : b.z1;
-  double* zc = (k & 1) ? b.z1 : b.z0;
-  double alpha, beta;
-  lzm_coeffs(b, k, red, alpha, beta);  // step k-1 (alpha 0, beta 1 before step 0)
-  if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-    b.alpha[k - 1] = alpha;
-    b.beta[k - 1] = beta;
-  }
-  const double inv = 1.0 / beta;
-  const float* qm = V + (size_t)(k - 1) * b.vstride;  // q_{k-1}
-  auto q_at = [&](int c) -> float { return (float)(fma(-alpha, (double)qm[c], zp[c]) * inv); };
-  const int row = blockIdx.x * kK1 + threadIdx.x;
+  const float* V = b.st->V;  // row k at V + k * vstride; row -1 is zero
+  const float* q = V + (size_t)k * b.vstride;
+  const double beta = k > 0 ? b.beta[k - 1] : 0.0;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int row = lb * kK1 + threadIdx.x;
   double d1 = 0, d2 = 0, d3 = 0;
   if (row < b.n) {
-    const float qv = q_at(row);
-    V[(size_t)k * b.vstride + row] = qv;
     float a[kStencilArea], x[kStencilArea];
 #pragma unroll
     for (int d = 0; d < kStencilArea; d++) a[d] = b.A[(size_t)d * b.pitch + row];
@@ -265,38 +247,54 @@ __global__ __launch_bounds__(kK1) void k_lzm(LzmBufs b, StencilOffsets so, int t
     for (int d = 0; d < kStencilArea; d++) {
       const int col = row + so.off[d];
       const bool in = (unsigned)col < (unsigned)b.n;
-      const float xv = q_at(in ? col : 0);
+      const float xv = q[in ? col : 0];
       x[d] = in ? xv : 0.0f;
     }
     double acc = 0.0;  // FP64 accumulation of exact FP32 products
 #pragma unroll
     for (int d = 0; d < kStencilArea; d++) acc = fma((double)a[d], (double)x[d], acc);
-    const double zj = fma(-beta, (double)qm[row], acc);
-    zc[row] = zj;
-    d1 = (double)qv * zj;
+    const double qv = (double)q[row];
+    const double zj = fma(-beta, (double)V[(long long)(k - 1) * (long long)b.vstride + row], acc);
+    b.z[row] = zj;
+    d1 = qv * zj;
     d2 = zj * zj;
-    d3 = (double)qv * (double)qv;
+    d3 = qv * qv;
   }
   d1 = block_sum<kK1>(d1, red);
   d2 = block_sum<kK1>(d2, red);
   d3 = block_sum<kK1>(d3, red);
   if (threadIdx.x == 0) {
-    const size_t o = (size_t)(k & 1) * b.nb + blockIdx.x;
-    b.p1[o] = d1;
-    b.p2[o] = d2;
-    b.p3[o] = d3;
+    b.p1[lb] = d1;
+    b.p2[lb] = d2;
+    b.p3[lb] = d3;
   }
 }
 
-// alpha and beta of the last step m-1, for the check.
-__global__ __launch_bounds__(kK1) void k_lzm_finish(LzmBufs b, int m) {
-  __shared__ double red[kK1 / kWave];
-  double alpha, beta;
-  lzm_coeffs(b, m, red, alpha, beta);
-  if (threadIdx.x == 0) {
-    b.alpha[m - 1] = alpha;
-    b.beta[m - 1] = beta;
-    b.st->m = m;
+__global__ __launch_bounds__(kK2) void k_lzm2(LzmBufs b, int t) {
+  __shared__ double red[kK2 / kWave];
+  const int k = b.st->k + t;
+  float* V = b.st->V;
+  double v1 = 0, v2 = 0, v3 = 0;
+  for (int i = threadIdx.x; i < b.nb1; i += kK2) {
+    v1 += b.p1[i];
+    v2 += b.p2[i];
+    v3 += b.p3[i];
+  }
+  const double alpha = block_sum<kK2>(v1, red);
+  const double s2 = block_sum<kK2>(v2, red), s3 = block_sum<kK2>(v3, red);
+  const double beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.alpha[k] = alpha;
+    b.beta[k] = beta;
+  }
+  const double inv = 1.0 / beta;
+  const float* q = V + (size_t)k * b.vstride;
+  float* qn = V + (size_t)(k + 1) * b.vstride;
+  const int base = blockIdx.x * kK2 * kK2Rows + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kK2Rows; j++) {
+    const int i = base + j * kK2;
+    if (i < b.n) qn[i] = (float)(fma(-alpha, (double)q[i], b.z[i]) * inv);
   }
 }
 
@@ -414,7 +412,48 @@ __global__ __launch_bounds__(kWave) void k_tri_screen(LzState* st, const double*
     if (!dup[j])
       for (int c = 0; c < ech; c++)
         if (fabs((acc[j] - wc[c]) / acc[j]) <= 1e-10) accept = false;
-    if (accept) st->vals[ns++] = acc[j];
+    if (accept) {
+      st->sel[ns] = ns;  // vectors are computed after screening, in this order
+      st->vals[ns++] = acc[j];
+    }
+  }
+  st->nscreened = ns;
+  st->fail = ns < nvec ? 1 : 0;
+}
+
+// Mixed path screening, after k_tri_vec has computed a vector and residual
+// for every candidate: values of T closer than kLumpMixed form one group
+// (copies of one eigenvalue), represented by its member with the smallest
+// residual; a single value within kSpurMixed (relative) of an eigenvalue of
+// T-hat is spurious (the Cullum-Willoughby test with tolerances for the
+// FP64-accumulated recurrence, whose copies agree to ~1e-10 while distinct
+// eigenvalues here lie >= 7e-6 apart).
+__global__ __launch_bounds__(kWave) void k_tri_screen_mixed(LzState* st, const double* __restrict__ w,
+                                                            const double* __restrict__ res_all, int ncheck,
+                                                            int nvec) {
+  if (threadIdx.x != 0) return;
+  const int m = st->m;
+  const double* wr = w;           // Ritz values of T, ascending
+  const double* wc = w + ncheck;  // of T-hat
+  const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
+  int ns = 0;
+  for (int j = 0; j < ec && ns < nvec;) {
+    int e = j + 1;
+    while (e < ec && wr[e] - wr[e - 1] < kLumpMixed) e++;
+    bool accept = true;
+    if (e == j + 1)
+      for (int c = 0; c < ech; c++)
+        if (fabs((wr[j] - wc[c]) / wr[j]) <= kSpurMixed) accept = false;
+    if (accept) {
+      int b = j;
+      for (int i = j + 1; i < e; i++)
+        if (res_all[i] < res_all[b]) b = i;
+      st->vals[ns] = wr[b];
+      st->res[ns] = res_all[b];
+      st->sel[ns] = b;
+      ns++;
+    }
+    j = e;
   }
   st->nscreened = ns;
   st->fail = ns < nvec ? 1 : 0;
@@ -424,14 +463,19 @@ __global__ __launch_bounds__(kWave) void k_tri_screen(LzState* st, const double*
 // recurrences) and its residual |beta_m s_{m,j}| (lanczos.cu:114-140: FP32
 // on the plain path).  When they fit, alpha, beta and the work arrays are
 // staged in LDS; otherwise the work arrays live in global memory.
+// Plain path: vector j of the screened value st->vals[j], residual to
+// st->res[j].  Mixed path (cand != null): vector j of candidate cand[j]
+// (every Ritz value of the check window, before screening), residual to
+// res_all[j]; a failed solve there only marks that candidate.
 __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ alpha, const double* __restrict__ beta,
                                                    LzState* st, int mixed, double* __restrict__ S,
-                                                   double* __restrict__ gwork, int lds) {
+                                                   double* __restrict__ gwork, int lds,
+                                                   const double* __restrict__ cand, double* __restrict__ res_all) {
   extern __shared__ double dyn[];
-  if (st->fail == 1) return;
+  if (!cand && st->fail == 1) return;
   const int j = blockIdx.x, m = st->m;
   double* sj = S + (size_t)j * m;
-  const double lam = st->vals[j];
+  const double lam = cand ? cand[j] : st->vals[j];
   int bad;
   if (lds) {
     double* sa = dyn;
@@ -453,6 +497,10 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
   }
   const double bm = beta[m - 1];
   const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
+  if (cand) {
+    res_all[j] = bad ? INFINITY : r;
+    return;
+  }
   st->res[j] = bad ? INFINITY : r;
   if (bad) st->fail = 2;
 }
@@ -470,11 +518,14 @@ __global__ __launch_bounds__(256) void k_ritz(int n, const LzState* st, size_t v
   double acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) acc[j] = 0;
+  const double* Sj[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) Sj[j] = S + (size_t)st->sel[j] * m;
   for (int k = 0; k < m; k++) {
     const double v = (double)V[(size_t)k * vstride + p];
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-      const double sj = S32 ? (double)(float)S[(size_t)j * m + k] : S[(size_t)j * m + k];
+      const double sj = S32 ? (double)(float)Sj[j][k] : Sj[j][k];
       acc[j] = fma(v, sj, acc[j]);
     }
   }
@@ -551,10 +602,10 @@ __global__ void k_normalize(int n, int nvec, const float* __restrict__ fin, floa
   }
 }
 
-// Basis row -1 (zeros), q_0's source in z1, and the step-(-1) partials of
-// the mixed path (alpha 0, |z'|^2 1, |q|^2 0 -> beta 1).
+// Solver state, q_0 in basis row 0 and q_0's source in z1 (the plain
+// path's first step reads q_0 from z1 with beta 1).
 template <typename Z>
-__global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n, double* p1, double* p2, double* p3, int nb) {
+__global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     st->k = 0;
@@ -567,11 +618,6 @@ __global__ void k_init(LzState* st, float* V, Z* z1, float q0, int n, double* p1
     V[i] = q0;
     z1[i] = (Z)q0;
   }
-  if (p1 && i < nb) {  // the step -1 buffer (parity 1): alpha 0, beta 1
-    p1[nb + i] = 0.0;
-    p2[nb + i] = i == 0 ? 1.0 : 0.0;
-    p3[nb + i] = 0.0;
-  }
 }
 
 __global__ void k_zero_row(float* row, int n) {
@@ -581,6 +627,7 @@ __global__ void k_zero_row(float* row, int n) {
 
 __global__ void k_set_basis(LzState* st, float* V) { st->V = V; }
 __global__ void k_set_step(LzState* st, int k) { st->k = k; }
+__global__ void k_set_m(LzState* st, int m) { st->m = m; }
 
 // ---------------------------------------------------------------- host
 // Captured chunks of kGraphSteps steps, per context; the kernels' arguments
@@ -623,7 +670,8 @@ struct Steps {
 void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
   for (int t = 0; t < steps; t++) {
     if (p.mixed) {
-      hipLaunchKernelGGL(k_lzm, dim3(p.mix.nb), dim3(kK1), 0, s, p.mix, p.so, t);
+      hipLaunchKernelGGL(k_lzm1, dim3(p.mix.nb1), dim3(kK1), 0, s, p.mix, p.so, t);
+      hipLaunchKernelGGL(k_lzm2, dim3(p.mix.nb2), dim3(kK2), 0, s, p.mix, t);
     } else {
       hipLaunchKernelGGL(k_lz1, dim3(p.plain.nb1), dim3(kK1), 0, s, p.plain, p.so, t);
       hipLaunchKernelGGL(k_lz2, dim3(p.plain.nb2), dim3(kK2), 0, s, p.plain, t);
@@ -724,13 +772,15 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const size_t o_st = carve(sizeof(LzState));
   const size_t o_z0 = carve(zb * vstride), o_z1 = carve(zb * vstride);
   const size_t o_rs = carve(sizeof(float) * vstride);
-  const size_t o_p1 = carve(sizeof(double) * 2 * nb1), o_p2 = carve(sizeof(double) * 2 * nb1),
-               o_p3 = carve(sizeof(double) * 2 * nb1);
+  const size_t o_p1 = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb1),
+               o_p3 = carve(sizeof(double) * nb1);
   const size_t o_al = carve(zb * (kMaxIter + 2)), o_be = carve(zb * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
   const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);
-  const size_t o_S = carve(sizeof(double) * kMaxEig * (kMaxIter + 2));
-  const size_t o_work = carve(sizeof(double) * 6 * kMaxEig * (kMaxIter + 2));
+  const int nvrows = mixed ? kMaxCand : kMaxEig;  // rows of S: candidates (mixed) or screened values
+  const size_t o_S = carve(sizeof(double) * nvrows * (kMaxIter + 2));
+  const size_t o_work = carve(sizeof(double) * 6 * nvrows * (kMaxIter + 2));
+  const size_t o_resall = carve(sizeof(double) * kMaxCand);
   const size_t o_mm = carve(sizeof(float) * 2 * kMaxEig * nblk), o_fin = carve(sizeof(float) * 2 * kMaxEig);
   void* base;
   DMSC_TRY(ctx->scratch(2, off, &base));
@@ -753,8 +803,8 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   p.so = make_stencil_offsets(w);
   p.plain = LzBufs{A, (size_t)pitch, n, vstride, (float*)(B + o_z0), (float*)(B + o_z1), (double*)(B + o_p1),
                    (double*)(B + o_p2), (float*)(B + o_al), (float*)(B + o_be), nb1, nb2, st};
-  p.mix = LzmBufs{A, (size_t)pitch, n, vstride, (double*)(B + o_z0), (double*)(B + o_z1), (double*)(B + o_p1),
-                  (double*)(B + o_p2), (double*)(B + o_p3), (double*)(B + o_al), (double*)(B + o_be), nb1, st};
+  p.mix = LzmBufs{A, (size_t)pitch, n, vstride, (double*)(B + o_z0), (double*)(B + o_p1), (double*)(B + o_p2),
+                  (double*)(B + o_p3), (double*)(B + o_al), (double*)(B + o_be), nb1, nb2, st};
 
   static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
   if (!invit_lds) {
@@ -774,10 +824,9 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const dim3 gi(div_up(std::max(n, nb1), 256));
   hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
   if (mixed)
-    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.mix.z1, q0, n, p.mix.p1, p.mix.p2, p.mix.p3, nb1);
+    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.mix.z, q0, n);
   else
-    hipLaunchKernelGGL(k_init<float>, gi, dim3(256), 0, s, st, V, p.plain.z1, q0, n, (double*)nullptr,
-                       (double*)nullptr, (double*)nullptr, 0);
+    hipLaunchKernelGGL(k_init<float>, gi, dim3(256), 0, s, st, V, p.plain.z1, q0, n);
   DMSC_HIP(hipGetLastError());
 
   int eigCheck = nvec + kCullumExtra;
@@ -801,7 +850,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       DMSC_TRY(run_steps(ctx, p, st, w, done + t, std::min(kGraphSteps, steps - t)));
     done += steps;
     if (mixed) {
-      hipLaunchKernelGGL(k_lzm_finish, dim3(1), dim3(kK1), 0, s, p.mix, done);
+      hipLaunchKernelGGL(k_set_m, dim3(1), dim3(1), 0, s, st, done);  // alpha, beta[done-1] written by K2m
       hipLaunchKernelGGL(k_widen<double>, dim3(div_up(done, 256)), dim3(256), 0, s, p.mix.alpha, p.mix.beta, ad, bd,
                          done);
     } else {
@@ -813,16 +862,25 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       const int ncheck = std::min(eigCheck, done);
       hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, s, ad, bd, st, wv,
                          ncheck);
-      hipLaunchKernelGGL(k_tri_screen, dim3(1), dim3(kWave), 0, s, st, wv, ncheck, nvec);
       const bool lds = (size_t)done * kInvitBytes <= invit_lds;
-      hipLaunchKernelGGL(k_tri_vec, dim3(nvec), dim3(kWave), lds ? (size_t)done * kInvitBytes : 0, s, ad, bd, st,
-                         mixed ? 1 : 0, S, (double*)(B + o_work), lds ? 1 : 0);
+      const size_t vlds = lds ? (size_t)done * kInvitBytes : 0;
+      if (mixed) {  // vectors and residuals of every candidate, then screening
+        double* res_all = (double*)(B + o_resall);
+        hipLaunchKernelGGL(k_tri_vec, dim3(ncheck), dim3(kWave), vlds, s, ad, bd, st, 1, S, (double*)(B + o_work),
+                           lds ? 1 : 0, (const double*)wv, res_all);
+        hipLaunchKernelGGL(k_tri_screen_mixed, dim3(1), dim3(kWave), 0, s, st, wv, res_all, ncheck, nvec);
+      } else {  // lanczos.cu:279-384 screening, then vectors of the screened values
+        hipLaunchKernelGGL(k_tri_screen, dim3(1), dim3(kWave), 0, s, st, wv, ncheck, nvec);
+        hipLaunchKernelGGL(k_tri_vec, dim3(nvec), dim3(kWave), vlds, s, ad, bd, st, 0, S, (double*)(B + o_work),
+                           lds ? 1 : 0, (const double*)nullptr, (double*)nullptr);
+      }
       DMSC_HIP(hipGetLastError());
       DMSC_HIP(hipMemcpyAsync(&hs, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
       DMSC_HIP(hipStreamSynchronize(s));
       if (hs.fail != 1 || eigCheck >= done) break;
       eigCheck += kCullumExtra;  // lanczos.cu:528-532: widen the screened window
-      if (eigCheck > kCullumMax) return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
+      if (eigCheck > (mixed ? kMaxCand : kCullumMax))
+        return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
     }
     if (hs.fail == 2) return fail(DMSC_ERR_INTERNAL, "inverse iteration failed on the Lanczos tridiagonal");
     if (hs.fail == 0) {

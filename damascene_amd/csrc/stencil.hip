@@ -2,14 +2,15 @@
 // transform and SpMV (reference: stencilMatrixMultiply/stencilMVM.cu).
 //
 // SpMV layout: A is DIA [81][pitch] (diagonal-major), so for a fixed
-// diagonal consecutive rows are consecutive floats: each lane owns 4
-// consecutive rows and reads one 16-byte vector per diagonal (a wave reads
-// 1 KiB contiguous per diagonal).  The x window a block needs (its rows plus
-// a 5-image-row halo each side) is staged in LDS with zeros outside [0, N),
-// which makes the reference's column bounds test branch-free: an
-// out-of-range column contributes fma(A, 0, acc) == acc.  Each row sums its
-// 81 products sequentially in diagonal order (fused multiply-add), exactly
-// as the CPU oracle.
+// diagonal consecutive rows are consecutive floats and a wave's matrix loads
+// are contiguous.  Lanes own 1, 2 or 4 consecutive rows (k_spmv<RPL>) or one
+// row with all 81 loads in flight (k_spmv_full); x is gathered through the
+// caches.  An out-of-range column reads x[0] and contributes
+// fma(A, 0, acc) == acc, which makes the reference's column bounds test
+// branch-free.  Each row sums its 81 products sequentially in diagonal
+// order (fused multiply-add), exactly as the CPU oracle.  Blocks are
+// numbered XCD-aware (dev::xcd_block), so each XCD's L2 holds one
+// contiguous slice of x.
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
 #include "kernels.hpp"
@@ -82,7 +83,7 @@ namespace {
 template <int RPL>
 __global__ __launch_bounds__(64) void k_spmv(int n, StencilOffsets so, const float* __restrict__ A, size_t pitch,
                                              const float* __restrict__ x, float* __restrict__ y) {
-  const int row = (blockIdx.x * 64 + threadIdx.x) * RPL;
+  const int row = (dev::xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x) * RPL;
   if (row >= n) return;
   float acc[RPL];
   spmv_rows<RPL>(acc, A, pitch, row, n, so, [&](int c) { return x[c]; });
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(64) void k_spmv(int n, StencilOffsets so, const flo
 __global__ __launch_bounds__(256) void k_spmv_full(int n, StencilOffsets so, const float* __restrict__ A,
                                                    size_t pitch, const float* __restrict__ x,
                                                    float* __restrict__ y) {
-  const int row = blockIdx.x * 256 + threadIdx.x;
+  const int row = dev::xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (row >= n) return;
   float a[kStencilArea], xv[kStencilArea];
 #pragma unroll

@@ -449,6 +449,10 @@ int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp,
   return fail(DMSC_ERR_ARG, "k-means supports 32 or 64 clusters");
 }
 
+const int* kmeans_iterations_ptr(const void* state) {
+  return reinterpret_cast<const int*>(&static_cast<const KmState*>(state)->iterations);
+}
+
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations) {
   KmState h;
   DMSC_HIP(hipMemcpyAsync(&h, state, sizeof(KmState), hipMemcpyDeviceToHost, s));

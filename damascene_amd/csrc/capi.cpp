@@ -191,12 +191,29 @@ int dmsc_tune_set(const char* key, int value) {
     spmv_rows_per_lane = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "lanczos_trace")) {
+    lanczos_trace = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_k1_threads")) {
+    DMSC_ARG(value == 64 || value == 256, "lanczos_k1_threads must be 64 or 256");
+    lanczos_k1_threads = value;
+    return DMSC_OK;
+  }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
 }
 int dmsc_tune_get(const char* key, int* value) {
   DMSC_ARG(key && value, "null argument");
   if (!std::strcmp(key, "spmv_rows_per_lane")) {
     *value = spmv_rows_per_lane;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_trace")) {
+    *value = lanczos_trace;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_k1_threads")) {
+    *value = lanczos_k1_threads;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);

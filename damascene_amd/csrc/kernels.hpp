@@ -86,6 +86,10 @@ __device__ __forceinline__ void spmv_rows(float (&acc)[RPL], const float* __rest
 
 // SpMV launch shape chosen by the tuner (0: k_spmv_full; 1, 2, 4 rows per lane).
 extern int spmv_rows_per_lane;
+// Non-zero: the eigensolver prints one line per check to stderr.
+extern int lanczos_trace;
+// Block size of the Lanczos SpMV step kernel (64 or 256).
+extern int lanczos_k1_threads;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,

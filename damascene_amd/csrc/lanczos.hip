@@ -34,7 +34,9 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <cstdio>
 #include <tuple>
+#include <vector>
 
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
@@ -43,6 +45,9 @@
 
 namespace dmsc {
 
+int lanczos_trace = 0;
+int lanczos_k1_threads = 256;
+
 namespace {
 
 constexpr int kGraphSteps = 25;   // steps per captured graph; periods are multiples of it
@@ -50,7 +55,9 @@ constexpr int kMaxEig = 25;
 constexpr int kCullumExtra = 20;  // eigCheck = nvec + 20 (lanczos.cu:456)
 constexpr int kCullumMax = 1000;  // lanczos.cu:531
 constexpr int kWave = 64;
-constexpr int kK1 = 256;          // K1: one stencil row per lane
+// K1 (the SpMV step kernel): one stencil row per lane, kK1 lanes per block
+// (a tuning choice: 64 or 256; larger blocks leave fewer partials for K2).
+constexpr int kRed = 256;         // single-block reductions
 constexpr int kK2 = 256;          // K2: kK2Rows rows per lane
 constexpr int kK2Rows = 8;
 constexpr int kBisect = 256;      // shifts per multisection round
@@ -131,6 +138,7 @@ struct LzBufs {
   LzState* st;
 };
 
+template <int kK1>
 __global__ __launch_bounds__(kK1) void k_lz1(LzBufs b, StencilOffsets so, int t) {
   __shared__ double red[kK1 / kWave];
   const int k = b.st->k + t;
@@ -152,22 +160,14 @@ __global__ __launch_bounds__(kK1) void k_lz1(LzBufs b, StencilOffsets so, int t)
   if (row < b.n) {
     const float qv = zp[row] * inv;
     V[(size_t)k * b.vstride + row] = qv;
-    float a[kStencilArea], x[kStencilArea];
-#pragma unroll
-    for (int d = 0; d < kStencilArea; d++) a[d] = b.A[(size_t)d * b.pitch + row];
-#pragma unroll
-    for (int d = 0; d < kStencilArea; d++) {
-      const int col = row + so.off[d];
-      const bool in = (unsigned)col < (unsigned)b.n;
-      const float xv = zp[in ? col : 0] * inv;
-      x[d] = in ? xv : 0.0f;  // fma(a, 0, acc) == acc: the reference's bounds test
-    }
-    float acc = 0.0f;  // diagonal order, as stencilMVM.cu:127-180
-#pragma unroll
-    for (int d = 0; d < kStencilArea; d++) acc = fmaf(a[d], x[d], acc);
-    if (k > 0) acc = fmaf(nbeta, V[(size_t)(k - 1) * b.vstride + row], acc);
-    zc[row] = acc;
-    dot = (double)qv * (double)acc;  // exact product
+    // diagonal order, as stencilMVM.cu:127-180; an out-of-range column
+    // contributes fma(a, 0, acc) == acc (the reference's bounds test)
+    float acc[1];
+    spmv_rows<1>(acc, b.A, b.pitch, row, b.n, so, [&](int c) { return zp[c] * inv; });
+    float z = acc[0];
+    if (k > 0) z = fmaf(nbeta, V[(size_t)(k - 1) * b.vstride + row], z);
+    zc[row] = z;
+    dot = (double)qv * (double)z;  // exact product
   }
   const double s = block_sum<kK1>(dot, red);
   if (threadIdx.x == 0) b.part[lb] = s;
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(kK2) void k_lz2(LzBufs b, int t) {
 }
 
 // beta of the last step m-1 (needed by the check's residuals).
-__global__ __launch_bounds__(kK1) void k_lz_beta(LzBufs b, int m) {
-  __shared__ double red[kK1 / kWave];
-  const float beta = (float)sqrt(sum_partials<kK1>(b.part2, b.nb2, red));
+__global__ __launch_bounds__(kRed) void k_lz_beta(LzBufs b, int m) {
+  __shared__ double red[kRed / kWave];
+  const float beta = (float)sqrt(sum_partials<kRed>(b.part2, b.nb2, red));
   if (threadIdx.x == 0) {
     b.beta[m - 1] = beta;
     b.st->m = m;
@@ -230,6 +230,7 @@ struct LzmBufs {
   LzState* st;
 };
 
+template <int kK1>
 __global__ __launch_bounds__(kK1) void k_lzm1(LzmBufs b, StencilOffsets so, int t) {
   __shared__ double red[kK1 / kWave];
   const int k = b.st->k + t;
@@ -240,19 +241,15 @@ __global__ __launch_bounds__(kK1) void k_lzm1(LzmBufs b, StencilOffsets so, int 
   const int row = lb * kK1 + threadIdx.x;
   double d1 = 0, d2 = 0, d3 = 0;
   if (row < b.n) {
-    float a[kStencilArea], x[kStencilArea];
-#pragma unroll
-    for (int d = 0; d < kStencilArea; d++) a[d] = b.A[(size_t)d * b.pitch + row];
-#pragma unroll
+    double acc = 0.0;  // FP64 accumulation of exact FP32 products, diagonal order
+#pragma unroll 9
     for (int d = 0; d < kStencilArea; d++) {
+      const float a = b.A[(size_t)d * b.pitch + row];
       const int col = row + so.off[d];
       const bool in = (unsigned)col < (unsigned)b.n;
       const float xv = q[in ? col : 0];
-      x[d] = in ? xv : 0.0f;
+      acc = fma((double)a, (double)(in ? xv : 0.0f), acc);
     }
-    double acc = 0.0;  // FP64 accumulation of exact FP32 products
-#pragma unroll
-    for (int d = 0; d < kStencilArea; d++) acc = fma((double)a[d], (double)x[d], acc);
     const double qv = (double)q[row];
     const double zj = fma(-beta, (double)V[(long long)(k - 1) * (long long)b.vstride + row], acc);
     b.z[row] = zj;
@@ -310,14 +307,26 @@ __global__ void k_widen(const Z* __restrict__ al, const Z* __restrict__ be, doub
 }
 
 // ---------------------------------------------------------------- check
+// Ritz values live in w[which * kWStride + idx] (which 0: T, 1: T-hat), so a
+// check can read the previous check's values whatever its window size.
+constexpr int kWStride = kCullumMax;
+
 // Block (idx, which): eigenvalue idx+1 of T (which 0, order m) or of T-hat
 // (which 1, order m-1: alpha[1..], beta[1..]) by multisection with kBisect
-// Sturm counts per round, to the relative accuracy of LAPACK dstebz with
-// abstol 0.  The (a_i, b_{i-1}^2) pairs are staged in LDS once and read by
-// every round; the Gershgorin interval is reduced across the block.
+// Sturm counts per round, to relative accuracy 4.4e-16 or absolute
+// 2.2e-16 |T|, whichever is larger (the accuracy T's entries carry).  The
+// (a_i, b_{i-1}^2) pairs are staged in LDS once and read by every round.
+//
+// Bracket: T (and T-hat) of the previous check at order prev_m < m is a
+// leading principal submatrix of this one, so by Cauchy interlacing its
+// value idx+1 bounds this one from above; and T is the Rayleigh quotient
+// of L_sym, which is positive semidefinite, so 0 bounds it from below up to
+// rounding.  The first round puts its two outer shifts on that bracket and
+// falls back to the Gershgorin interval if either side does not hold, so
+// the result never depends on the bracket being right.
 __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict__ alpha,
                                                         const double* __restrict__ beta, const LzState* st,
-                                                        double* __restrict__ wout, int ncheck) {
+                                                        double* __restrict__ w, int prev_n) {
   extern __shared__ double2 sab[];
   __shared__ double rlo[kBisect / kWave], rhi[kBisect / kWave], rb2[kBisect / kWave];
   __shared__ unsigned long long masks[kBisect / kWave];
@@ -349,19 +358,42 @@ __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict
   }
   __syncthreads();
 #pragma unroll
-  for (int w = 0; w < kBisect / kWave; w++) {
-    lo = fmin(lo, rlo[w]);
-    hi = fmax(hi, rhi[w]);
-    bmax = fmax(bmax, rb2[w]);
+  for (int v = 0; v < kBisect / kWave; v++) {
+    lo = fmin(lo, rlo[v]);
+    hi = fmax(hi, rhi[v]);
+    bmax = fmax(bmax, rb2[v]);
   }
   const double tnorm = fmax(fabs(lo), fabs(hi));
   const double pivmin = fmax(2.2250738585072014e-308 * fmax(1.0, bmax), 1e-300);
+  const double abstol = 2.2e-16 * tnorm;
   const double pad = 2.2e-16 * tnorm * n + 2 * pivmin;
   lo -= pad;
   hi += pad;
   const int kk = idx + 1;
-  for (int round = 0; round < 20; round++) {
-    if (hi - lo <= 4.4e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+  double* slot = w + which * kWStride + idx;
+  if (idx < prev_n - which) {  // first round on the interlacing / semidefinite bracket
+    const double bl = -64 * abstol - 2 * pivmin;
+    const double bh = fmin(hi, *slot + 8.8e-16 * fabs(*slot) + 64 * abstol + 2 * pivmin);
+    if (bl < bh) {
+      const double step = (bh - bl) / (double)(kBisect - 1);
+      const double x = threadIdx.x == kBisect - 1 ? bh : bl + step * threadIdx.x;
+      const int c = tri::sturm_pairs(n, sab, x);
+      const unsigned long long ge = __ballot(c >= kk);
+      if ((threadIdx.x & (kWave - 1)) == 0) masks[threadIdx.x / kWave] = ge;
+      __syncthreads();
+      int first = -1;
+#pragma unroll
+      for (int v = 0; v < kBisect / kWave; v++)
+        if (first < 0 && masks[v]) first = v * kWave + __ffsll((unsigned long long)masks[v]) - 1;
+      __syncthreads();
+      if (first > 0) {  // count(bl) < kk <= count(bh): the bracket holds
+        lo = bl + step * (first - 1);
+        hi = first == kBisect - 1 ? bh : bl + step * first;
+      }
+    }
+  }
+  for (int round = 0; round < 24; round++) {
+    if (hi - lo <= fmax(4.4e-16 * fmax(fabs(lo), fabs(hi)), abstol) + pivmin) break;
     const double step = (hi - lo) / (double)(kBisect + 1);
     const int c = tri::sturm_pairs(n, sab, lo + step * (threadIdx.x + 1));
     // the first shift with at least kk values below it bounds lambda_kk above
@@ -370,8 +402,8 @@ __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict
     __syncthreads();
     int first = -1;
 #pragma unroll
-    for (int w = 0; w < kBisect / kWave; w++)
-      if (first < 0 && masks[w]) first = w * kWave + __ffsll((unsigned long long)masks[w]) - 1;
+    for (int v = 0; v < kBisect / kWave; v++)
+      if (first < 0 && masks[v]) first = v * kWave + __ffsll((unsigned long long)masks[v]) - 1;
     __syncthreads();
     if (first >= 0) {
       const double nhi = lo + step * (first + 1);
@@ -381,7 +413,7 @@ __global__ __launch_bounds__(kBisect) void k_tri_bisect(const double* __restrict
       lo = lo + step * kBisect;
     }
   }
-  if (threadIdx.x == 0) wout[which * ncheck + idx] = 0.5 * (lo + hi);
+  if (threadIdx.x == 0) *slot = 0.5 * (lo + hi);
 }
 
 // Screening (lanczos.cu:279-384): lump values closer than 1e-5, drop single
@@ -394,7 +426,7 @@ __global__ __launch_bounds__(kWave) void k_tri_screen(LzState* st, const double*
   if (threadIdx.x != 0) return;
   const int m = st->m;
   const double* wr = w;           // Ritz values of T, ascending
-  const double* wc = w + ncheck;  // of T-hat
+  const double* wc = w + kWStride;  // of T-hat
   const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
   int na = 0;
   for (int j = 0; j < ec; j++) {
@@ -434,7 +466,7 @@ __global__ __launch_bounds__(kWave) void k_tri_screen_mixed(LzState* st, const d
   if (threadIdx.x != 0) return;
   const int m = st->m;
   const double* wr = w;           // Ritz values of T, ascending
-  const double* wc = w + ncheck;  // of T-hat
+  const double* wc = w + kWStride;  // of T-hat
   const int ec = min(ncheck, m), ech = min(ncheck, m - 1);
   int ns = 0;
   for (int j = 0; j < ec && ns < nvec;) {
@@ -476,6 +508,9 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
   const int j = blockIdx.x, m = st->m;
   double* sj = S + (size_t)j * m;
   const double lam = cand ? cand[j] : st->vals[j];
+  // one solve for a candidate (its residual and, if selected, its vector:
+  // the other eigenvectors stay below eps |T| / gap), two for the plain path
+  const int iters = cand ? 1 : kInvitIters;
   int bad;
   if (lds) {
     double* sa = dyn;
@@ -488,12 +523,12 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
     if (threadIdx.x != 0) return;
     double* wk = dyn + 2 * m;
     bad = tri::invit(m, sa, sb, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
-                     (unsigned char*)(wk + 5 * m), kInvitIters);
+                     (unsigned char*)(wk + 5 * m), iters);
   } else {
     if (threadIdx.x != 0) return;
     double* wk = gwork + (size_t)j * 6 * m;
     bad = tri::invit(m, alpha, beta, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
-                     (unsigned char*)(wk + 5 * m), kInvitIters);
+                     (unsigned char*)(wk + 5 * m), iters);
   }
   const double bm = beta[m - 1];
   const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
@@ -512,23 +547,45 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
 template <int NV, bool S32>
 __global__ __launch_bounds__(256) void k_ritz(int n, const LzState* st, size_t vstride, int m,
                                               const double* __restrict__ S, float* __restrict__ out) {
+  // S is staged through LDS in chunks of kRitzChunk steps (every lane reads
+  // the same coefficient, a broadcast); V is read 8 rows at a time so each
+  // lane keeps 8 loads in flight.  The sums run in step order as before.
+  constexpr int kRitzChunk = 256;
+  __shared__ double sc[NV][kRitzChunk];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const float* V = st->V;
+  const bool live = p < n;
+  const float* V = st->V + (live ? p : 0);
   double acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) acc[j] = 0;
-  const double* Sj[NV];
+  for (int k0 = 0; k0 < m; k0 += kRitzChunk) {
+    const int len = min(kRitzChunk, m - k0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NV * kRitzChunk; i += blockDim.x) {
+      const int j = i / kRitzChunk, k = i - j * kRitzChunk;
+      if (k < len) {
+        const double v = S[(size_t)st->sel[j] * m + k0 + k];
+        sc[j][k] = S32 ? (double)(float)v : v;
+      }
+    }
+    __syncthreads();
+    int k = 0;
+    for (; k + 8 <= len; k += 8) {
+      double v[8];
 #pragma unroll
-  for (int j = 0; j < NV; j++) Sj[j] = S + (size_t)st->sel[j] * m;
-  for (int k = 0; k < m; k++) {
-    const double v = (double)V[(size_t)k * vstride + p];
+      for (int u = 0; u < 8; u++) v[u] = (double)V[(size_t)(k0 + k + u) * vstride];
 #pragma unroll
-    for (int j = 0; j < NV; j++) {
-      const double sj = S32 ? (double)(float)Sj[j][k] : Sj[j][k];
-      acc[j] = fma(v, sj, acc[j]);
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int j = 0; j < NV; j++) acc[j] = fma(v[u], sc[j][k + u], acc[j]);
+    }
+    for (; k < len; k++) {
+      const double v = (double)V[(size_t)(k0 + k) * vstride];
+#pragma unroll
+      for (int j = 0; j < NV; j++) acc[j] = fma(v, sc[j][k], acc[j]);
     }
   }
+  if (!live) return;
 #pragma unroll
   for (int j = 0; j < NV; j++) out[(size_t)j * n + p] = (float)acc[j];
 }
@@ -630,15 +687,17 @@ __global__ void k_set_step(LzState* st, int k) { st->k = k; }
 __global__ void k_set_m(LzState* st, int m) { st->m = m; }
 
 // ---------------------------------------------------------------- host
+
 // Captured chunks of kGraphSteps steps, per context; the kernels' arguments
 // are baked into the graph, so the key holds every pointer and size they
 // depend on.
 struct GraphKey {
   const void* A;
   const void* st;
-  int pitch, n, w, mixed, steps;
+  int pitch, n, w, mixed, steps, k1;
   bool operator<(const GraphKey& o) const {
-    return std::tie(A, st, pitch, n, w, mixed, steps) < std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps);
+    return std::tie(A, st, pitch, n, w, mixed, steps, k1) <
+           std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps, o.k1);
   }
 };
 
@@ -662,6 +721,7 @@ std::map<const Ctx*, std::unique_ptr<GraphCache>>& graph_caches() {
 
 struct Steps {
   bool mixed;
+  int k1;  // K1 block size
   LzBufs plain;
   LzmBufs mix;
   StencilOffsets so;
@@ -670,10 +730,16 @@ struct Steps {
 void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
   for (int t = 0; t < steps; t++) {
     if (p.mixed) {
-      hipLaunchKernelGGL(k_lzm1, dim3(p.mix.nb1), dim3(kK1), 0, s, p.mix, p.so, t);
+      if (p.k1 == 64)
+        hipLaunchKernelGGL(k_lzm1<64>, dim3(p.mix.nb1), dim3(64), 0, s, p.mix, p.so, t);
+      else
+        hipLaunchKernelGGL(k_lzm1<256>, dim3(p.mix.nb1), dim3(256), 0, s, p.mix, p.so, t);
       hipLaunchKernelGGL(k_lzm2, dim3(p.mix.nb2), dim3(kK2), 0, s, p.mix, t);
     } else {
-      hipLaunchKernelGGL(k_lz1, dim3(p.plain.nb1), dim3(kK1), 0, s, p.plain, p.so, t);
+      if (p.k1 == 64)
+        hipLaunchKernelGGL(k_lz1<64>, dim3(p.plain.nb1), dim3(64), 0, s, p.plain, p.so, t);
+      else
+        hipLaunchKernelGGL(k_lz1<256>, dim3(p.plain.nb1), dim3(256), 0, s, p.plain, p.so, t);
       hipLaunchKernelGGL(k_lz2, dim3(p.plain.nb2), dim3(kK2), 0, s, p.plain, t);
     }
   }
@@ -686,7 +752,7 @@ int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps
   if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
-                     steps};
+                     steps, p.k1};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
@@ -756,7 +822,8 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const size_t vstride = (size_t)pitch_floats(n);
-  const int nb1 = div_up(n, kK1), nb2 = div_up(n, kK2 * kK2Rows);
+  const int k1 = lanczos_k1_threads == 64 ? 64 : 256;
+  const int nb1 = div_up(n, k1), nb2 = div_up(n, kK2 * kK2Rows);
   int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
   const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
   const int nblk = std::min(div_up(n, 256), 1024);
@@ -776,7 +843,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
                o_p3 = carve(sizeof(double) * nb1);
   const size_t o_al = carve(zb * (kMaxIter + 2)), o_be = carve(zb * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
-  const size_t o_w = carve(sizeof(double) * 2 * kCullumMax);
+  const size_t o_w = carve(sizeof(double) * 2 * kWStride);
   const int nvrows = mixed ? kMaxCand : kMaxEig;  // rows of S: candidates (mixed) or screened values
   const size_t o_S = carve(sizeof(double) * nvrows * (kMaxIter + 2));
   const size_t o_work = carve(sizeof(double) * 6 * nvrows * (kMaxIter + 2));
@@ -800,6 +867,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
 
   Steps p;
   p.mixed = mixed;
+  p.k1 = k1;
   p.so = make_stencil_offsets(w);
   p.plain = LzBufs{A, (size_t)pitch, n, vstride, (float*)(B + o_z0), (float*)(B + o_z1), (double*)(B + o_p1),
                    (double*)(B + o_p2), (float*)(B + o_al), (float*)(B + o_be), nb1, nb2, st};
@@ -830,17 +898,20 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   DMSC_HIP(hipGetLastError());
 
   int eigCheck = nvec + kCullumExtra;
-  int done = 0, prev_m = 0;
+  int done = 0, prev_m = 0, prev_n = 0;  // prev_n: Ritz values of the last check left in wv
   double prev_res = 0;
   LzState hs{};
   bool converged = false;
   while (done < kMaxIter) {
     const int steps = std::min(period, kMaxIter - done);
-    if ((size_t)done + steps + 2 > cap) {  // grow the basis, keeping the zero row and q_0..q_{done-1}
+    if ((size_t)done + steps + 2 > cap) {
+      // grow the basis, keeping the zero row and q_0..q_done: the mixed
+      // path's last K2 has already written q_done, which its next K1 reads
+      // (the plain path's K1 writes its own row; copying it is harmless)
       const size_t ncap = std::min<size_t>(std::max(cap * 2, (size_t)done + steps + 2), kMaxIter + 2);
       void* nbuf;
       DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * (ncap + 1)));
-      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * (done + 1), hipMemcpyDeviceToDevice, s));
+      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * (done + 2), hipMemcpyDeviceToDevice, s));
       DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * (ncap + 1)));  // synchronises before freeing
       V = (float*)nbuf + vstride;
       cap = ncap;
@@ -854,14 +925,15 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       hipLaunchKernelGGL(k_widen<double>, dim3(div_up(done, 256)), dim3(256), 0, s, p.mix.alpha, p.mix.beta, ad, bd,
                          done);
     } else {
-      hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kK1), 0, s, p.plain, done);
+      hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kRed), 0, s, p.plain, done);
       hipLaunchKernelGGL(k_widen<float>, dim3(div_up(done, 256)), dim3(256), 0, s, p.plain.alpha, p.plain.beta, ad,
                          bd, done);
     }
     for (;;) {
       const int ncheck = std::min(eigCheck, done);
       hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, s, ad, bd, st, wv,
-                         ncheck);
+                         prev_n);
+      prev_n = ncheck;
       const bool lds = (size_t)done * kInvitBytes <= invit_lds;
       const size_t vlds = lds ? (size_t)done * kInvitBytes : 0;
       if (mixed) {  // vectors and residuals of every candidate, then screening
@@ -877,10 +949,23 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       DMSC_HIP(hipGetLastError());
       DMSC_HIP(hipMemcpyAsync(&hs, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
       DMSC_HIP(hipStreamSynchronize(s));
+      if (lanczos_trace) {
+        double r = 0;
+        for (int j = 0; j < std::min(hs.nscreened, nvec); j++) r = std::max(r, hs.res[j]);
+        std::fprintf(stderr, "lanczos %s m=%d ncheck=%d screened=%d fail=%d maxres=%.3e vals=[%.6e .. %.6e]\n",
+                     mixed ? "mixed" : "plain", done, ncheck, hs.nscreened, hs.fail, r, hs.vals[0],
+                     hs.vals[std::max(0, std::min(hs.nscreened, nvec) - 1)]);
+        if (lanczos_trace > 1 && hs.fail == 1) {
+          std::vector<double> wh(2 * kWStride);
+          DMSC_HIP(hipMemcpy(wh.data(), wv, sizeof(double) * 2 * kWStride, hipMemcpyDeviceToHost));
+          for (int j = 0; j < ncheck; j++) std::fprintf(stderr, "  T %.9e  That %.9e\n", wh[j], wh[kWStride + j]);
+        }
+      }
       if (hs.fail != 1 || eigCheck >= done) break;
       eigCheck += kCullumExtra;  // lanczos.cu:528-532: widen the screened window
       if (eigCheck > (mixed ? kMaxCand : kCullumMax))
-        return fail(DMSC_ERR_NOCONV, "Cullum screening kept fewer than %d values", nvec);
+        return fail(DMSC_ERR_NOCONV, "Cullum screening kept %d of %d values at m=%d with %d candidates",
+                    hs.nscreened, nvec, done, ncheck);
     }
     if (hs.fail == 2) return fail(DMSC_ERR_INTERNAL, "inverse iteration failed on the Lanczos tridiagonal");
     if (hs.fail == 0) {

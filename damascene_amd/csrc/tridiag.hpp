@@ -127,10 +127,15 @@ __host__ __device__ inline double kth(int n, const double* a, const double* b, i
 // dgttrf/dstein) for the eigenvalue lam, `iters` solves from a fixed
 // pseudo-random start.  dl, d, du, du2, x: n doubles each, piv: n bytes
 // (device callers place them in LDS).  No orthogonalisation against
-// neighbouring vectors: the callers pass values at least 1e-5 apart
-// (Cullum-Willoughby lumping), which inverse iteration separates by a factor
-// 1e-5 / (|T| eps) per solve.  The result is normalised with its
-// largest-magnitude entry positive.  Returns -1 on a non-finite iterate.
+// neighbouring vectors: a value separated from the rest of the spectrum of T
+// by g is separated by a factor g / (|T| eps) per solve (1e5 for the 1e-10
+// copies the mixed path keeps apart, far more for distinct eigenvalues).
+// The result is normalised with its largest-magnitude entry positive.
+// Returns -1 on a non-finite iterate.
+//
+// Every loop reads its inputs two rows ahead into registers, so on the
+// device the memory latency overlaps the serial recurrence instead of
+// adding to it.
 __host__ __device__ inline int invit(int n, const double* __restrict__ a, const double* __restrict__ b, double lam,
                                      double* __restrict__ v, double* __restrict__ dl, double* __restrict__ d,
                                      double* __restrict__ du, double* __restrict__ du2, double* __restrict__ x,
@@ -140,11 +145,21 @@ __host__ __device__ inline int invit(int n, const double* __restrict__ a, const 
     tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0));
   const double eps = 2.2e-16 * fmax(tnorm, 1e-300);
   // factorisation (dgttrf), carrying the running diagonal entry di and the
-  // running upper entry ui (modified by a preceding row interchange)
+  // running upper entry ui (modified by a preceding row interchange); row i
+  // uses li = b[i], dn = a[i+1] - lam, un = b[i+1] (0 past the end)
+  auto rowin = [&](int i, double& li, double& dn, double& un) {
+    li = b[i];
+    dn = a[i + 1] - lam;
+    un = i < n - 2 ? b[i + 1] : 0.0;
+  };
   double di = a[0] - lam, ui = n > 1 ? b[0] : 0.0;
-#pragma unroll 4
+  double l1 = 0, d1 = 0, u1 = 0, l2 = 0, d2 = 0, u2 = 0;
+  if (n > 1) rowin(0, l1, d1, u1);
+  if (n > 2) rowin(1, l2, d2, u2);
   for (int i = 0; i < n - 1; i++) {
-    const double li = b[i], dn = a[i + 1] - lam, un = i < n - 2 ? b[i + 1] : 0.0;
+    const double li = l1, dn = d1, un = u1;
+    l1 = l2; d1 = d2; u1 = u2;
+    if (i + 2 < n - 1) rowin(i + 2, l2, d2, u2);
     if (fabs(di) >= fabs(li)) {
       piv[i] = 0;
       if (di == 0) di = eps;
@@ -168,27 +183,39 @@ __host__ __device__ inline int invit(int n, const double* __restrict__ a, const 
   }
   d[n - 1] = di;
   piv[n - 1] = 0;
+  // reciprocal pivots and the start vector (a hash of the row index: no
+  // serial dependence between rows)
   for (int i = 0; i < n; i++) {
     double t = d[i];
     if (fabs(t) < eps) t = t < 0 ? -eps : eps;
-    d[i] = recip(t);  // reciprocal pivots
-  }
-  unsigned long long s = 0x9E3779B97F4A7C15ULL;
-  for (int i = 0; i < n; i++) {
-    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
-    x[i] = ((double)(s >> 11) / 9007199254740992.0) - 0.5;
+    d[i] = recip(t);
+    unsigned long long h = (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 27;
+    x[i] = ((double)(h >> 11) / 9007199254740992.0) - 0.5;
   }
   for (int iter = 0; iter < iters; iter++) {
     double xi = x[0];
-#pragma unroll 8
-    for (int i = 0; i < n - 1; i++) {  // forward: L with interchanges
-      const double xn = x[i + 1], f = dl[i];
-      if (piv[i]) {
-        x[i] = xn;
-        xi = xi - f * xn;
-      } else {
-        x[i] = xi;
-        xi = xn - f * xi;
+    {  // forward: L with interchanges (row i reads x[i+1], dl[i], piv[i])
+      double xa = n > 1 ? x[1] : 0, fa = dl[0], xb = n > 2 ? x[2] : 0, fb = n > 1 ? dl[1] : 0;
+      int pa = piv[0], pb = n > 1 ? piv[1] : 0;
+      for (int i = 0; i < n - 1; i++) {
+        const double xn = xa, f = fa;
+        const int pv = pa;
+        xa = xb; fa = fb; pa = pb;
+        if (i + 2 < n - 1) {
+          xb = x[i + 3];
+          fb = dl[i + 2];
+          pb = piv[i + 2];
+        }
+        if (pv) {
+          x[i] = xn;
+          xi = xi - f * xn;
+        } else {
+          x[i] = xi;
+          xi = xn - f * xi;
+        }
       }
     }
     double x1 = xi * d[n - 1], x2 = 0;  // backward: U (three bands)
@@ -199,13 +226,20 @@ __host__ __device__ inline int invit(int n, const double* __restrict__ a, const 
       x[n - 2] = x1;
     }
     double nrm = x1 * x1 + x2 * x2;
-#pragma unroll 8
-    for (int i = n - 3; i >= 0; i--) {
-      const double xi0 = (x[i] - du[i] * x1 - du2[i] * x2) * d[i];
-      x[i] = xi0;
-      x2 = x1;
-      x1 = xi0;
-      nrm += xi0 * xi0;
+    if (n > 2) {  // row i reads x[i], du[i], du2[i], d[i]
+      double ra[4], rb[4] = {0, 0, 0, 0};
+      ra[0] = x[n - 3]; ra[1] = du[n - 3]; ra[2] = du2[n - 3]; ra[3] = d[n - 3];
+      if (n > 3) { rb[0] = x[n - 4]; rb[1] = du[n - 4]; rb[2] = du2[n - 4]; rb[3] = d[n - 4]; }
+      for (int i = n - 3; i >= 0; i--) {
+        const double xv = ra[0], u = ra[1], u2 = ra[2], dv = ra[3];
+        ra[0] = rb[0]; ra[1] = rb[1]; ra[2] = rb[2]; ra[3] = rb[3];
+        if (i - 2 >= 0) { rb[0] = x[i - 2]; rb[1] = du[i - 2]; rb[2] = du2[i - 2]; rb[3] = d[i - 2]; }
+        const double xi0 = (xv - u * x1 - u2 * x2) * dv;
+        x[i] = xi0;
+        x2 = x1;
+        x1 = xi0;
+        nrm += xi0 * xi0;
+      }
     }
     nrm = sqrt(nrm);
     if (!(nrm > 0) || !isfinite(nrm)) return -1;

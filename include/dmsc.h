@@ -190,6 +190,11 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *   "spmv_rows_per_lane": 0 = one row per lane in 256-thread blocks with all
  *                         81 loads in flight (default); 1, 2, 4 = one wave
  *                         per 64*RPL rows.
+ *   "lanczos_k1_threads": block size of the eigensolver's SpMV step kernel,
+ *                         64 or 256 (default).
+ *   "lanczos_trace":      non-zero prints one line per eigensolver check to
+ *                         stderr (2: also the Ritz values of a failed
+ *                         screening).
  * Returns DMSC_ERR_ARG for an unknown key or value. */
 int dmsc_tune_set(const char* key, int value);
 int dmsc_tune_get(const char* key, int* value);

@@ -32,12 +32,30 @@ Ctx::~Ctx() {
     if (arena[i]) (void)hipFree(arena[i]);
   if (d_texton_coeffs) (void)hipFree(d_texton_coeffs);
   if (h_pinned) (void)hipHostFree(h_pinned);
+  for (hipEvent_t e : aux_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (aux) (void)hipStreamDestroy(aux);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
+// Pinned block layout: bytes [0, 256) the pipeline's k-means iteration
+// count, [1024, 4096) the eigensolver's status record.
 int Ctx::pinned(int** out) {
-  if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 256, hipHostMallocDefault));
+  if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 4096, hipHostMallocDefault));
   *out = h_pinned;
+  return DMSC_OK;
+}
+
+int Ctx::aux_stream(hipStream_t* s, hipEvent_t* ev0, hipEvent_t* ev1) {
+  if (!aux) {
+    DMSC_HIP(hipSetDevice(device));
+    DMSC_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    DMSC_HIP(hipEventCreateWithFlags(&aux_ev[0], hipEventDisableTiming));
+    DMSC_HIP(hipEventCreateWithFlags(&aux_ev[1], hipEventDisableTiming));
+  }
+  *s = aux;
+  *ev0 = aux_ev[0];
+  *ev1 = aux_ev[1];
   return DMSC_OK;
 }
 
@@ -196,8 +214,14 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "lanczos_k1_threads")) {
-    DMSC_ARG(value == 64 || value == 256, "lanczos_k1_threads must be 64 or 256");
+    DMSC_ARG(value == 0 || value == 64 || value == 256 || value == 512,
+             "lanczos_k1_threads must be 0 (automatic), 64, 256 or 512");
     lanczos_k1_threads = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_mixed_fused")) {
+    DMSC_ARG(value == 0 || value == 1, "lanczos_mixed_fused must be 0 or 1");
+    lanczos_mixed_fused = value;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -214,6 +238,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "lanczos_k1_threads")) {
     *value = lanczos_k1_threads;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_mixed_fused")) {
+    *value = lanczos_mixed_fused;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);

@@ -32,9 +32,12 @@ struct Ctx {
   void* arena[kArenas] = {};
   size_t arena_bytes[kArenas] = {};
   float* d_texton_coeffs = nullptr;
-  int* h_pinned = nullptr;  // small pinned host block for asynchronous read-backs
+  int* h_pinned = nullptr;  // pinned host block (4 KiB) for asynchronous read-backs
+  hipStream_t aux = nullptr;  // second stream (default priority) for work overlapped with `stream`
+  hipEvent_t aux_ev[2] = {};
   ~Ctx();
   int pinned(int** out);
+  int aux_stream(hipStream_t* s, hipEvent_t* ev0, hipEvent_t* ev1);
   int scratch(int slot, size_t bytes, void** ptr);
   int adopt(int slot, void* p, size_t bytes);  // replace arena `slot` by p
   int texton_coeffs(const float** out);

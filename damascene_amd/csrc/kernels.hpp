@@ -88,8 +88,12 @@ __device__ __forceinline__ void spmv_rows(float (&acc)[RPL], const float* __rest
 extern int spmv_rows_per_lane;
 // Non-zero: the eigensolver prints one line per check to stderr.
 extern int lanczos_trace;
-// Block size of the Lanczos SpMV step kernel (64 or 256).
+// Rows per block of the Lanczos SpMV step kernel: 0 = automatic (256 for the
+// two-kernel steps, 512 for the fused mixed step), or 64, 256, 512.
 extern int lanczos_k1_threads;
+// Mixed eigensolver: 1 = one fused kernel per step (LDS-staged basis
+// vector, when the stencil halo fits), 0 = two kernels per step.
+extern int lanczos_mixed_fused;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,

@@ -46,7 +46,8 @@
 namespace dmsc {
 
 int lanczos_trace = 0;
-int lanczos_k1_threads = 256;
+int lanczos_k1_threads = 0;
+int lanczos_mixed_fused = 1;
 
 namespace {
 
@@ -72,6 +73,7 @@ constexpr int kMaxGraphs = 16;
 // residual; an isolated value within kSpurMixed (relative) of an eigenvalue
 // of T-hat is spurious.
 constexpr int kMaxCand = 128;
+constexpr int kSpecSteps = 50;  // mixed path: steps enqueued past a check while it runs
 constexpr double kLumpMixed = 1e-8;
 constexpr double kSpurMixed = 1e-7;
 
@@ -220,10 +222,11 @@ struct LzmBufs {
   size_t pitch;
   int n;
   size_t vstride;
-  double* z;   // z'_k
-  double* p1;  // [nb1] q.z' partials
-  double* p2;  // [nb1] |z'|^2 partials
-  double* p3;  // [nb1] |q|^2 partials
+  double* z;   // z'_k (two-kernel step); z' of even steps (fused step)
+  double* z1;  // fused step: z' of odd steps (holds q_0 before step 0)
+  double* p1;  // [2][nb1] q.z' partials (fused step: buffer = step parity)
+  double* p2;  // [2][nb1] |z'|^2 partials
+  double* p3;  // [2][nb1] |q|^2 partials
   double* alpha;
   double* beta;
   int nb1, nb2;
@@ -292,6 +295,123 @@ __global__ __launch_bounds__(kK2) void k_lzm2(LzmBufs b, int t) {
   for (int j = 0; j < kK2Rows; j++) {
     const int i = base + j * kK2;
     if (i < b.n) qn[i] = (float)(fma(-alpha, (double)q[i], b.z[i]) * inv);
+  }
+}
+
+// Fused mixed step (one kernel per step).  Kernel k reduces the partials
+// kernel k-1 left in buffer (k-1)&1 to alpha_{k-1} and beta_{k-1} (same
+// expansion as k_lzm2), stages q_k = (z'_{k-1} - alpha q_{k-1}) / beta_{k-1}
+// (rounded to FP32, as k_lzm2 writes it) for its rows plus the stencil halo
+// into LDS, writes q_k to the basis for its own rows, and runs the FP64-
+// accumulated SpMV from LDS.  Columns outside [0, n) are staged as 0, which
+// is the reference's bounds test (fma(a, 0, acc) == acc).  z' and the
+// partials alternate between two buffers by step parity, so a block that
+// finishes early never overwrites a value another block has yet to read.
+// Before step 0 the parity-1 buffers hold z' = q_0 and partials (0, 1, 0)
+// (alpha 0, beta 1), and basis row -1 is zero.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t) {
+  extern __shared__ float sq[];
+  __shared__ double red[NT / kWave];
+  const int k = b.st->k + t;
+  float* V = b.st->V;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int row0 = lb * NT, halo = so.halo;
+  const size_t po = (size_t)((k - 1) & 1) * b.nb1, pc = (size_t)(k & 1) * b.nb1;
+  const double* zp = (k & 1) ? b.z : b.z1;  // z'_{k-1}
+  double* zc = (k & 1) ? b.z1 : b.z;        // z'_k
+  const float* qm = V + (long long)(k - 1) * (long long)b.vstride;  // q_{k-1}
+  const int c0 = row0 - halo, len = NT + 2 * halo;
+  // the first kPre staged inputs of this thread are loaded before the
+  // reduction of the partials, so the two memory round trips overlap
+  constexpr int kPre = 12;
+  double zr[kPre];
+  float qr[kPre];
+#pragma unroll
+  for (int j = 0; j < kPre; j++) {
+    const int c = c0 + (int)threadIdx.x + j * NT;
+    const bool ok = j * NT + (int)threadIdx.x < len && (unsigned)c < (unsigned)b.n;
+    zr[j] = ok ? zp[c] : 0.0;
+    qr[j] = ok ? qm[c] : 0.0f;
+  }
+  double v1 = 0, v2 = 0, v3 = 0;
+  for (int i = threadIdx.x; i < b.nb1; i += NT) {
+    v1 += b.p1[po + i];
+    v2 += b.p2[po + i];
+    v3 += b.p3[po + i];
+  }
+  const double alpha = block_sum<NT>(v1, red);
+  const double s2 = block_sum<NT>(v2, red), s3 = block_sum<NT>(v3, red);
+  const double beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
+  if (k > 0 && lb == 0 && threadIdx.x == 0) {
+    b.alpha[k - 1] = alpha;
+    b.beta[k - 1] = beta;
+  }
+  const double inv = 1.0 / beta;
+#pragma unroll
+  for (int j = 0; j < kPre; j++) {
+    const int i = (int)threadIdx.x + j * NT;
+    if (i < len) sq[i] = (unsigned)(c0 + i) < (unsigned)b.n ? (float)(fma(-alpha, (double)qr[j], zr[j]) * inv) : 0.0f;
+  }
+  for (int i = (int)threadIdx.x + kPre * NT; i < len; i += NT) {
+    const int c = c0 + i;
+    float v = 0.0f;
+    if ((unsigned)c < (unsigned)b.n) v = (float)(fma(-alpha, (double)qm[c], zp[c]) * inv);
+    sq[i] = v;
+  }
+  __syncthreads();
+  const int row = row0 + threadIdx.x;
+  double d1 = 0, d2 = 0, d3 = 0;
+  if (row < b.n) {
+    const float* x = sq + halo + threadIdx.x;
+    const float qf = x[0];
+    V[(size_t)k * b.vstride + row] = qf;
+    double acc = 0.0;  // FP64 accumulation of exact FP32 products, diagonal order
+#pragma unroll 9
+    for (int d = 0; d < kStencilArea; d++) acc = fma((double)b.A[(size_t)d * b.pitch + row], (double)x[so.off[d]], acc);
+    const double qv = (double)qf;
+    const double zj = fma(-beta, (double)qm[row], acc);
+    zc[row] = zj;
+    d1 = qv * zj;
+    d2 = zj * zj;
+    d3 = qv * qv;
+  }
+  d1 = block_sum<NT>(d1, red);
+  d2 = block_sum<NT>(d2, red);
+  d3 = block_sum<NT>(d3, red);
+  if (threadIdx.x == 0) {
+    b.p1[pc + lb] = d1;
+    b.p2[pc + lb] = d2;
+    b.p3[pc + lb] = d3;
+  }
+}
+
+// alpha and beta of the last fused step m-1, for the check.
+__global__ __launch_bounds__(kRed) void k_lzmf_finish(LzmBufs b, int m) {
+  __shared__ double red[kRed / kWave];
+  const size_t po = (size_t)((m - 1) & 1) * b.nb1;
+  double v1 = 0, v2 = 0, v3 = 0;
+  for (int i = threadIdx.x; i < b.nb1; i += kRed) {
+    v1 += b.p1[po + i];
+    v2 += b.p2[po + i];
+    v3 += b.p3[po + i];
+  }
+  const double alpha = block_sum<kRed>(v1, red);
+  const double s2 = block_sum<kRed>(v2, red), s3 = block_sum<kRed>(v3, red);
+  if (threadIdx.x == 0) {
+    b.alpha[m - 1] = alpha;
+    b.beta[m - 1] = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
+    b.st->m = m;
+  }
+}
+
+// Fused-step initial state: the parity-1 partials give alpha 0, beta 1.
+__global__ void k_lzmf_init(double* p1, double* p2, double* p3, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) {
+    p1[nb + i] = 0.0;
+    p2[nb + i] = i == 0 ? 1.0 : 0.0;
+    p3[nb + i] = 0.0;
   }
 }
 
@@ -694,10 +814,10 @@ __global__ void k_set_m(LzState* st, int m) { st->m = m; }
 struct GraphKey {
   const void* A;
   const void* st;
-  int pitch, n, w, mixed, steps, k1;
+  int pitch, n, w, mixed, steps, k1, fused;
   bool operator<(const GraphKey& o) const {
-    return std::tie(A, st, pitch, n, w, mixed, steps, k1) <
-           std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps, o.k1);
+    return std::tie(A, st, pitch, n, w, mixed, steps, k1, fused) <
+           std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps, o.k1, o.fused);
   }
 };
 
@@ -721,7 +841,8 @@ std::map<const Ctx*, std::unique_ptr<GraphCache>>& graph_caches() {
 
 struct Steps {
   bool mixed;
-  int k1;  // K1 block size
+  int k1;           // K1 block size
+  size_t fused_lds;  // mixed path: LDS bytes of the fused step kernel, 0 = two-kernel step
   LzBufs plain;
   LzmBufs mix;
   StencilOffsets so;
@@ -729,7 +850,12 @@ struct Steps {
 
 void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
   for (int t = 0; t < steps; t++) {
-    if (p.mixed) {
+    if (p.mixed && p.fused_lds) {
+      if (p.k1 == 512)
+        hipLaunchKernelGGL(k_lzmf<512>, dim3(p.mix.nb1), dim3(512), p.fused_lds, s, p.mix, p.so, t);
+      else
+        hipLaunchKernelGGL(k_lzmf<256>, dim3(p.mix.nb1), dim3(256), p.fused_lds, s, p.mix, p.so, t);
+    } else if (p.mixed) {
       if (p.k1 == 64)
         hipLaunchKernelGGL(k_lzm1<64>, dim3(p.mix.nb1), dim3(64), 0, s, p.mix, p.so, t);
       else
@@ -752,7 +878,7 @@ int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps
   if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
-                     steps, p.k1};
+                     steps, p.k1, (int)p.fused_lds};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
@@ -822,7 +948,36 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const size_t vstride = (size_t)pitch_floats(n);
-  const int k1 = lanczos_k1_threads == 64 ? 64 : 256;
+  static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
+  if (!invit_lds) {
+    int lds_max = 0;
+    DMSC_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    if ((size_t)lds_max < kBisectLdsMax + 1024)
+      return fail(DMSC_ERR_INTERNAL, "device LDS per block %d below the %zu the eigensolver needs", lds_max,
+                  kBisectLdsMax + 1024);
+    const size_t vec_lds = std::min(kInvitLds, (size_t)lds_max);
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kBisectLdsMax));
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_vec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_lzmf<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
+    DMSC_HIP(hipFuncSetAttribute((const void*)k_lzmf<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
+    invit_lds = vec_lds;
+  }
+  // Step kernel shape.  The fused mixed step stages its rows plus the
+  // stencil halo in LDS (256 or 512 rows per block) and falls back to the
+  // two-kernel step when that does not fit; the two-kernel steps take 64 or
+  // 256 rows per block.
+  const int halo = make_stencil_offsets(w).halo;
+  int k1 = lanczos_k1_threads == 64 ? 64 : 256;
+  size_t fused_lds = 0;
+  if (mixed && lanczos_mixed_fused) {
+    const int kf = (lanczos_k1_threads == 0 || lanczos_k1_threads == 512) ? 512 : 256;
+    const size_t bytes = sizeof(float) * ((size_t)kf + 2 * (size_t)halo);
+    if (bytes <= invit_lds) {
+      k1 = kf;
+      fused_lds = bytes;
+    }
+  }
   const int nb1 = div_up(n, k1), nb2 = div_up(n, kK2 * kK2Rows);
   int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
   const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
@@ -839,8 +994,8 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const size_t o_st = carve(sizeof(LzState));
   const size_t o_z0 = carve(zb * vstride), o_z1 = carve(zb * vstride);
   const size_t o_rs = carve(sizeof(float) * vstride);
-  const size_t o_p1 = carve(sizeof(double) * nb1), o_p2 = carve(sizeof(double) * nb1),
-               o_p3 = carve(sizeof(double) * nb1);
+  const size_t o_p1 = carve(sizeof(double) * 2 * nb1), o_p2 = carve(sizeof(double) * 2 * nb1),
+               o_p3 = carve(sizeof(double) * 2 * nb1);
   const size_t o_al = carve(zb * (kMaxIter + 2)), o_be = carve(zb * (kMaxIter + 2));
   const size_t o_ad = carve(sizeof(double) * (kMaxIter + 2)), o_bd = carve(sizeof(double) * (kMaxIter + 2));
   const size_t o_w = carve(sizeof(double) * 2 * kWStride);
@@ -868,62 +1023,93 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   Steps p;
   p.mixed = mixed;
   p.k1 = k1;
+  p.fused_lds = fused_lds;
   p.so = make_stencil_offsets(w);
   p.plain = LzBufs{A, (size_t)pitch, n, vstride, (float*)(B + o_z0), (float*)(B + o_z1), (double*)(B + o_p1),
                    (double*)(B + o_p2), (float*)(B + o_al), (float*)(B + o_be), nb1, nb2, st};
-  p.mix = LzmBufs{A, (size_t)pitch, n, vstride, (double*)(B + o_z0), (double*)(B + o_p1), (double*)(B + o_p2),
-                  (double*)(B + o_p3), (double*)(B + o_al), (double*)(B + o_be), nb1, nb2, st};
+  p.mix = LzmBufs{A,
+                  (size_t)pitch,
+                  n,
+                  vstride,
+                  (double*)(B + o_z0),
+                  (double*)(B + o_z1),
+                  (double*)(B + o_p1),
+                  (double*)(B + o_p2),
+                  (double*)(B + o_p3),
+                  (double*)(B + o_al),
+                  (double*)(B + o_be),
+                  nb1,
+                  nb2,
+                  st};
 
-  static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
-  if (!invit_lds) {
-    int lds_max = 0;
-    DMSC_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-    if ((size_t)lds_max < kBisectLdsMax + 1024)
-      return fail(DMSC_ERR_INTERNAL, "device LDS per block %d below the %zu the eigensolver needs", lds_max,
-                  kBisectLdsMax + 1024);
-    const size_t vec_lds = std::min(kInvitLds, (size_t)lds_max);
-    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)kBisectLdsMax));
-    DMSC_HIP(hipFuncSetAttribute((const void*)k_tri_vec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
-    invit_lds = vec_lds;
-  }
   DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
   const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
   const dim3 gi(div_up(std::max(n, nb1), 256));
   hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
-  if (mixed)
-    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.mix.z, q0, n);
-  else
+  if (mixed) {
+    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.fused_lds ? p.mix.z1 : p.mix.z, q0, n);
+    if (p.fused_lds)
+      hipLaunchKernelGGL(k_lzmf_init, dim3(div_up(nb1, 256)), dim3(256), 0, s, p.mix.p1, p.mix.p2, p.mix.p3, nb1);
+  } else {
     hipLaunchKernelGGL(k_init<float>, gi, dim3(256), 0, s, st, V, p.plain.z1, q0, n);
+  }
   DMSC_HIP(hipGetLastError());
 
   int eigCheck = nvec + kCullumExtra;
   int done = 0, prev_m = 0, prev_n = 0;  // prev_n: Ritz values of the last check left in wv
   double prev_res = 0;
-  LzState hs{};
+  LzState hs_plain{};
+  LzState* hsp = &hs_plain;  // the host copy of the status record
+  // The mixed path overlaps each check with the next Lanczos steps: the check
+  // runs on the context's second stream, after an event recorded behind the
+  // step that ends its period, while up to kSpecSteps further steps go to the
+  // main stream.  Those steps only write basis rows and coefficients beyond
+  // the check's order m, which the check never reads; if the check converges
+  // they are discarded.  The status record comes back through pinned memory.
+  hipStream_t cs = s;
+  hipEvent_t ev_step = nullptr, ev_chk = nullptr;
+  if (mixed) {
+    DMSC_TRY(ctx->aux_stream(&cs, &ev_step, &ev_chk));
+    int* pin;
+    DMSC_TRY(ctx->pinned(&pin));
+    hsp = reinterpret_cast<LzState*>(reinterpret_cast<char*>(pin) + 1024);
+  }
+  const LzState& hs = *hsp;
+  int ahead = 0;  // steps already enqueued beyond `done`
   bool converged = false;
   while (done < kMaxIter) {
-    const int steps = std::min(period, kMaxIter - done);
-    if ((size_t)done + steps + 2 > cap) {
-      // grow the basis, keeping the zero row and q_0..q_done: the mixed
-      // path's last K2 has already written q_done, which its next K1 reads
-      // (the plain path's K1 writes its own row; copying it is harmless)
-      const size_t ncap = std::min<size_t>(std::max(cap * 2, (size_t)done + steps + 2), kMaxIter + 2);
+    // the next check comes after max(period, ahead) steps
+    const int steps = std::min(std::max(period, ahead), kMaxIter - done);
+    const int spec = mixed ? kSpecSteps : 0;
+    if ((size_t)done + steps + spec + 2 > cap) {
+      // grow the basis, keeping the zero row and every row already written:
+      // q_0..q_{done+ahead} (the mixed path's K2 writes the row its next K1
+      // reads; the plain path's K1 writes its own row, so one extra row is
+      // copied harmlessly)
+      const size_t ncap =
+          std::min<size_t>(std::max(cap * 2, (size_t)done + steps + spec + 2), kMaxIter + kSpecSteps + 2);
       void* nbuf;
       DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * (ncap + 1)));
-      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * (done + 2), hipMemcpyDeviceToDevice, s));
+      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * (done + ahead + 2),
+                              hipMemcpyDeviceToDevice, s));
       DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * (ncap + 1)));  // synchronises before freeing
       V = (float*)nbuf + vstride;
       cap = ncap;
       hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
     }
-    for (int t = 0; t < steps; t += kGraphSteps)
+    for (int t = ahead; t < steps; t += kGraphSteps)
       DMSC_TRY(run_steps(ctx, p, st, w, done + t, std::min(kGraphSteps, steps - t)));
     done += steps;
+    ahead = 0;
     if (mixed) {
-      hipLaunchKernelGGL(k_set_m, dim3(1), dim3(1), 0, s, st, done);  // alpha, beta[done-1] written by K2m
+      if (p.fused_lds)
+        hipLaunchKernelGGL(k_lzmf_finish, dim3(1), dim3(kRed), 0, s, p.mix, done);
+      else
+        hipLaunchKernelGGL(k_set_m, dim3(1), dim3(1), 0, s, st, done);  // alpha, beta[done-1] written by K2m
       hipLaunchKernelGGL(k_widen<double>, dim3(div_up(done, 256)), dim3(256), 0, s, p.mix.alpha, p.mix.beta, ad, bd,
                          done);
+      DMSC_HIP(hipEventRecord(ev_step, s));
+      DMSC_HIP(hipStreamWaitEvent(cs, ev_step, 0));
     } else {
       hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kRed), 0, s, p.plain, done);
       hipLaunchKernelGGL(k_widen<float>, dim3(div_up(done, 256)), dim3(256), 0, s, p.plain.alpha, p.plain.beta, ad,
@@ -931,24 +1117,34 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     }
     for (;;) {
       const int ncheck = std::min(eigCheck, done);
-      hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, s, ad, bd, st, wv,
+      hipLaunchKernelGGL(k_tri_bisect, dim3(ncheck, 2), dim3(kBisect), sizeof(double2) * done, cs, ad, bd, st, wv,
                          prev_n);
       prev_n = ncheck;
       const bool lds = (size_t)done * kInvitBytes <= invit_lds;
       const size_t vlds = lds ? (size_t)done * kInvitBytes : 0;
       if (mixed) {  // vectors and residuals of every candidate, then screening
         double* res_all = (double*)(B + o_resall);
-        hipLaunchKernelGGL(k_tri_vec, dim3(ncheck), dim3(kWave), vlds, s, ad, bd, st, 1, S, (double*)(B + o_work),
+        hipLaunchKernelGGL(k_tri_vec, dim3(ncheck), dim3(kWave), vlds, cs, ad, bd, st, 1, S, (double*)(B + o_work),
                            lds ? 1 : 0, (const double*)wv, res_all);
-        hipLaunchKernelGGL(k_tri_screen_mixed, dim3(1), dim3(kWave), 0, s, st, wv, res_all, ncheck, nvec);
+        hipLaunchKernelGGL(k_tri_screen_mixed, dim3(1), dim3(kWave), 0, cs, st, wv, res_all, ncheck, nvec);
       } else {  // lanczos.cu:279-384 screening, then vectors of the screened values
-        hipLaunchKernelGGL(k_tri_screen, dim3(1), dim3(kWave), 0, s, st, wv, ncheck, nvec);
-        hipLaunchKernelGGL(k_tri_vec, dim3(nvec), dim3(kWave), vlds, s, ad, bd, st, 0, S, (double*)(B + o_work),
+        hipLaunchKernelGGL(k_tri_screen, dim3(1), dim3(kWave), 0, cs, st, wv, ncheck, nvec);
+        hipLaunchKernelGGL(k_tri_vec, dim3(nvec), dim3(kWave), vlds, cs, ad, bd, st, 0, S, (double*)(B + o_work),
                            lds ? 1 : 0, (const double*)nullptr, (double*)nullptr);
       }
       DMSC_HIP(hipGetLastError());
-      DMSC_HIP(hipMemcpyAsync(&hs, st, sizeof(LzState), hipMemcpyDeviceToHost, s));
-      DMSC_HIP(hipStreamSynchronize(s));
+      DMSC_HIP(hipMemcpyAsync(hsp, st, sizeof(LzState), hipMemcpyDeviceToHost, cs));
+      if (mixed) {
+        DMSC_HIP(hipEventRecord(ev_chk, cs));
+        if (ahead == 0) {  // steps past the check, overlapped with it
+          ahead = std::min(kSpecSteps, kMaxIter - done);
+          for (int t = 0; t < ahead; t += kGraphSteps)
+            DMSC_TRY(run_steps(ctx, p, st, w, done + t, std::min(kGraphSteps, ahead - t)));
+        }
+        DMSC_HIP(hipEventSynchronize(ev_chk));
+      } else {
+        DMSC_HIP(hipStreamSynchronize(s));
+      }
       if (lanczos_trace) {
         double r = 0;
         for (int j = 0; j < std::min(hs.nscreened, nvec); j++) r = std::max(r, hs.res[j]);

@@ -190,8 +190,13 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *   "spmv_rows_per_lane": 0 = one row per lane in 256-thread blocks with all
  *                         81 loads in flight (default); 1, 2, 4 = one wave
  *                         per 64*RPL rows.
- *   "lanczos_k1_threads": block size of the eigensolver's SpMV step kernel,
- *                         64 or 256 (default).
+ *   "lanczos_k1_threads": rows per block of the eigensolver's SpMV step
+ *                         kernel: 0 = automatic (default: 256 for two-kernel
+ *                         steps, 512 for the fused mixed step), 64, 256 or
+ *                         512 (512 applies to the fused step only).
+ *   "lanczos_mixed_fused": mixed eigensolver step as one kernel staging the
+ *                         basis vector in LDS (1, default, when the stencil
+ *                         halo fits) or two kernels (0).
  *   "lanczos_trace":      non-zero prints one line per eigensolver check to
  *                         stderr (2: also the Ritz values of a failed
  *                         screening).

@@ -10,14 +10,19 @@ against the FP64 ground truth in tests/golden/truth_481x321.npz (eigenvalue
 relative error, subspace angle of eigenvectors 1..8, gPb max-abs error,
 nonmax mismatches against the oracle).
 
-roofline: the stencil SpMV (81 diagonals, 332 algorithmic bytes per row),
-average launch duration measured with HIP events on the library's stream
-after a sweep of its launch shapes; `traffic` is the HBM bytes per launch
-from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed under profiles/.
+roofline: the dominant kernel of the timed region, the mixed path's fused
+Lanczos step k_lzmt<64, 4> (one launch per Krylov step: FP64-accumulated
+81-diagonal stencil SpMV from an LDS-staged basis vector plus the three-term
+update; 348 algorithmic bytes per row), its average launch duration measured
+with HIP events on the library's stream over 500 steps; `traffic` is the HBM
+bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed
+under profiles/.  `spmv` beside it: the standalone SpMV (332 B per row) over
+its launch shapes.
 cpu_baseline: the oracle's reference-order pipeline on one image on the host
 cores (OpenMP).
 """
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -33,8 +38,14 @@ W, H, CELLS, SEED, NVEC, TOL = 481, 321, 40, 20261202, 9, 1e-3
 HBM_PEAK_GBS = 8000.0
 SPMV_BYTES_PER_ROW = 81 * 4 + 4 + 4  # 81 matrix entries + x + y (algorithmic)
 SPMV_SHAPES = {0: "k_spmv_full", 1: "k_spmv<1>", 2: "k_spmv<2>", 4: "k_spmv<4>"}
+# fused mixed Lanczos step, per row: 81 matrix entries (324 B), the staged
+# z'_{k-1} (8 B) and q_{k-1} (4 B) read once, q_k (4 B) and z'_k (8 B) written
+STEP_KERNEL = "k_lzmt<64, 4>"
+STEP_BYTES_PER_ROW = 81 * 4 + 8 + 4 + 4 + 8
+STEP_TIMED = 500
 TRUTH = os.path.join(ROOT, "tests", "golden", "truth_481x321.npz")
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "stencil_spmv_481x321_pmc.txt")
+PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("lanczos_step_481x321_pmc.txt", "stencil_spmv_481x321_pmc.txt")]
 
 
 def parse():
@@ -46,18 +57,21 @@ def parse():
     p.add_argument("--spmv-reps", type=int, default=200)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the workload of the rocprofv3 counter passes: one 256 MiB device copy "
-                        "(calibration) and 20 launches of the selected SpMV shape, then exit")
-    p.add_argument("--spmv-shape", type=int, default=0, choices=sorted(SPMV_SHAPES),
+                        "(calibration), 20 launches of the selected SpMV shape and 100 mixed Lanczos steps, "
+                        "then exit")
+    p.add_argument("--spmv-shape", type=int, default=1, choices=sorted(SPMV_SHAPES),
                    help="SpMV launch shape for --pmc-workload (0 = k_spmv_full, else rows per lane)")
     p.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                    help="summarise two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE passes of "
-                        "--pmc-workload) into HBM bytes per SpMV launch, calibrated on the 256 MiB copy")
+                        "--pmc-workload) into HBM bytes per launch of --kernel, calibrated on the 256 MiB copy")
+    p.add_argument("--kernel", default=STEP_KERNEL,
+                   help="kernel for --pmc-summarize: the Lanczos step kernel or an SpMV shape name")
     return p.parse_args()
 
 
-def pmc_summarize(fetch_dir, write_dir, shape):
-    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the SpMV launches, each
-    scaled by what the same counter reads for the 256 MiB copy (expected
+def pmc_summarize(fetch_dir, write_dir, kernel):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the launches of `kernel`,
+    each scaled by what the same counter reads for the 256 MiB copy (expected
     262,144 KiB read and 262,144 KiB written)."""
     import csv
     import glob
@@ -81,26 +95,26 @@ def pmc_summarize(fetch_dir, write_dir, shape):
             raise SystemExit(f"no {counter} values under {d}")
         return vals
 
-    kname = SPMV_SHAPES[shape].split("<")[0]
+    pattern = re.escape(kernel) + r"\("  # the name as rocprofv3 prints it, up to the argument list
     out = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         v = per_kernel(d, counter)
         copies = [x for (k, _), x in v.items() if "copy" in k.lower()]
-        spmv = [x for (k, _), x in v.items() if kname in k and "spmv" in k]
-        if not copies or not spmv:
-            raise SystemExit(f"{counter}: copy kernel or {kname} not found in {d}")
+        hits = [x for (k, _), x in v.items() if re.search(pattern, k)]
+        if not copies or not hits:
+            raise SystemExit(f"{counter}: copy kernel or {kernel} not found in {d}")
         cal = max(copies)
-        out[counter] = dict(copy_kib=cal, scale=262144.0 / cal, spmv_kib=sum(spmv) / len(spmv), launches=len(spmv))
+        out[counter] = dict(copy_kib=cal, scale=262144.0 / cal, kib=sum(hits) / len(hits), launches=len(hits))
     f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
-    traffic = (f["spmv_kib"] * f["scale"] + w["spmv_kib"] * w["scale"]) * 1024
-    algo = SPMV_BYTES_PER_ROW * W * H
+    traffic = (f["kib"] * f["scale"] + w["kib"] * w["scale"]) * 1024
+    algo = (STEP_BYTES_PER_ROW if kernel == STEP_KERNEL else SPMV_BYTES_PER_ROW) * W * H
     lines = [
         f"calibration copy (256 MiB): FETCH_SIZE {f['copy_kib']:.0f} KiB (scale {f['scale']:.4f}), "
         f"WRITE_SIZE {w['copy_kib']:.0f} KiB (scale {w['scale']:.4f})",
-        f"{SPMV_SHAPES[shape]} ({f['launches']} launches): FETCH_SIZE {f['spmv_kib']:.1f} KiB, "
-        f"WRITE_SIZE {w['spmv_kib']:.1f} KiB per launch",
+        f"{kernel} ({f['launches']} launches): FETCH_SIZE {f['kib']:.1f} KiB, WRITE_SIZE {w['kib']:.1f} KiB "
+        f"per launch",
         f"algorithmic bytes per launch {algo}",
-        f"traffic_bytes_per_launch {SPMV_SHAPES[shape]} {traffic:.0f}",
+        f"traffic_bytes_per_launch {kernel} {traffic:.0f}",
         f"traffic / algorithmic = {traffic / algo:.3f}",
     ]
     print("\n".join(lines))
@@ -109,7 +123,8 @@ def pmc_summarize(fetch_dir, write_dir, shape):
 def pmc_workload(shape):
     """Counter-pass workload: a 256 MiB copy whose FETCH_SIZE/WRITE_SIZE
     calibrate the counters (256 MiB read + 256 MiB written), then 20 stencil
-    SpMVs on the 481x321 L_sym with the given launch shape."""
+    SpMVs on the 481x321 L_sym with the given launch shape, then 125 fused
+    mixed Lanczos steps (25 warm-up + 100 timed) on the same matrix."""
     import torch
 
     import damascene_amd as dm
@@ -130,7 +145,33 @@ def pmc_workload(shape):
     for _ in range(20):
         check(lib().dmsc_stencil_spmv(ctx.handle, W, H, m.data_ptr(), m.shape[1], x.data_ptr(), y.data_ptr()))
     torch.cuda.synchronize()
-    print(f"pmc workload done: copy 256 MiB, 20 x {SPMV_SHAPES[shape]}", flush=True)
+    m2 = ctx.intervene(out["nmax"])
+    ms = ctypes.c_float()
+    check(lib().dmsc_lanczos_time_steps(ctx.handle, W, H, m2.data_ptr(), m2.shape[1], dm.PREC_MIXED, 100,
+                                        ctypes.byref(ms)))
+    torch.cuda.synchronize()
+    print(f"pmc workload done: copy 256 MiB, 20 x {SPMV_SHAPES[shape]}, 125 mixed Lanczos steps", flush=True)
+
+
+def step_roofline(ctx, nmax):
+    """Average duration of the fused mixed Lanczos step (HIP events on the
+    library's stream over STEP_TIMED steps, no checks) and its roofline."""
+    from damascene_amd._lib import check, lib
+    import damascene_amd as dm
+
+    m = ctx.intervene(nmax)
+    ms = ctypes.c_float()
+    check(lib().dmsc_lanczos_time_steps(ctx.handle, W, H, m.data_ptr(), m.shape[1], dm.PREC_MIXED, STEP_TIMED,
+                                        ctypes.byref(ms)))
+    avg_s = ms.value / 1e3
+    algo = STEP_BYTES_PER_ROW * W * H
+    achieved = algo / avg_s / 1e9
+    return {
+        "kernel": STEP_KERNEL + " (fused mixed Lanczos step: FP64-accumulated 81-diagonal SpMV + update)",
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(STEP_KERNEL),
+        "avg_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": algo, "launches_timed": STEP_TIMED,
+    }
 
 
 def time_pipeline(ctx, d_rgb, precision, steps, warmup):
@@ -190,12 +231,14 @@ def spmv_sweep(ctx, nmax, reps):
 
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
-    if not os.path.exists(PMC_SUMMARY):
-        return None
-    text = open(PMC_SUMMARY).read()
-    m = re.search(r"^traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+(\d+)", text, re.M)
-    return int(m.group(1)) if m else None
+    summaries (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
+    for path in PMC_SUMMARIES:
+        if not os.path.exists(path):
+            continue
+        m = re.search(r"^traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+(\d+)", open(path).read(), re.M)
+        if m:
+            return int(m.group(1))
+    return None
 
 
 def errors_vs_truth(out, truth):
@@ -243,7 +286,7 @@ def main():
         pmc_workload(a.spmv_shape)
         return
     if a.pmc_summarize:
-        pmc_summarize(*a.pmc_summarize, a.spmv_shape)
+        pmc_summarize(*a.pmc_summarize, a.kernel)
         return
     import torch
 
@@ -256,7 +299,8 @@ def main():
     t_mixed, out_mixed = time_pipeline(ctx, d_rgb, dm.PREC_MIXED, a.steps, a.warmup)
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
-    roof = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
+    spmv = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
+    roof = step_roofline(ctx, out_plain["nmax"])
     torch.cuda.synchronize()
     res = {
         "metric": "gPb images/s @481x321 (mixed vs plain FP32, 1 GPU); SpMV GB/s; err vs FP64",
@@ -282,6 +326,7 @@ def main():
         "speedup_mixed_vs_plain": round(t_plain / t_mixed, 3),
         "stage_names": ["textons", "localcues", "combine+nonmax", "intervene", "eigensolve", "spectralPb+gPb"],
         "roofline": roof,
+        "spmv": spmv,
     }
     if os.path.exists(TRUTH):
         truth = np.load(TRUTH)

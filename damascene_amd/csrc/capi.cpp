@@ -28,6 +28,7 @@ int fail(int code, const char* fmt, ...) {
 
 Ctx::~Ctx() {
   release_eigensolver(this);
+  release_local_cues(this);
   for (int i = 0; i < kArenas; i++)
     if (arena[i]) (void)hipFree(arena[i]);
   if (d_texton_coeffs) (void)hipFree(d_texton_coeffs);
@@ -220,8 +221,14 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "lanczos_mixed_fused")) {
-    DMSC_ARG(value == 0 || value == 1, "lanczos_mixed_fused must be 0 or 1");
+    DMSC_ARG(value >= 0 && value <= 2, "lanczos_mixed_fused must be 0, 1 or 2");
     lanczos_mixed_fused = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_tile")) {
+    DMSC_ARG(value == 6404 || value == 6408 || value == 6402 || value == 3208 || value == 3204,
+             "lanczos_tile must be 6404, 6408, 6402, 3208 or 3204 (tile width * 100 + height)");
+    lanczos_tile = value;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -242,6 +249,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "lanczos_mixed_fused")) {
     *value = lanczos_mixed_fused;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_tile")) {
+    *value = lanczos_tile;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -462,6 +473,15 @@ int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* M, int pitch
   DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
   DMSC_ARG((pitch % 4) == 0 && ((uintptr_t)M % 16) == 0, "matrix must be 16-byte aligned with pitch % 4 == 0");
   return run_eigensolve(ctx, w, h, M, pitch, nvec, tol, precision, eigvals, eigvecs, iterations);
+}
+
+int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* M, int pitch, int precision, int steps,
+                            float* ms_per_step) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(M && ms_per_step && pitch >= w * h, "bad arguments");
+  DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
+  return time_lanczos_steps(ctx, w, h, M, pitch, precision, steps, ms_per_step);
 }
 
 int dmsc_spectral_pb(dmsc_ctx* ctx, const float* eigvals, const float* eigvecs, int w, int h, int nvec, float* spb,

@@ -43,8 +43,10 @@ int launch_calc_gpb(hipStream_t s, int n, int pitch, int norient, const float* c
 int launch_intervene(hipStream_t s, int w, int h, const float* mpb, float* M, int pitch, float sigma);
 
 struct StencilOffsets {
-  int off[kStencilArea];  // cumulative linear offset of each diagonal
-  int halo;               // 5*w + 5
+  int off[kStencilArea];          // cumulative linear offset of each diagonal
+  signed char dy[kStencilArea];   // the diagonal's image-row offset (off = dy * w + dx)
+  signed char dx[kStencilArea];   // and column offset
+  int halo;                       // 5*w + 5
 };
 StencilOffsets make_stencil_offsets(int w);
 int launch_convert_matrix(hipStream_t s, int w, int h, float* A, int pitch, float* dsum, float* rs);
@@ -91,14 +93,20 @@ extern int lanczos_trace;
 // Rows per block of the Lanczos SpMV step kernel: 0 = automatic (256 for the
 // two-kernel steps, 512 for the fused mixed step), or 64, 256, 512.
 extern int lanczos_k1_threads;
-// Mixed eigensolver: 1 = one fused kernel per step (LDS-staged basis
-// vector, when the stencil halo fits), 0 = two kernels per step.
+// Mixed eigensolver step: 2 = one fused kernel per step over pixel tiles
+// (default), 1 = one fused kernel over runs of consecutive rows (when
+// the stencil halo fits in LDS), 0 = two kernels per step.
 extern int lanczos_mixed_fused;
+// Tile of the tiled fused mixed step, TW * 100 + TH: 6404 (default), 6408,
+// 6402, 3208 or 3204.
+extern int lanczos_tile;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,
                    float* h_eigvals, float* d_eigvecs, int* iterations);
 void release_eigensolver(const Ctx* ctx);  // drops the context's captured graphs
+void release_local_cues(const Ctx* ctx);   // drops the context's cached local-cue plans
+int time_lanczos_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, float* ms_per_step);
 
 // ------------------------------------------------------------- post-processing
 int run_postprocess(Ctx* ctx, int w, int h, float* gpb, const float* nmax, float* gpb_thin, int* iterations);

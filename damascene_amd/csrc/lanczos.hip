@@ -16,24 +16,28 @@
 //    (multiples of 25 steps) and stopped when every residual
 //    |beta_m s_{m,j}| is at most tol * 1e-3.
 //
-// One step is two kernels and no host round trip:
+// A plain step is two kernels and no host round trip:
 //   K1: beta_{k-1} from the |z|^2 partials (every block reduces them in the
 //       same order, so all blocks agree bit for bit), q_k = z_{k-1}/beta_{k-1}
 //       formed on the fly for the stencil gather and written to the basis,
 //       z_k = A q_k - beta_{k-1} q_{k-1}, partial sums of q_k . z_k;
-//   K2: alpha_k from the partials, z_k -= alpha_k q_k, partial |z_k|^2;
-//       block 0 advances the step counter in device memory (K2 reads the
-//       step index from the copy K1 left, so no block races the update).
-// Because the step index lives on the device, a whole check period is
-// captured once into a HIP graph and replayed.  At a check point one block
-// per wanted value finds the FP64 Ritz values of T and of T without its first
+//   K2: alpha_k from the partials, z_k -= alpha_k q_k, partial |z_k|^2.
+// A mixed step is one kernel by default (k_lzmt: pixel tiles whose basis
+// values and stencil halo are staged in LDS; see k_lzmf and k_lzmt), with
+// the two-kernel form k_lzm1/k_lzm2 kept as a tuning alternative.
+// Each node of a captured chunk carries its step offset and the host sets
+// the chunk's first step in device memory, so a whole chunk is captured once
+// into a HIP graph and replayed.  At a check point one block per wanted
+// value finds the FP64 Ritz values of T and of T without its first
 // row/column by 256-way multisection over division-free Sturm counts, one
 // block screens them, one block per Ritz pair runs inverse iteration with its
-// arrays in LDS, and the host reads back a small status record.
+// arrays in LDS, and the host reads back a small status record.  Mixed
+// checks run on a second stream while further steps are already enqueued.
 #include <algorithm>
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <tuple>
 #include <vector>
@@ -47,7 +51,8 @@ namespace dmsc {
 
 int lanczos_trace = 0;
 int lanczos_k1_threads = 0;
-int lanczos_mixed_fused = 1;
+int lanczos_mixed_fused = 2;
+int lanczos_tile = 6404;
 
 namespace {
 
@@ -369,6 +374,95 @@ __global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t
     double acc = 0.0;  // FP64 accumulation of exact FP32 products, diagonal order
 #pragma unroll 9
     for (int d = 0; d < kStencilArea; d++) acc = fma((double)b.A[(size_t)d * b.pitch + row], (double)x[so.off[d]], acc);
+    const double qv = (double)qf;
+    const double zj = fma(-beta, (double)qm[row], acc);
+    zc[row] = zj;
+    d1 = qv * zj;
+    d2 = zj * zj;
+    d3 = qv * qv;
+  }
+  d1 = block_sum<NT>(d1, red);
+  d2 = block_sum<NT>(d2, red);
+  d3 = block_sum<NT>(d3, red);
+  if (threadIdx.x == 0) {
+    b.p1[pc + lb] = d1;
+    b.p2[pc + lb] = d2;
+    b.p3[pc + lb] = d3;
+  }
+}
+
+// Tiled fused mixed step: as k_lzmf, but each block owns a TW x TH tile of
+// image pixels instead of a run of consecutive rows, so the staged basis
+// values (tile plus a 5-pixel frame, (TW + 10) x (TH + 10)) are 4x the rows
+// for 64 x 4 tiles instead of 10x for 512 consecutive rows.  A staged slot
+// (sy, sx) holds the basis value at linear index (ty0 - 5 + sy) * w +
+// (tx0 - 5 + sx) -- the reference's linear indexing, including its wrap
+// across image rows at the left and right edges -- or 0 outside [0, n), so
+// every product is the one k_lzmf forms.  Blocks are numbered XCD-aware
+// over the row-major tile order.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, int w, int h, int t) {
+  constexpr int NT = TW * TH, SW = TW + 2 * kStencilRadius, SH = TH + 2 * kStencilRadius;
+  __shared__ float sq[SW * SH];
+  __shared__ double red[NT / kWave];
+  const int k = b.st->k + t;
+  float* V = b.st->V;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int tiles_x = (w + TW - 1) / TW;
+  const int tx0 = (lb % tiles_x) * TW, ty0 = (lb / tiles_x) * TH;
+  const size_t po = (size_t)((k - 1) & 1) * b.nb1, pc = (size_t)(k & 1) * b.nb1;
+  const double* zp = (k & 1) ? b.z : b.z1;  // z'_{k-1}
+  double* zc = (k & 1) ? b.z1 : b.z;        // z'_k
+  const float* qm = V + (long long)(k - 1) * (long long)b.vstride;  // q_{k-1}
+  const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+  const int x = tx0 + lx, y = ty0 + ly;
+  const bool own = x < w && y < h;
+  const int row = own ? y * w + x : 0;
+  constexpr int kPre = (SW * SH + NT - 1) / NT;
+  double zr[kPre];
+  float qr[kPre];
+#pragma unroll
+  for (int j = 0; j < kPre; j++) {  // staged inputs first: their latency overlaps the reduction
+    const int i = (int)threadIdx.x + j * NT;
+    const int sy = i / SW, sx = i - sy * SW;
+    const int c = (ty0 - kStencilRadius + sy) * w + (tx0 - kStencilRadius + sx);
+    const bool ok = i < SW * SH && (unsigned)c < (unsigned)b.n;
+    zr[j] = ok ? zp[c] : 0.0;
+    qr[j] = ok ? qm[c] : 0.0f;
+  }
+  double v1 = 0, v2 = 0, v3 = 0;
+  for (int i = threadIdx.x; i < b.nb1; i += NT) {
+    v1 += b.p1[po + i];
+    v2 += b.p2[po + i];
+    v3 += b.p3[po + i];
+  }
+  const double alpha = block_sum<NT>(v1, red);
+  const double s2 = block_sum<NT>(v2, red), s3 = block_sum<NT>(v3, red);
+  const double beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
+  if (k > 0 && lb == 0 && threadIdx.x == 0) {
+    b.alpha[k - 1] = alpha;
+    b.beta[k - 1] = beta;
+  }
+  const double inv = 1.0 / beta;
+#pragma unroll
+  for (int j = 0; j < kPre; j++) {
+    const int i = (int)threadIdx.x + j * NT;
+    if (i < SW * SH) {
+      const int sy = i / SW, sx = i - sy * SW;
+      const int c = (ty0 - kStencilRadius + sy) * w + (tx0 - kStencilRadius + sx);
+      sq[i] = (unsigned)c < (unsigned)b.n ? (float)(fma(-alpha, (double)qr[j], zr[j]) * inv) : 0.0f;
+    }
+  }
+  __syncthreads();
+  double d1 = 0, d2 = 0, d3 = 0;
+  if (own) {
+    const float* xs = sq + (ly + kStencilRadius) * SW + (lx + kStencilRadius);
+    const float qf = xs[0];
+    V[(size_t)k * b.vstride + row] = qf;
+    double acc = 0.0;  // FP64 accumulation of exact FP32 products, diagonal order
+#pragma unroll 9
+    for (int d = 0; d < kStencilArea; d++)
+      acc = fma((double)b.A[(size_t)d * b.pitch + row], (double)xs[so.dy[d] * SW + so.dx[d]], acc);
     const double qv = (double)qf;
     const double zj = fma(-beta, (double)qm[row], acc);
     zc[row] = zj;
@@ -834,15 +928,30 @@ struct GraphCache {
   }
 };
 
+// One cache per context; the map is shared by all contexts (and host
+// threads), so it is only touched under graph_lock().  Each cache is then
+// used by its own context's thread alone.
 std::map<const Ctx*, std::unique_ptr<GraphCache>>& graph_caches() {
   static std::map<const Ctx*, std::unique_ptr<GraphCache>> g;
   return g;
 }
+std::mutex& graph_lock() {
+  static std::mutex m;
+  return m;
+}
+GraphCache* graph_cache(const Ctx* ctx) {
+  std::lock_guard<std::mutex> lk(graph_lock());
+  auto& gc = graph_caches()[ctx];
+  if (!gc) gc.reset(new GraphCache());
+  return gc.get();
+}
 
 struct Steps {
   bool mixed;
-  int k1;           // K1 block size
-  size_t fused_lds;  // mixed path: LDS bytes of the fused step kernel, 0 = two-kernel step
+  int k1;            // K1 block size
+  size_t fused_lds;  // mixed path: LDS bytes of the linear fused step kernel, 0 = not used
+  int tile_w, tile_h;  // mixed path: tile of the tiled fused step, 0 = not used
+  int w, h;
   LzBufs plain;
   LzmBufs mix;
   StencilOffsets so;
@@ -850,7 +959,14 @@ struct Steps {
 
 void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
   for (int t = 0; t < steps; t++) {
-    if (p.mixed && p.fused_lds) {
+    if (p.mixed && p.tile_h) {
+      const int code = p.tile_w * 100 + p.tile_h;
+#define TILE_CASE(TW, TH)                                                                                 \
+  if (code == TW * 100 + TH)                                                                              \
+    hipLaunchKernelGGL((k_lzmt<TW, TH>), dim3(p.mix.nb1), dim3(TW * TH), 0, s, p.mix, p.so, p.w, p.h, t);
+      TILE_CASE(64, 4) TILE_CASE(64, 8) TILE_CASE(64, 2) TILE_CASE(32, 8) TILE_CASE(32, 4)
+#undef TILE_CASE
+    } else if (p.mixed && p.fused_lds) {
       if (p.k1 == 512)
         hipLaunchKernelGGL(k_lzmf<512>, dim3(p.mix.nb1), dim3(512), p.fused_lds, s, p.mix, p.so, t);
       else
@@ -873,12 +989,11 @@ void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
 
 // Steps first .. first+steps-1 on ctx->stream through a cached HIP graph.
 int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps) {
-  auto& gc = graph_caches()[ctx];
-  if (!gc) gc.reset(new GraphCache());
+  GraphCache* gc = graph_cache(ctx);
   if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
-                     steps, p.k1, (int)p.fused_lds};
+                     steps, p.k1, (int)p.fused_lds + 1000000 * (p.tile_w * 100 + p.tile_h)};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
@@ -943,12 +1058,19 @@ int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, dou
   return (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
 }
 
+// time_steps > 0: run that many steps with no checks between two HIP
+// events on the context's stream and return the average ms per step in
+// *ms_per_step (the measurement mode of dmsc_lanczos_time_steps).
 int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool mixed, float* h_eigvals,
-          float* d_eigvecs, int* iterations) {
+          float* d_eigvecs, int* iterations, int time_steps = 0, float* ms_per_step = nullptr) {
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const size_t vstride = (size_t)pitch_floats(n);
-  static size_t invit_lds = 0;  // dynamic LDS above the 64 KiB default, within the device's limit
+  // dynamic LDS above the 64 KiB default, within the device's limit; set
+  // once per process (a kernel attribute), under a lock for host threads
+  static size_t invit_lds = 0;
+  static std::mutex attr_lock;
+  std::unique_lock<std::mutex> attr_lk(attr_lock);
   if (!invit_lds) {
     int lds_max = 0;
     DMSC_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
@@ -963,6 +1085,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     DMSC_HIP(hipFuncSetAttribute((const void*)k_lzmf<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)vec_lds));
     invit_lds = vec_lds;
   }
+  attr_lk.unlock();
   // Step kernel shape.  The fused mixed step stages its rows plus the
   // stencil halo in LDS (256 or 512 rows per block) and falls back to the
   // two-kernel step when that does not fit; the two-kernel steps take 64 or
@@ -970,7 +1093,12 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const int halo = make_stencil_offsets(w).halo;
   int k1 = lanczos_k1_threads == 64 ? 64 : 256;
   size_t fused_lds = 0;
-  if (mixed && lanczos_mixed_fused) {
+  int tile_w = 0, tile_h = 0;
+  if (mixed && lanczos_mixed_fused == 2) {  // pixel tiles, TW x TH
+    tile_w = lanczos_tile / 100;
+    tile_h = lanczos_tile % 100;
+    k1 = tile_w * tile_h;
+  } else if (mixed && lanczos_mixed_fused == 1) {
     const int kf = (lanczos_k1_threads == 0 || lanczos_k1_threads == 512) ? 512 : 256;
     const size_t bytes = sizeof(float) * ((size_t)kf + 2 * (size_t)halo);
     if (bytes <= invit_lds) {
@@ -978,7 +1106,8 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       fused_lds = bytes;
     }
   }
-  const int nb1 = div_up(n, k1), nb2 = div_up(n, kK2 * kK2Rows);
+  const int nb1 = tile_h ? div_up(w, tile_w) * div_up(h, tile_h) : div_up(n, k1);
+  const int nb2 = div_up(n, kK2 * kK2Rows);
   int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
   const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
   const int nblk = std::min(div_up(n, 256), 1024);
@@ -1024,6 +1153,10 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   p.mixed = mixed;
   p.k1 = k1;
   p.fused_lds = fused_lds;
+  p.tile_w = tile_w;
+  p.tile_h = tile_h;
+  p.w = w;
+  p.h = h;
   p.so = make_stencil_offsets(w);
   p.plain = LzBufs{A, (size_t)pitch, n, vstride, (float*)(B + o_z0), (float*)(B + o_z1), (double*)(B + o_p1),
                    (double*)(B + o_p2), (float*)(B + o_al), (float*)(B + o_be), nb1, nb2, st};
@@ -1047,13 +1180,44 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const dim3 gi(div_up(std::max(n, nb1), 256));
   hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
   if (mixed) {
-    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, p.fused_lds ? p.mix.z1 : p.mix.z, q0, n);
-    if (p.fused_lds)
+    const bool fused = p.fused_lds || p.tile_h;
+    hipLaunchKernelGGL(k_init<double>, gi, dim3(256), 0, s, st, V, fused ? p.mix.z1 : p.mix.z, q0, n);
+    if (fused)
       hipLaunchKernelGGL(k_lzmf_init, dim3(div_up(nb1, 256)), dim3(256), 0, s, p.mix.p1, p.mix.p2, p.mix.p3, nb1);
   } else {
     hipLaunchKernelGGL(k_init<float>, gi, dim3(256), 0, s, st, V, p.plain.z1, q0, n);
   }
   DMSC_HIP(hipGetLastError());
+
+  if (time_steps > 0) {
+    const int steps = std::min(time_steps, kMaxIter - kGraphSteps);
+    if ((size_t)steps + kGraphSteps + 2 > cap) {
+      void* nbuf;
+      const size_t ncap = (size_t)steps + kGraphSteps + 2;
+      DMSC_HIP(hipMalloc(&nbuf, vstride * sizeof(float) * (ncap + 1)));
+      DMSC_HIP(hipMemcpyAsync(nbuf, V - vstride, vstride * sizeof(float) * 2, hipMemcpyDeviceToDevice, s));
+      DMSC_TRY(ctx->adopt(3, nbuf, vstride * sizeof(float) * (ncap + 1)));
+      V = (float*)nbuf + vstride;
+      cap = ncap;
+      hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
+    }
+    // steps 0..24 capture the graph and warm up; steps 25.. are timed
+    DMSC_TRY(run_steps(ctx, p, st, w, 0, kGraphSteps));
+    hipEvent_t e0, e1;
+    DMSC_HIP(hipEventCreate(&e0));
+    DMSC_HIP(hipEventCreate(&e1));
+    DMSC_HIP(hipEventRecord(e0, s));
+    for (int t = 0; t < steps; t += kGraphSteps)
+      DMSC_TRY(run_steps(ctx, p, st, w, kGraphSteps + t, std::min(kGraphSteps, steps - t)));
+    DMSC_HIP(hipEventRecord(e1, s));
+    DMSC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DMSC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_step = ms / (float)steps;
+    return DMSC_OK;
+  }
 
   int eigCheck = nvec + kCullumExtra;
   int done = 0, prev_m = 0, prev_n = 0;  // prev_n: Ritz values of the last check left in wv
@@ -1102,7 +1266,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     done += steps;
     ahead = 0;
     if (mixed) {
-      if (p.fused_lds)
+      if (p.fused_lds || p.tile_h)
         hipLaunchKernelGGL(k_lzmf_finish, dim3(1), dim3(kRed), 0, s, p.mix, done);
       else
         hipLaunchKernelGGL(k_set_m, dim3(1), dim3(1), 0, s, st, done);  // alpha, beta[done-1] written by K2m
@@ -1202,6 +1366,21 @@ int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float 
   return solve(ctx, w, h, A, pitch, nvec, tol, precision == DMSC_PREC_MIXED, h_eigvals, d_eigvecs, iterations);
 }
 
-void release_eigensolver(const Ctx* ctx) { graph_caches().erase(ctx); }
+int time_lanczos_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, float* ms_per_step) {
+  if (pitch % 4) return fail(DMSC_ERR_ARG, "matrix pitch must be a multiple of 4 floats");
+  if (steps < kGraphSteps || steps % kGraphSteps)
+    return fail(DMSC_ERR_ARG, "steps must be a positive multiple of %d", kGraphSteps);
+  return solve(ctx, w, h, A, pitch, 9, 1e-3f, precision == DMSC_PREC_MIXED, nullptr, nullptr, nullptr, steps,
+               ms_per_step);
+}
+
+void release_eigensolver(const Ctx* ctx) {
+  std::unique_ptr<GraphCache> gone;  // destroyed outside the lock
+  std::lock_guard<std::mutex> lk(graph_lock());
+  auto it = graph_caches().find(ctx);
+  if (it == graph_caches().end()) return;
+  gone = std::move(it->second);
+  graph_caches().erase(it);
+}
 
 }  // namespace dmsc

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 
 #include "dmsc_device.hpp"
@@ -510,19 +511,41 @@ struct PlanCache {
 
 }  // namespace
 
-// Cached per (w, h, texture bin pitch) on the context.
+// Plans are cached per (w, h, texture bin pitch) and context; the map is
+// shared by all contexts (and host threads), so it is only touched under
+// plan_lock(), and a context's plans are dropped when it is destroyed.
+static std::map<const Ctx*, PlanCache>& plan_caches() {
+  static std::map<const Ctx*, PlanCache> caches;
+  return caches;
+}
+static std::mutex& plan_lock() {
+  static std::mutex m;
+  return m;
+}
+
 static int get_plan(Ctx* ctx, int w, int h, int tbp, CuePlan** out) {
-  static std::map<Ctx*, PlanCache> caches;  // keyed by context; cleared by ctx teardown
-  PlanCache& pc = caches[ctx];
-  auto key = std::make_tuple(w, h, tbp);
-  auto it = pc.plans.find(key);
-  if (it == pc.plans.end()) {
-    std::unique_ptr<CuePlan> p(new CuePlan());
-    DMSC_TRY(build_plan(ctx->stream, w, h, tbp, *p));
-    it = pc.plans.emplace(key, std::move(p)).first;
+  const auto key = std::make_tuple(w, h, tbp);
+  {
+    std::lock_guard<std::mutex> lk(plan_lock());
+    PlanCache& pc = plan_caches()[ctx];
+    auto it = pc.plans.find(key);
+    if (it != pc.plans.end()) {
+      *out = it->second.get();
+      return DMSC_OK;
+    }
   }
+  // built outside the lock: only this context's thread builds its plans
+  std::unique_ptr<CuePlan> p(new CuePlan());
+  DMSC_TRY(build_plan(ctx->stream, w, h, tbp, *p));
+  std::lock_guard<std::mutex> lk(plan_lock());
+  auto it = plan_caches()[ctx].plans.emplace(key, std::move(p)).first;
   *out = it->second.get();
   return DMSC_OK;
+}
+
+void release_local_cues(const Ctx* ctx) {
+  std::lock_guard<std::mutex> lk(plan_lock());
+  plan_caches().erase(ctx);
 }
 
 size_t local_cues_scratch_bytes(int w, int h) {

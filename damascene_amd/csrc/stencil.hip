@@ -73,6 +73,14 @@ StencilOffsets make_stencil_offsets(int w) {
     acc += delta[d];
     so.off[d] = acc;
   }
+  // (dy, dx) of each diagonal, in the enumeration order of stencil_geometry
+  for (int dy = -kStencilRadius; dy <= kStencilRadius; dy++)
+    for (int dx = -kStencilRadius; dx <= kStencilRadius; dx++) {
+      const int d = diag[(dy + kStencilRadius) * (2 * kStencilRadius + 1) + dx + kStencilRadius];
+      if (d < 0) continue;
+      so.dy[d] = (signed char)dy;
+      so.dx[d] = (signed char)dx;
+    }
   so.halo = kStencilRadius * w + kStencilRadius;
   return so;
 }

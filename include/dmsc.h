@@ -195,12 +195,23 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *                         steps, 512 for the fused mixed step), 64, 256 or
  *                         512 (512 applies to the fused step only).
  *   "lanczos_mixed_fused": mixed eigensolver step as one kernel staging the
- *                         basis vector in LDS (1, default, when the stencil
- *                         halo fits) or two kernels (0).
+ *                         basis vector in LDS, over pixel tiles (2, default)
+ *                         or runs of consecutive rows (1, when the stencil
+ *                         halo fits), or as two kernels (0).
+ *   "lanczos_tile":       tile of the tiled fused step, width * 100 + height:
+ *                         6404 (default), 6408, 6402, 3208, 3204.
  *   "lanczos_trace":      non-zero prints one line per eigensolver check to
  *                         stderr (2: also the Ritz values of a failed
  *                         screening).
  * Returns DMSC_ERR_ARG for an unknown key or value. */
+/* Measurement (no reference counterpart; it backs the roofline figure of
+ * the benchmark): converts d_matrix in place as the eigensolver does, runs
+ * `steps` (a multiple of 25) Lanczos steps of the given precision with no
+ * convergence checks between two HIP events on the context's stream, after
+ * 25 untimed steps, and returns the average milliseconds per step.  The step
+ * kernels are the ones dmsc_generalized_eigensolve uses. */
+int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch, int precision,
+                            int steps, float* ms_per_step);
 int dmsc_tune_set(const char* key, int value);
 int dmsc_tune_get(const char* key, int* value);
 

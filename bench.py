@@ -1,6 +1,6 @@
 """gPb throughput on one MI355X at 481x321 (synthetic Voronoi/grating image).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 
 A step is one full gPb pass (RGB8 resident in HBM -> oriented gPb) over one
 481x321 image.  `value` is images/s of the mixed-precision path (FP64-
@@ -18,6 +18,12 @@ with HIP events on the library's stream over 500 steps; `traffic` is the HBM
 bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed
 under profiles/.  `spmv` beside it: the standalone SpMV (332 B per row) over
 its launch shapes.
+
+batched: dmsc_gpb_batch over 16 images of 481x321 (seeds 20261202 + i) and
+over 64 images of uneven size (sides uniform in [200, 1024]), 4 concurrent
+lanes, both precisions; image 0 of the 16 is the single-image workload and
+is checked bit for bit against the single-image call.
+
 cpu_baseline: the oracle's reference-order pipeline on one image on the host
 cores (OpenMP).
 """
@@ -36,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 W, H, CELLS, SEED, NVEC, TOL = 481, 321, 40, 20261202, 9, 1e-3
 HBM_PEAK_GBS = 8000.0
+BATCH, BATCH_LANES, UNEVEN = 16, 4, 64  # 481x321 batch, concurrent lanes, uneven-size batch
 SPMV_BYTES_PER_ROW = 81 * 4 + 4 + 4  # 81 matrix entries + x + y (algorithmic)
 SPMV_SHAPES = {0: "k_spmv_full", 1: "k_spmv<1>", 2: "k_spmv<2>", 4: "k_spmv<4>"}
 # fused mixed Lanczos step, per row: 81 matrix entries (324 B), the staged
@@ -54,6 +61,7 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched measurements")
     p.add_argument("--spmv-reps", type=int, default=200)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the workload of the rocprofv3 counter passes: one 256 MiB device copy "
@@ -187,6 +195,64 @@ def time_pipeline(ctx, d_rgb, precision, steps, warmup):
     return (time.perf_counter() - t0) / steps, out
 
 
+def batch_images(ctx):
+    """The 16-image batch at 481x321 (seeds SEED + i; image 0 is the
+    single-image workload) and the uneven 64-image batch: width and height
+    each uniform in [200, 1024] (sizes drawn with seed SEED), cells = area /
+    4000, image seeds SEED + 100 + i."""
+    import damascene_amd as dm
+
+    same = [ctx.upload(dm.generate_image(W, H, CELLS, SEED + i)) for i in range(BATCH)]
+    rng = np.random.default_rng(SEED)
+    sizes = [(int(rng.integers(200, 1025)), int(rng.integers(200, 1025))) for _ in range(UNEVEN)]
+    uneven = [ctx.upload(dm.generate_image(w, h, max(4, w * h // 4000), SEED + 100 + i))
+              for i, (w, h) in enumerate(sizes)]
+    return same, uneven, sizes
+
+
+def time_batch(ctx, rgbs, precision, steps, warmup):
+    """Seconds per batch call (dmsc_gpb_batch, BATCH_LANES lanes) and the
+    outputs of the last call; two or more warm-up calls size every lane's
+    arenas (a lane may take other images on the second call)."""
+    import torch
+
+    bufs = ctx.gpb_batch_buffers(rgbs)
+    for _ in range(max(2, warmup)):
+        ctx.gpb_batch(rgbs, precision=precision, lanes=BATCH_LANES, buffers=bufs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ctx.gpb_batch(rgbs, precision=precision, lanes=BATCH_LANES, buffers=bufs)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def same_bits(a, b):
+    fields = ("textons", "nmax", "eigvecs", "gpb_all", "gpb")
+    return all(np.array_equal(a[f].cpu().numpy().view(np.uint32), b[f].cpu().numpy().view(np.uint32))
+               for f in fields) and np.array_equal(a["eigvals"].view(np.uint32), b["eigvals"].view(np.uint32))
+
+
+def batched(ctx, out_plain, out_mixed, steps, warmup):
+    same, uneven, sizes = batch_images(ctx)
+    res = {"lanes": BATCH_LANES, "images": BATCH, "size": f"{W}x{H}",
+           "what": "dmsc_gpb_batch: images spread over concurrent lanes (child contexts with their own "
+                   "streams and host threads), largest first; per-image results bit-identical to single calls"}
+    for name, prec, single in (("plain_fp32", 0, out_plain), ("mixed", 1, out_mixed)):
+        t, out = time_batch(ctx, same, prec, steps, warmup)
+        res[name] = {"images_per_s": round(BATCH / t, 3), "ms_per_batch": round(t * 1e3, 3),
+                     "lanczos_iterations": [o["lanczos_iterations"] for o in out],
+                     "image0_bit_identical_to_single": bool(same_bits(out[0], single))}
+    tu = {}
+    for name, prec in (("plain_fp32", 0), ("mixed", 1)):
+        t, out = time_batch(ctx, uneven, prec, 1, 2)
+        tu[name] = {"images_per_s": round(UNEVEN / t, 3), "s_per_batch": round(t, 3),
+                    "max_lanczos_iterations": max(o["lanczos_iterations"] for o in out)}
+    res["uneven"] = {"images": UNEVEN, "sides": "uniform in [200, 1024]", "megapixels": round(
+        sum(w * h for w, h in sizes) / 1e6, 2), **tu}
+    return res
+
+
 def spmv_sweep(ctx, nmax, reps):
     """Average duration of each SpMV launch shape (HIP events on the library
     stream, the stream every dmsc call is enqueued on); leaves the fastest
@@ -299,6 +365,7 @@ def main():
     t_mixed, out_mixed = time_pipeline(ctx, d_rgb, dm.PREC_MIXED, a.steps, a.warmup)
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
+    batch = None if a.no_batch else batched(ctx, out_plain, out_mixed, max(3, a.steps), a.warmup)
     spmv = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
     roof = step_roofline(ctx, out_plain["nmax"])
     torch.cuda.synchronize()
@@ -328,6 +395,9 @@ def main():
         "roofline": roof,
         "spmv": spmv,
     }
+    if batch:
+        res["batched"] = batch
+        res["batched"]["mixed_batched_vs_plain_single"] = round(batch["mixed"]["images_per_s"] * t_plain, 3)
     if os.path.exists(TRUTH):
         truth = np.load(TRUTH)
         res["plain_fp32"]["err_vs_fp64"] = errors_vs_truth(out_plain, truth)

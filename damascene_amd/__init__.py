@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DmscError, GpbOutputs, GpbParams, check, lib
+from ._lib import BatchImage, DmscError, GpbOutputs, GpbParams, StepShape, check, lib
 
 __all__ = [
     "DmscError", "Context", "generate_image", "boundary_mask", "pitch_in_floats", "gpb",
@@ -171,6 +171,31 @@ class Context:
         check(lib().dmsc_convert_matrix(self.handle, w, h, _p(m), m.shape[1], _p(d), _p(rs)))
         return d, rs
 
+    def autotune_eigensolver(self, w, h, m, precision=PREC_FP32, steps=200):
+        """dmsc_autotune_eigensolver on the DIA matrix m (converted in place):
+        times every step-kernel shape and keeps the fastest for this size and
+        precision.  Returns (best, trials) as dicts."""
+        best = StepShape()
+        tried = (StepShape * 16)()
+        nt = ctypes.c_int()
+        check(lib().dmsc_autotune_eigensolver(self.handle, w, h, _p(m), m.shape[1], precision, steps,
+                                              ctypes.byref(best), tried, 16, ctypes.byref(nt)))
+
+        def d(x):
+            return {"mixed_fused": x.mixed_fused, "tile": x.tile, "k1_threads": x.k1_threads,
+                    "ms_per_step": x.ms_per_step}
+        return d(best), [d(tried[i]) for i in range(min(nt.value, 16))]
+
+    def autotune_spmv(self, w, h, m, reps=100):
+        """dmsc_autotune_spmv: returns (rows per lane selected, {shape: us})."""
+        x = self._torch.rand(h, w, device=self.device)
+        y = self.empty(h, w)
+        best = ctypes.c_int()
+        us = (ctypes.c_float * 4)()
+        check(lib().dmsc_autotune_spmv(self.handle, w, h, _p(m), m.shape[1], _p(x), _p(y), reps,
+                                       ctypes.byref(best), us))
+        return best.value, {s: us[i] for i, s in enumerate((0, 1, 2, 4))}
+
     def spmv(self, w, h, m, x):
         y = self.empty(h, w)
         check(lib().dmsc_stencil_spmv(self.handle, w, h, _p(m), m.shape[1], _p(x), _p(y)))
@@ -215,9 +240,7 @@ class Context:
         check(lib().dmsc_normalize_gpb_all(self.handle, npix, out.shape[0], out.shape[1], _p(out)))
         return out
 
-    def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False, texton_choice=TEXTON32):
-        """Whole pipeline on a device RGB8 image (int32/uint32 [h, w] tensor)."""
-        h, w = rgb.shape
+    def _gpb_buffers(self, w, h, n_eig):
         n = w * h
         pitch = pitch_in_floats(n)
         torch = self._torch
@@ -226,17 +249,50 @@ class Context:
             "mpb": self.zeros(8, pitch), "combined_g": self.zeros(8, pitch),
             "nmax": self.empty(h, w), "eigvecs": self.empty(n_eig, h, w),
             "spb": self.zeros(8, pitch), "gpb_all": self.zeros(8, pitch), "gpb": self.empty(h, w),
+            "pitch": pitch,
         }
         o = GpbOutputs()
         o.d_textons, o.d_mpb, o.d_combined_g = out["textons"].data_ptr(), out["mpb"].data_ptr(), out["combined_g"].data_ptr()
         o.d_nmax, o.d_eigvecs = out["nmax"].data_ptr(), out["eigvecs"].data_ptr()
         o.d_spb, o.d_gpb_all, o.d_gpb = out["spb"].data_ptr(), out["gpb_all"].data_ptr(), out["gpb"].data_ptr()
         o.pitch = pitch
-        p = GpbParams(n_eig, tol, texton_choice, precision)
-        check(lib().dmsc_gpb(self.handle, w, h, _p(rgb), ctypes.byref(p), ctypes.byref(o), 1 if timing else 0))
+        return out, o
+
+    @staticmethod
+    def _gpb_results(out, o, n_eig):
         out["eigvals"] = np.array(o.h_eigvals[:n_eig], np.float32)
         out["kmeans_iterations"] = o.kmeans_iterations
         out["lanczos_iterations"] = o.lanczos_iterations
         out["stage_ms"] = list(o.stage_ms[:6])
-        out["pitch"] = pitch
         return out
+
+    def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False, texton_choice=TEXTON32):
+        """Whole pipeline on a device RGB8 image (int32/uint32 [h, w] tensor)."""
+        h, w = rgb.shape
+        out, o = self._gpb_buffers(w, h, n_eig)
+        p = GpbParams(n_eig, tol, texton_choice, precision)
+        check(lib().dmsc_gpb(self.handle, w, h, _p(rgb), ctypes.byref(p), ctypes.byref(o), 1 if timing else 0))
+        return self._gpb_results(out, o, n_eig)
+
+    def gpb_batch(self, rgbs, n_eig=9, tol=1e-3, precision=PREC_FP32, texton_choice=TEXTON32, lanes=0,
+                  buffers=None):
+        """Whole pipeline on a list of device RGB8 images of any sizes
+        (dmsc_gpb_batch: concurrent lanes, largest images first).  Returns one
+        output dict per image, as gpb() does.  `buffers` (what
+        gpb_batch_buffers returned for images of the same sizes) reuses
+        output tensors across calls."""
+        if buffers is None:
+            buffers = self.gpb_batch_buffers(rgbs, n_eig)
+        outs, objs = buffers
+        items = (BatchImage * len(rgbs))()
+        for i, rgb in enumerate(rgbs):
+            h, w = rgb.shape
+            items[i].w, items[i].h, items[i].d_rgb = w, h, rgb.data_ptr()
+            items[i].out = ctypes.pointer(objs[i])
+        p = GpbParams(n_eig, tol, texton_choice, precision)
+        check(lib().dmsc_gpb_batch(self.handle, len(rgbs), items, ctypes.byref(p), lanes))
+        return [self._gpb_results(outs[i], objs[i], n_eig) for i in range(len(rgbs))]
+
+    def gpb_batch_buffers(self, rgbs, n_eig=9):
+        pairs = [self._gpb_buffers(r.shape[1], r.shape[0], n_eig) for r in rgbs]
+        return [q[0] for q in pairs], [q[1] for q in pairs]

@@ -26,6 +26,14 @@ class GpbOutputs(ctypes.Structure):
     ]
 
 
+class StepShape(ctypes.Structure):
+    _fields_ = [("mixed_fused", c_int), ("tile", c_int), ("k1_threads", c_int), ("ms_per_step", c_float)]
+
+
+class BatchImage(ctypes.Structure):
+    _fields_ = [("w", c_int), ("h", c_int), ("d_rgb", P), ("out", ctypes.POINTER(GpbOutputs))]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "dmsc_version": (ctypes.c_char_p, []),
@@ -76,6 +84,13 @@ SIGNATURES = {
     "dmsc_spectral_pb": (c_int, [P, P, P, c_int, c_int, c_int, P, c_int]),
     "dmsc_gpb_default_params": (None, [P]),
     "dmsc_gpb": (c_int, [P, c_int, c_int, P, P, P, c_int]),
+    "dmsc_gpb_batch": (c_int, [P, c_int, P, P, c_int]),
+    "dmsc_autotune_eigensolver": (c_int, [P, c_int, c_int, P, c_int, c_int, c_int, P, P, c_int, P]),
+    "dmsc_autotune_spmv": (c_int, [P, c_int, c_int, P, c_int, P, P, c_int, P, P]),
+    "dmsc_autotune_clear": (c_int, []),
+    "dmsc_combine_weights": (c_int, [c_int, P, P]),
+    "dmsc_spb_combine_forward": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, c_int, P, P]),
+    "dmsc_spb_combine_backward": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, P, P, P]),
 }
 
 _lib = None

@@ -2,6 +2,7 @@
 #include <cctype>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -27,6 +28,10 @@ int fail(int code, const char* fmt, ...) {
 }
 
 Ctx::~Ctx() {
+  for (Ctx* c : lanes) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete static_cast<dmsc_ctx*>(c);
+  }
   release_eigensolver(this);
   release_local_cues(this);
   for (int i = 0; i < kArenas; i++)
@@ -231,6 +236,11 @@ int dmsc_tune_set(const char* key, int value) {
     lanczos_tile = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "batch_lanes")) {
+    DMSC_ARG(value >= 1 && value <= 16, "batch_lanes must be in [1, 16]");
+    batch_lanes = value;
+    return DMSC_OK;
+  }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
 }
 int dmsc_tune_get(const char* key, int* value) {
@@ -253,6 +263,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "lanczos_tile")) {
     *value = lanczos_tile;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "batch_lanes")) {
+    *value = batch_lanes;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -482,6 +496,99 @@ int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* M, int pitch, in
   DMSC_ARG(M && ms_per_step && pitch >= w * h, "bad arguments");
   DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
   return time_lanczos_steps(ctx, w, h, M, pitch, precision, steps, ms_per_step);
+}
+
+int dmsc_autotune_eigensolver(dmsc_ctx* ctx, int w, int h, float* M, int pitch, int precision, int steps,
+                              dmsc_step_shape* best, dmsc_step_shape* tried, int max_tried, int* n_tried) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(M && pitch >= w * h && max_tried >= 0 && (tried || max_tried == 0), "bad arguments");
+  DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
+  std::vector<StepShape> t(std::max(max_tried, 0));
+  StepShape b{};
+  int nt = 0;
+  DMSC_TRY(autotune_steps(ctx, w, h, M, pitch, precision, steps, &b, t.data(), max_tried, &nt));
+  auto put = [](const StepShape& x, dmsc_step_shape* o) {
+    o->mixed_fused = x.fused;
+    o->tile = x.tile;
+    o->k1_threads = x.k1;
+    o->ms_per_step = x.ms;
+  };
+  if (best) put(b, best);
+  for (int i = 0; i < std::min(nt, max_tried); i++) put(t[i], &tried[i]);
+  if (n_tried) *n_tried = nt;
+  return DMSC_OK;
+}
+
+int dmsc_autotune_spmv(dmsc_ctx* ctx, int w, int h, const float* M, int pitch, const float* x, float* y, int reps,
+                       int* best_rows_per_lane, float* us_per_launch) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(w > 5 && h > 5 && pitch >= w * h && M && x && y && reps > 0, "bad arguments");
+  DMSC_ARG(((uintptr_t)M % 16) == 0 && ((uintptr_t)y % 16) == 0, "matrix and y must be 16-byte aligned");
+  const int shapes[4] = {0, 1, 2, 4};
+  hipEvent_t e0, e1;
+  DMSC_HIP(hipEventCreate(&e0));
+  DMSC_HIP(hipEventCreate(&e1));
+  const int saved = spmv_rows_per_lane;
+  int best = saved, rc = DMSC_OK;
+  float best_us = 0.f;
+  for (int i = 0; i < 4 && rc == DMSC_OK; i++) {
+    spmv_rows_per_lane = shapes[i];
+    rc = launch_spmv(ctx->stream, w, h, M, pitch, x, y);  // warm-up
+    if (rc == DMSC_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = fail(DMSC_ERR_HIP, "event record");
+    for (int r = 0; r < reps && rc == DMSC_OK; r++) rc = launch_spmv(ctx->stream, w, h, M, pitch, x, y);
+    float ms = 0.f;
+    if (rc == DMSC_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = fail(DMSC_ERR_HIP, "SpMV timing events failed");
+    const float us = 1e3f * ms / reps;
+    if (us_per_launch) us_per_launch[i] = us;
+    if (rc == DMSC_OK && (best_us == 0.f || us < best_us)) {
+      best_us = us;
+      best = shapes[i];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  spmv_rows_per_lane = rc == DMSC_OK ? best : saved;
+  if (rc == DMSC_OK && best_rows_per_lane) *best_rows_per_lane = best;
+  return rc;
+}
+
+int dmsc_combine_weights(int texton_choice, float* h_coef, float* h_wts) {
+  DMSC_ARG(h_coef && h_wts, "null pointer");
+  DMSC_ARG(texton_choice == DMSC_TEXTON32 || texton_choice == DMSC_TEXTON64, "texton_choice must be 1 or 2");
+  combine_tables(texton_count(texton_choice), h_coef, h_wts);
+  return DMSC_OK;
+}
+
+int dmsc_spb_combine_forward(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* bg, const float* cga,
+                             const float* cgb, const float* tg, int nvec, const float* eigvecs, const float* params,
+                             float* gpb_all, int pitch, float* gpb_pre, int* argmax) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(bg && cga && cgb && tg && eigvecs && params && gpb_pre && argmax, "null pointer");
+  DMSC_ARG(cue_pitch >= w * h && (!gpb_all || pitch >= w * h), "bad pitch");
+  return spb_combine_forward(ctx, w, h, cue_pitch, bg, cga, cgb, tg, nvec, eigvecs, params, gpb_all, pitch, gpb_pre,
+                             argmax);
+}
+
+int dmsc_spb_combine_backward(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* bg, const float* cga,
+                              const float* cgb, const float* tg, int nvec, const float* eigvecs, const float* params,
+                              const int* argmax, const float* grad_gpb_pre, float* grad_params,
+                              float* grad_eigvecs) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(bg && cga && cgb && tg && eigvecs && params && argmax && grad_gpb_pre && grad_params && grad_eigvecs,
+           "null pointer");
+  DMSC_ARG(cue_pitch >= w * h, "bad pitch");
+  return spb_combine_backward(ctx, w, h, cue_pitch, bg, cga, cgb, tg, nvec, eigvecs, params, argmax, grad_gpb_pre,
+                              grad_params, grad_eigvecs);
+}
+
+int dmsc_autotune_clear(void) {
+  autotune_clear();
+  return DMSC_OK;
 }
 
 int dmsc_spectral_pb(dmsc_ctx* ctx, const float* eigvals, const float* eigvecs, int w, int h, int nvec, float* spb,

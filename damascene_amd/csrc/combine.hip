@@ -126,6 +126,14 @@ static CombineW combine_weights(int ntex) {
                    0.0035f, 0.0188f}};
 }
 
+void combine_tables(int ntex, float* coef, float* wts) {
+  const CombineW cw = combine_weights(ntex);
+  for (int k = 0; k < 12; k++) {
+    coef[k] = cw.coef[k];
+    wts[k] = cw.wts[k];
+  }
+}
+
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga,
                    const float* cgb, const float* tg, int ntex, float* mpb, float* cg) {
   if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "combine supports 32 or 64 textons");

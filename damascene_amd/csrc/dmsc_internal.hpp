@@ -35,6 +35,7 @@ struct Ctx {
   int* h_pinned = nullptr;  // pinned host block (4 KiB) for asynchronous read-backs
   hipStream_t aux = nullptr;  // second stream (default priority) for work overlapped with `stream`
   hipEvent_t aux_ev[2] = {};
+  std::vector<Ctx*> lanes;  // child contexts of batch calls, each with its own stream
   ~Ctx();
   int pinned(int** out);
   int aux_stream(hipStream_t* s, hipEvent_t* ev0, hipEvent_t* ev1);

@@ -100,6 +100,7 @@ extern int lanczos_mixed_fused;
 // Tile of the tiled fused mixed step, TW * 100 + TH: 6404 (default), 6408,
 // 6402, 3208 or 3204.
 extern int lanczos_tile;
+extern int batch_lanes;
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,
@@ -107,6 +108,16 @@ int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float 
 void release_eigensolver(const Ctx* ctx);  // drops the context's captured graphs
 void release_local_cues(const Ctx* ctx);   // drops the context's cached local-cue plans
 int time_lanczos_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, float* ms_per_step);
+// Step-kernel shape of the Lanczos solver: mixed step form (0 two kernels,
+// 1 linear fused, 2 tiled fused), tile code (width * 100 + height) and
+// block size (0 = default); ms is the measured time per step.
+struct StepShape {
+  int fused, tile, k1;
+  float ms;
+};
+int autotune_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, StepShape* best,
+                   StepShape* tried, int max_tried, int* n_tried);
+void autotune_clear();
 
 // ------------------------------------------------------------- post-processing
 int run_postprocess(Ctx* ctx, int w, int h, float* gpb, const float* nmax, float* gpb_thin, int* iterations);
@@ -115,5 +126,19 @@ int launch_normalize_gpb_all(hipStream_t s, int npix, int norient, int pitch, fl
 // ------------------------------------------------------------- spectral Pb
 int launch_spectral_pb(hipStream_t s, const float* h_eigvals, const float* d_eigvecs, int w, int h, int nvec,
                        float* spb, int pitch);
+
+// combine.cu:16-21: mPb coefficients and combinedG weights (12 each) for
+// 32 or 64 textons.
+void combine_tables(int ntex, float* coef, float* wts);
+
+// Differentiable spectral Pb + combine (params [12 cue weights, gamma,
+// eta_0..eta_{nvec-1}] in device memory; see backward.hip).
+constexpr int kMaxDiffVec = 16;
+int spb_combine_forward(Ctx* ctx, int w, int h, int cue_pitch, const float* bg, const float* cga, const float* cgb,
+                        const float* tg, int nvec, const float* V, const float* prm, float* all, int pitch,
+                        float* gpb_pre, int* amax);
+int spb_combine_backward(Ctx* ctx, int w, int h, int cue_pitch, const float* bg, const float* cga, const float* cgb,
+                         const float* tg, int nvec, const float* V, const float* prm, const int* amax,
+                         const float* grad, float* grad_prm, float* dV);
 
 }  // namespace dmsc

@@ -902,9 +902,10 @@ __global__ void k_set_m(LzState* st, int m) { st->m = m; }
 
 // ---------------------------------------------------------------- host
 
-// Captured chunks of kGraphSteps steps, per context; the kernels' arguments
+// Graphs of kGraphSteps-step chunks, per context; the kernels' arguments
 // are baked into the graph, so the key holds every pointer and size they
-// depend on.
+// depend on (the solver's buffers are carved from its arena at offsets that
+// depend only on n, so st and n fix them).
 struct GraphKey {
   const void* A;
   const void* st;
@@ -916,16 +917,12 @@ struct GraphKey {
 };
 
 struct GraphCache {
-  hipStream_t cap = nullptr;
   std::map<GraphKey, hipGraphExec_t> execs;
   void clear() {
     for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
     execs.clear();
   }
-  ~GraphCache() {
-    clear();
-    if (cap) (void)hipStreamDestroy(cap);
-  }
+  ~GraphCache() { clear(); }
 };
 
 // One cache per context; the map is shared by all contexts (and host
@@ -957,56 +954,73 @@ struct Steps {
   StencilOffsets so;
 };
 
-void enqueue_steps(hipStream_t s, const Steps& p, int steps) {
+// Appends the kernel node fn<<<grid, block, lds>>>(args...) to g after
+// *last (the graph copies the arguments when the node is created).  Graphs
+// are built node by node rather than by stream capture, so building one on
+// a host thread never interacts with other threads' HIP calls.
+template <typename... A>
+int add_node(hipGraph_t g, hipGraphNode_t* last, const void* fn, dim3 grid, dim3 block, size_t lds, A... args) {
+  void* ptrs[] = {(void*)&args...};
+  hipKernelNodeParams kp{};
+  kp.func = const_cast<void*>(fn);
+  kp.gridDim = grid;
+  kp.blockDim = block;
+  kp.sharedMemBytes = (unsigned)lds;
+  kp.kernelParams = ptrs;
+  kp.extra = nullptr;
+  hipGraphNode_t n = nullptr;
+  DMSC_HIP(hipGraphAddKernelNode(&n, g, *last ? last : nullptr, *last ? 1 : 0, &kp));
+  *last = n;
+  return DMSC_OK;
+}
+
+// The kernels of `steps` consecutive Lanczos steps as a chain of graph
+// nodes; node t carries its offset t inside the chunk.
+int add_steps(hipGraph_t g, const Steps& p, int steps) {
+  hipGraphNode_t last = nullptr;
   for (int t = 0; t < steps; t++) {
     if (p.mixed && p.tile_h) {
       const int code = p.tile_w * 100 + p.tile_h;
-#define TILE_CASE(TW, TH)                                                                                 \
-  if (code == TW * 100 + TH)                                                                              \
-    hipLaunchKernelGGL((k_lzmt<TW, TH>), dim3(p.mix.nb1), dim3(TW * TH), 0, s, p.mix, p.so, p.w, p.h, t);
-      TILE_CASE(64, 4) TILE_CASE(64, 8) TILE_CASE(64, 2) TILE_CASE(32, 8) TILE_CASE(32, 4)
-#undef TILE_CASE
+      const void* fn = nullptr;
+      if (code == 6404) fn = (const void*)&k_lzmt<64, 4>;
+      else if (code == 6408) fn = (const void*)&k_lzmt<64, 8>;
+      else if (code == 6402) fn = (const void*)&k_lzmt<64, 2>;
+      else if (code == 3208) fn = (const void*)&k_lzmt<32, 8>;
+      else if (code == 3204) fn = (const void*)&k_lzmt<32, 4>;
+      else return fail(DMSC_ERR_INTERNAL, "no tiled step kernel for tile code %d", code);
+      DMSC_TRY(add_node(g, &last, fn, dim3(p.mix.nb1), dim3(p.tile_w * p.tile_h), 0, p.mix, p.so, p.w, p.h, t));
     } else if (p.mixed && p.fused_lds) {
-      if (p.k1 == 512)
-        hipLaunchKernelGGL(k_lzmf<512>, dim3(p.mix.nb1), dim3(512), p.fused_lds, s, p.mix, p.so, t);
-      else
-        hipLaunchKernelGGL(k_lzmf<256>, dim3(p.mix.nb1), dim3(256), p.fused_lds, s, p.mix, p.so, t);
+      const void* fn = p.k1 == 512 ? (const void*)&k_lzmf<512> : (const void*)&k_lzmf<256>;
+      DMSC_TRY(add_node(g, &last, fn, dim3(p.mix.nb1), dim3(p.k1 == 512 ? 512 : 256), p.fused_lds, p.mix, p.so, t));
     } else if (p.mixed) {
-      if (p.k1 == 64)
-        hipLaunchKernelGGL(k_lzm1<64>, dim3(p.mix.nb1), dim3(64), 0, s, p.mix, p.so, t);
-      else
-        hipLaunchKernelGGL(k_lzm1<256>, dim3(p.mix.nb1), dim3(256), 0, s, p.mix, p.so, t);
-      hipLaunchKernelGGL(k_lzm2, dim3(p.mix.nb2), dim3(kK2), 0, s, p.mix, t);
+      const void* fn = p.k1 == 64 ? (const void*)&k_lzm1<64> : (const void*)&k_lzm1<256>;
+      DMSC_TRY(add_node(g, &last, fn, dim3(p.mix.nb1), dim3(p.k1 == 64 ? 64 : 256), 0, p.mix, p.so, t));
+      DMSC_TRY(add_node(g, &last, (const void*)&k_lzm2, dim3(p.mix.nb2), dim3(kK2), 0, p.mix, t));
     } else {
-      if (p.k1 == 64)
-        hipLaunchKernelGGL(k_lz1<64>, dim3(p.plain.nb1), dim3(64), 0, s, p.plain, p.so, t);
-      else
-        hipLaunchKernelGGL(k_lz1<256>, dim3(p.plain.nb1), dim3(256), 0, s, p.plain, p.so, t);
-      hipLaunchKernelGGL(k_lz2, dim3(p.plain.nb2), dim3(kK2), 0, s, p.plain, t);
+      const void* fn = p.k1 == 64 ? (const void*)&k_lz1<64> : (const void*)&k_lz1<256>;
+      DMSC_TRY(add_node(g, &last, fn, dim3(p.plain.nb1), dim3(p.k1 == 64 ? 64 : 256), 0, p.plain, p.so, t));
+      DMSC_TRY(add_node(g, &last, (const void*)&k_lz2, dim3(p.plain.nb2), dim3(kK2), 0, p.plain, t));
     }
   }
+  return DMSC_OK;
 }
 
 // Steps first .. first+steps-1 on ctx->stream through a cached HIP graph.
 int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps) {
   GraphCache* gc = graph_cache(ctx);
-  if (!gc->cap) DMSC_HIP(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
                      steps, p.k1, (int)p.fused_lds + 1000000 * (p.tile_w * 100 + p.tile_h)};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
-    DMSC_HIP(hipStreamBeginCapture(gc->cap, hipStreamCaptureModeThreadLocal));
-    enqueue_steps(gc->cap, p, steps);
-    const hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(gc->cap, &g);
-    DMSC_HIP(le);
-    DMSC_HIP(ce);
-    hipGraphExec_t exec;
-    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    DMSC_HIP(hipGraphCreate(&g, 0));
+    const int rc = add_steps(g, p, steps);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = rc == DMSC_OK ? hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) : hipSuccess;
     (void)hipGraphDestroy(g);
+    if (rc != DMSC_OK) return rc;
     DMSC_HIP(ie);
     it = gc->execs.emplace(key, exec).first;
   }
@@ -1058,11 +1072,38 @@ int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, dou
   return (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
 }
 
+// Step-kernel shapes chosen by the autotuner, per (w, h, precision); a
+// solve on a size with an entry uses it instead of the process-wide tuning
+// values.
+std::map<std::tuple<int, int, int>, StepShape>& tuned_shapes() {
+  static std::map<std::tuple<int, int, int>, StepShape> t;
+  return t;
+}
+std::mutex& tuned_lock() {
+  static std::mutex m;
+  return m;
+}
+
+StepShape default_shape(int w, int h, bool mixed) {
+  {
+    std::lock_guard<std::mutex> lk(tuned_lock());
+    auto it = tuned_shapes().find(std::make_tuple(w, h, mixed ? 1 : 0));
+    if (it != tuned_shapes().end()) return it->second;
+  }
+  return StepShape{mixed ? lanczos_mixed_fused : 0, lanczos_tile, lanczos_k1_threads};
+}
+
 // time_steps > 0: run that many steps with no checks between two HIP
 // events on the context's stream and return the average ms per step in
-// *ms_per_step (the measurement mode of dmsc_lanczos_time_steps).
+// *ms_per_step (the measurement mode of dmsc_lanczos_time_steps).  `force`
+// selects the step-kernel shape (the autotuner's trials); otherwise the
+// autotuned shape of this size, else the process-wide tuning values.
+// convert == false: A already holds L_sym (a later timing run on the same
+// matrix).
 int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool mixed, float* h_eigvals,
-          float* d_eigvecs, int* iterations, int time_steps = 0, float* ms_per_step = nullptr) {
+          float* d_eigvecs, int* iterations, int time_steps = 0, float* ms_per_step = nullptr,
+          const StepShape* force = nullptr, bool convert = true) {
+  const StepShape shape = force ? *force : default_shape(w, h, mixed);
   hipStream_t s = ctx->stream;
   const int n = w * h;
   const size_t vstride = (size_t)pitch_floats(n);
@@ -1091,15 +1132,15 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   // two-kernel step when that does not fit; the two-kernel steps take 64 or
   // 256 rows per block.
   const int halo = make_stencil_offsets(w).halo;
-  int k1 = lanczos_k1_threads == 64 ? 64 : 256;
+  int k1 = shape.k1 == 64 ? 64 : 256;
   size_t fused_lds = 0;
   int tile_w = 0, tile_h = 0;
-  if (mixed && lanczos_mixed_fused == 2) {  // pixel tiles, TW x TH
-    tile_w = lanczos_tile / 100;
-    tile_h = lanczos_tile % 100;
+  if (mixed && shape.fused == 2) {  // pixel tiles, TW x TH
+    tile_w = shape.tile / 100;
+    tile_h = shape.tile % 100;
     k1 = tile_w * tile_h;
-  } else if (mixed && lanczos_mixed_fused == 1) {
-    const int kf = (lanczos_k1_threads == 0 || lanczos_k1_threads == 512) ? 512 : 256;
+  } else if (mixed && shape.fused == 1) {
+    const int kf = (shape.k1 == 0 || shape.k1 == 512) ? 512 : 256;
     const size_t bytes = sizeof(float) * ((size_t)kf + 2 * (size_t)halo);
     if (bytes <= invit_lds) {
       k1 = kf;
@@ -1175,7 +1216,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
                   nb2,
                   st};
 
-  DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
+  if (convert) DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
   const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
   const dim3 gi(div_up(std::max(n, nb1), 256));
   hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
@@ -1372,6 +1413,48 @@ int time_lanczos_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precisio
     return fail(DMSC_ERR_ARG, "steps must be a positive multiple of %d", kGraphSteps);
   return solve(ctx, w, h, A, pitch, 9, 1e-3f, precision == DMSC_PREC_MIXED, nullptr, nullptr, nullptr, steps,
                ms_per_step);
+}
+
+// Times every step-kernel shape for this size and precision (25 warm-up
+// and `steps` timed steps each, on the matrix converted once in place),
+// records the fastest as the size's shape and returns every trial.
+int autotune_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, StepShape* best,
+                   StepShape* tried, int max_tried, int* n_tried) {
+  if (pitch % 4) return fail(DMSC_ERR_ARG, "matrix pitch must be a multiple of 4 floats");
+  if (steps < kGraphSteps || steps % kGraphSteps)
+    return fail(DMSC_ERR_ARG, "steps must be a positive multiple of %d", kGraphSteps);
+  const bool mixed = precision == DMSC_PREC_MIXED;
+  std::vector<StepShape> cand;
+  if (mixed) {
+    for (int tile : {6404, 6408, 6402, 3208, 3204}) cand.push_back(StepShape{2, tile, 0});
+    for (int k1 : {512, 256}) cand.push_back(StepShape{1, 6404, k1});
+    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 6404, k1});
+  } else {
+    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 6404, k1});
+  }
+  int nt = 0;
+  StepShape top{};
+  top.ms = 0.f;
+  for (StepShape c : cand) {
+    float ms = 0.f;
+    DMSC_TRY(solve(ctx, w, h, A, pitch, 9, 1e-3f, mixed, nullptr, nullptr, nullptr, steps, &ms, &c, nt == 0));
+    c.ms = ms;
+    if (tried && nt < max_tried) tried[nt] = c;
+    nt++;
+    if (top.ms == 0.f || ms < top.ms) top = c;
+  }
+  {
+    std::lock_guard<std::mutex> lk(tuned_lock());
+    tuned_shapes()[std::make_tuple(w, h, mixed ? 1 : 0)] = top;
+  }
+  if (best) *best = top;
+  if (n_tried) *n_tried = nt;
+  return DMSC_OK;
+}
+
+void autotune_clear() {
+  std::lock_guard<std::mutex> lk(tuned_lock());
+  tuned_shapes().clear();
 }
 
 void release_eigensolver(const Ctx* ctx) {

@@ -82,6 +82,9 @@ int dmsc_halfdisc_offsets(int w, int h, float thetaPi, int radius, const int* h_
 int dmsc_blur_kernel(float sigma, float* h_kernel /*<=64*/, int* radius); /* rotate.cu:29 */
 int dmsc_stencil(int radius, int width, int* h_offsets, int* h_diag_map); /* Stencil.cpp:6 */
 int dmsc_gamma_lut(float gamma, float* h_lut /*256*/);
+/* combine.cu:16-21: the 12 mPb coefficients and 12 combinedG weights for
+ * texton_choice 1 (32 textons) or 2 (64). */
+int dmsc_combine_weights(int texton_choice, float* h_coef /*12*/, float* h_wts /*12*/);
 /* convert.cu:88-95 loadPPM_rgbU (cutLoadPPM4ub packing): binary PPM (P6,
  * maxval 255) -> host uint32 per pixel, R in bits 0-7, G 8-15, B 16-23.
  * h_rgb may be NULL to query the size. */
@@ -212,6 +215,33 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  * kernels are the ones dmsc_generalized_eigensolve uses. */
 int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch, int precision,
                             int steps, float* ms_per_step);
+/* Autotuner (no reference counterpart; the reference's launch shapes are
+ * fixed).  dmsc_autotune_eigensolver converts d_matrix in place once, then
+ * times every step-kernel shape of the given precision -- mixed: tiled
+ * fused steps with tiles 64x4, 64x8, 64x2, 32x8, 32x4, linear fused steps
+ * of 512 and 256 rows per block, two-kernel steps of 256 and 64; plain:
+ * two-kernel steps of 256 and 64 -- with `steps` (a multiple of 25) timed
+ * steps each, and records the fastest as the shape every later eigensolve
+ * of this size and precision uses (in this process).  Shapes change results
+ * only through the rounding of block partial sums.  `tried` receives up to
+ * max_tried trials, *n_tried their number. */
+typedef struct {
+  int mixed_fused;    /* 0 two kernels, 1 linear fused, 2 tiled fused (mixed only) */
+  int tile;           /* tiled step: tile width * 100 + height */
+  int k1_threads;     /* block size of the (first) step kernel, 0 = default */
+  float ms_per_step;  /* measured */
+} dmsc_step_shape;
+int dmsc_autotune_eigensolver(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch, int precision,
+                              int steps, dmsc_step_shape* best, dmsc_step_shape* tried, int max_tried,
+                              int* n_tried);
+/* Times the four SpMV launch shapes (rows per lane 0 = one row per lane
+ * over all rows, 1, 2, 4; `reps` launches each on d_x -> d_y), leaves the
+ * fastest selected process-wide ("spmv_rows_per_lane") and returns it with
+ * the four average launch durations in microseconds. */
+int dmsc_autotune_spmv(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch, const float* d_x,
+                       float* d_y, int reps, int* best_rows_per_lane, float* us_per_launch /* [4] */);
+/* Forgets every autotuned eigensolver shape. */
+int dmsc_autotune_clear(void);
 int dmsc_tune_set(const char* key, int value);
 int dmsc_tune_get(const char* key, int* value);
 
@@ -247,6 +277,50 @@ void dmsc_gpb_default_params(dmsc_gpb_params* p);
  * per-stage milliseconds with HIP events (adds synchronisation). */
 int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* d_rgb, const dmsc_gpb_params* params,
              dmsc_gpb_outputs* out, int timing);
+
+/* Differentiable spectral Pb + combine (no reference counterpart: the
+ * reference has no backward pass).  The forward generalises
+ * combine.cu:24-70, spectralPb.cu:651-659 and globalPb.cu:5-49 with free
+ * weights, d_params = [w_0..w_11 (cue x scale weights, combine.cu:21 order),
+ * gamma (the sPb weight, 0.0139 in globalPb.cu:13), eta_0..eta_{nvec-1}
+ * (eigenvector weights, 1/sqrt(lambda_i) in spectralPb.cu:759; eta_0
+ * unused)], a device array of 13 + nvec floats:
+ *   gPbAll_o = sum_k w_k C_{k,o} + gamma sum_{i>=1} eta_i |K_o * mirror3(v_i)|,
+ *   gPbPre   = max_o gPbAll_o, argmax = the first maximising orientation.
+ * With the reference's weights gPbAll is bit-identical to dmsc_gpb's.
+ * d_gpb_all ([8][pitch]) may be NULL.  nvec in [2, 16].
+ * The backward takes dL/dgPbPre and the forward's argmax and writes
+ * dL/dparams (13 + nvec floats, device; eta_0's is 0) and dL/deigvecs
+ * ([nvec][N], device; vector 0's is 0).  Parameter gradients are reduced in
+ * a fixed order (deterministic). */
+int dmsc_spb_combine_forward(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* d_bg, const float* d_cga,
+                             const float* d_cgb, const float* d_tg, int nvec, const float* d_eigvecs,
+                             const float* d_params, float* d_gpb_all, int pitch, float* d_gpb_pre, int* d_argmax);
+int dmsc_spb_combine_backward(dmsc_ctx* ctx, int w, int h, int cue_pitch, const float* d_bg, const float* d_cga,
+                              const float* d_cgb, const float* d_tg, int nvec, const float* d_eigvecs,
+                              const float* d_params, const int* d_argmax, const float* d_grad_gpb_pre,
+                              float* d_grad_params, float* d_grad_eigvecs);
+
+/* One image of a batch: its size, its RGB8 pixels (device) and the
+ * outputs, exactly as the arguments of dmsc_gpb. */
+typedef struct {
+  int w, h;
+  const uint32_t* d_rgb;
+  dmsc_gpb_outputs* out;
+} dmsc_batch_image;
+
+/* No counterpart in the reference, which runs one image per process
+ * (damascene.cu:130-207).  Runs every image of the batch through the
+ * dmsc_gpb pipeline with the same parameters, spread over `lanes` child
+ * contexts of ctx (0 = the "batch_lanes" tuning value, default 4), each with
+ * its own stream and host thread, largest images first.  The lanes start
+ * after the work already enqueued on ctx's stream; the call returns when
+ * every image is finished.  Each image's results are bit-identical to a
+ * dmsc_gpb call on it.  Images may differ in size; their outputs must not
+ * overlap.  The first failing image stops the batch and its error is
+ * returned. */
+int dmsc_gpb_batch(dmsc_ctx* ctx, int count, const dmsc_batch_image* images, const dmsc_gpb_params* params,
+                   int lanes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,139 @@
+"""Differentiable spectral Pb + combine on the GPU (dmsc_spb_combine_forward /
+dmsc_spb_combine_backward behind torch.autograd).
+
+* Forward: with the reference's weights the gPbAll planes are bit-identical
+  to the pipeline's (dmsc_gpb), and gPbPre / argmax are their max / first
+  argmax.
+* Backward: the loss (class-balanced BCE on the PostProcess logit) and its
+  gradients with respect to the 12 cue weights, gamma, the eigenvector
+  weights and the eigenvector maps against the FP64 reference
+  (oracle/grad_ref.py: FP64 forward + automatic differentiation, itself
+  pinned by finite differences on the CPU).  Bounds: 1e-3 relative for the
+  parameters; for the eigenvector maps 1e-4 of the largest entry away from
+  the 7 x 7 windows of pixels whose maximising orientation or response sign
+  is a near-tie in the FP64 forward (where FP32 may legitimately route the
+  gradient elsewhere: inside flat cells the eigenvectors are nearly constant
+  and their responses nearly 0, and on the edge frame the mirrored windows
+  nearly cancel), with more than 30 % of the 4-pixel edge frame (the mirror
+  adjoint) compared; and 1e-3 in relative L2 overall.
+* Refit: Adam on the cue weights and gamma lowers the loss.
+"""
+import numpy as np
+import pytest
+import torch
+
+import damascene_amd as dm
+from damascene_amd import autograd as AG
+from oracle import grad_ref as G
+
+pytestmark = pytest.mark.gpu
+
+W, H, CELLS, SEED = 160, 120, 12, 20261202
+
+
+@pytest.fixture(scope="module")
+def data(ctx):
+    print(f"case: backward {W}x{H} cells={CELLS} seed={SEED}", flush=True)
+    rgb, cells = dm.generate_image(W, H, CELLS, SEED, with_cells=True)
+    mask = dm.boundary_mask(cells).astype(bool)
+    d_rgb = ctx.upload(rgb)
+    out = ctx.gpb(d_rgb, precision=dm.PREC_MIXED)
+    grey = ctx.rgb_to_grey(d_rgb)
+    L, a, b = ctx.rgb_to_lab(d_rgb)
+    tex, _ = ctx.find_textons(grey)
+    cues = ctx.local_cues(L, a, b, tex)
+    return out, cues, mask
+
+
+def test_forward_bit_identical_to_pipeline(ctx, data):
+    out, cues, _ = data
+    n = W * H
+    op = AG.SpbCombine(ctx, W, H, cues)
+    prm = AG.reference_params(out["eigvals"], device=ctx.device)
+    all_ = ctx.zeros(8, out["pitch"])
+    pre, amax = op.forward(prm, out["eigvecs"], gpb_all=all_)
+    torch.cuda.synchronize()
+    got = all_[:, :n].cpu().numpy()
+    want = out["gpb_all"][:, :n].cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), "gPbAll differs from the pipeline's"
+    np.testing.assert_array_equal(pre.cpu().numpy().ravel(), want.max(axis=0))
+    np.testing.assert_array_equal(amax.cpu().numpy().ravel(), want.argmax(axis=0))
+
+
+def test_backward_against_fp64(ctx, data):
+    out, cues, mask = data
+    op = AG.SpbCombine(ctx, W, H, cues)
+    prm = AG.reference_params(out["eigvals"], device=ctx.device).requires_grad_(True)
+    vec = out["eigvecs"].clone().requires_grad_(True)
+    loss = AG.balanced_bce(op(prm, vec), torch.from_numpy(mask).to(ctx.device))
+    loss.backward()
+    gp, gv = prm.grad.cpu().numpy(), vec.grad.cpu().numpy()
+    C = G.cue_planes(*[c.cpu().numpy() for c in cues], W, H)
+    l64, gp64, gv64 = G.loss_and_grads(prm.detach().cpu().numpy(), vec.detach().cpu().numpy(), C, mask, W, H)
+    print(f"loss gpu {loss.item():.8f} fp64 {l64:.8f}", flush=True)
+    dpar = np.abs(gp - gp64) / (np.abs(gp64) + 1e-6 * np.abs(gp64).max())
+    diff = np.abs(gv - gv64)
+    scale = np.abs(gv64).max()
+    l2 = np.linalg.norm(gv - gv64) / np.linalg.norm(gv64)
+    bad = diff > 1e-3 * scale
+    i, y, x = np.unravel_index(np.argmax(diff), diff.shape)
+    # entries within 4 px of the image edge (where the mirror adjoint folds back)
+    edge = np.zeros((H, W), bool)
+    edge[:4], edge[-4:], edge[:, :4], edge[:, -4:] = True, True, True, True
+    print("param grads gpu ", np.array2string(gp, precision=5), "\n           fp64", np.array2string(gp64, precision=5),
+          f"\nmax rel param diff {dpar.max():.2e}; eigvec grads: rel L2 {l2:.2e}, max diff / max {diff.max() / scale:.2e} "
+          f"at {(i, y, x)}, entries off by > 1e-3 max: {int(bad.sum())} of {bad.size} "
+          f"({int(bad[:, edge].sum())} near an edge); edge rel L2 "
+          f"{np.linalg.norm((gv - gv64)[:, edge]) / np.linalg.norm(gv64[:, edge]):.2e}", flush=True)
+    assert abs(loss.item() - l64) <= 1e-5 * abs(l64)
+    assert dpar.max() < 1e-3
+    # eigenvector maps: a pixel whose maximising orientation or response sign
+    # is ambiguous at FP32 (a near-tie in the FP64 forward) may route its
+    # gradient differently in the two forwards, which moves the gradient of
+    # the 7 x 7 window around it (mirrored reads stay inside that window);
+    # everywhere else the maps agree to FP32 rounding
+    all64, R64 = G.parts64(torch.as_tensor(prm.detach().cpu().numpy(), dtype=torch.float64),
+                           torch.as_tensor(vec.detach().cpu().numpy(), dtype=torch.float64), C, W, H)
+    all64, R64 = all64.numpy(), R64.numpy()
+    top2 = np.sort(all64, axis=0)[-2:]
+    ostar = all64.argmax(axis=0)
+    rstar = np.take_along_axis(np.abs(R64), ostar[None, None], axis=1)[:, 0]
+    tie = top2[1] - top2[0] < 1e-5 * np.abs(all64).max()  # orientation: moves every vector's gradient
+    tiny = rstar < 1e-5 * np.abs(R64).max()  # response sign of vector i: moves vector i's gradient
+    clean, kept, namb = 0.0, [], 0
+    for i in range(1, vec.shape[0]):
+        amb = tie | tiny[i - 1]
+        namb += int(amb.sum())
+        near = np.zeros_like(amb)
+        for dy in range(-3, 4):
+            for dx in range(-3, 4):
+                near |= np.roll(np.roll(amb, dy, 0), dx, 1)
+        clean = max(clean, diff[i][~near].max() / scale)
+        kept.append((edge & ~near).sum() / edge.sum())
+    edge_kept = float(np.mean(kept))
+    print(f"ambiguous (vector, pixel) pairs {namb} (orientation near-ties {int(tie.sum())}); eigvec grads away "
+          f"from their windows: max diff / max {clean:.2e} ({edge_kept:.0%} of the edge frame compared)", flush=True)
+    assert clean < 1e-4
+    assert edge_kept > 0.3  # the mirror adjoint is exercised
+    assert l2 < 1e-3
+    assert gp[13] == 0 and np.all(gv[0] == 0)
+
+
+def test_refit_lowers_loss(ctx, data):
+    out, cues, mask = data
+    op = AG.SpbCombine(ctx, W, H, cues)
+    prm0 = AG.reference_params(out["eigvals"], device=ctx.device)
+    wg = prm0[:13].clone().requires_grad_(True)
+    eta = prm0[13:]
+    y = torch.from_numpy(mask).to(ctx.device)
+    opt = torch.optim.Adam([wg], lr=1e-2)
+    losses = []
+    for _ in range(30):
+        opt.zero_grad()
+        loss = AG.balanced_bce(op(torch.cat([wg, eta]), out["eigvecs"]), y)
+        loss.backward()
+        opt.step()
+        losses.append(loss.item())
+    print("refit losses", [round(v, 5) for v in losses[::5]], "->", round(losses[-1], 5), flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.9 * losses[0]

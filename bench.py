@@ -11,7 +11,7 @@ relative error, subspace angle of eigenvectors 1..8, gPb max-abs error,
 nonmax mismatches against the oracle).
 
 roofline: the dominant kernel of the timed region, the mixed path's fused
-Lanczos step k_lzmt<64, 4> (one launch per Krylov step: FP64-accumulated
+Lanczos step k_lzmt<32, 8> (one launch per Krylov step: FP64-accumulated
 81-diagonal stencil SpMV from an LDS-staged basis vector plus the three-term
 update; 348 algorithmic bytes per row), its average launch duration measured
 with HIP events on the library's stream over 500 steps; `traffic` is the HBM
@@ -23,6 +23,10 @@ batched: dmsc_gpb_batch over 16 images of 481x321 (seeds 20261202 + i) and
 over 64 images of uneven size (sides uniform in [200, 1024]), 4 concurrent
 lanes, both precisions; image 0 of the 16 is the single-image workload and
 is checked bit for bit against the single-image call.
+
+differentiable: 200 Adam steps (lr 1e-2) refitting the combine weights on
+16 images of 481x321 through the hand-written spectral Pb + combine backward
+(loss before and after, time per forward + backward over the 16 images).
 
 cpu_baseline: the oracle's reference-order pipeline on one image on the host
 cores (OpenMP).
@@ -47,7 +51,7 @@ SPMV_BYTES_PER_ROW = 81 * 4 + 4 + 4  # 81 matrix entries + x + y (algorithmic)
 SPMV_SHAPES = {0: "k_spmv_full", 1: "k_spmv<1>", 2: "k_spmv<2>", 4: "k_spmv<4>"}
 # fused mixed Lanczos step, per row: 81 matrix entries (324 B), the staged
 # z'_{k-1} (8 B) and q_{k-1} (4 B) read once, q_k (4 B) and z'_k (8 B) written
-STEP_KERNEL = "k_lzmt<64, 4>"
+STEP_KERNEL = "k_lzmt<32, 8>"
 STEP_BYTES_PER_ROW = 81 * 4 + 8 + 4 + 4 + 8
 STEP_TIMED = 500
 TRUTH = os.path.join(ROOT, "tests", "golden", "truth_481x321.npz")
@@ -61,7 +65,7 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    p.add_argument("--no-batch", action="store_true", help="skip the batched measurements")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched and differentiable measurements")
     p.add_argument("--spmv-reps", type=int, default=200)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the workload of the rocprofv3 counter passes: one 256 MiB device copy "
@@ -253,6 +257,57 @@ def batched(ctx, out_plain, out_mixed, steps, warmup):
     return res
 
 
+def differentiable(ctx, steps=200, images=16, lr=1e-2):
+    """Refit of the combine weights through the hand-written spectral Pb +
+    combine backward: 16 images of 481x321 (seeds SEED + 200 + i) with their
+    Voronoi cell boundaries dilated by 1 px as the masks, eigenvectors from
+    the mixed eigensolver, class-balanced BCE before non-max; Adam on the
+    12 cue x scale weights and gamma.  Reports the loss before and after and
+    the time of one forward + backward over the 16 images."""
+    import torch
+
+    import damascene_amd as dm
+    from damascene_amd import autograd as AG
+
+    ops, vecs, masks, etas = [], [], [], []
+    for i in range(images):
+        rgb, cells = dm.generate_image(W, H, CELLS, SEED + 200 + i, with_cells=True)
+        d_rgb = ctx.upload(rgb)
+        out = ctx.gpb(d_rgb, precision=dm.PREC_MIXED)
+        L, a, b = ctx.rgb_to_lab(d_rgb)
+        tex, _ = ctx.find_textons(ctx.rgb_to_grey(d_rgb))
+        ops.append(AG.SpbCombine(ctx, W, H, ctx.local_cues(L, a, b, tex)))
+        vecs.append(out["eigvecs"])
+        masks.append(torch.from_numpy(dm.boundary_mask(cells).astype(bool)).to(ctx.device))
+        etas.append(AG.reference_params(out["eigvals"], device=ctx.device)[13:])
+    w0 = AG.reference_params(np.ones(NVEC, np.float32), device=ctx.device)[:13]
+    wg = w0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([wg], lr=lr)
+
+    def step():
+        opt.zero_grad()
+        loss = sum(AG.balanced_bce(op(torch.cat([wg, e]), v), m) for op, v, m, e in zip(ops, vecs, masks, etas))
+        loss = loss / images
+        loss.backward()
+        return loss
+
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        losses.append(step().item())
+        opt.step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"images": images, "size": f"{W}x{H}", "adam_steps": steps, "lr": lr,
+            "loss_initial": round(losses[0], 6), "loss_final": round(losses[-1], 6),
+            "loss_drop": round(1 - losses[-1] / losses[0], 4),
+            "ms_per_step_fwd_bwd_16_images": round(dt * 1e3, 3),
+            "what": "Adam on the 12 cue x scale weights and the sPb weight through dmsc_spb_combine_forward/"
+                    "backward (torch.autograd.Function); class-balanced BCE on the PostProcess logit of the "
+                    "pre-non-max gPb against cell boundaries dilated 1 px"}
+
+
 def spmv_sweep(ctx, nmax, reps):
     """Average duration of each SpMV launch shape (HIP events on the library
     stream, the stream every dmsc call is enqueued on); leaves the fastest
@@ -366,6 +421,7 @@ def main():
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
     batch = None if a.no_batch else batched(ctx, out_plain, out_mixed, max(3, a.steps), a.warmup)
+    diff = None if a.no_batch else differentiable(ctx)
     spmv = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
     roof = step_roofline(ctx, out_plain["nmax"])
     torch.cuda.synchronize()
@@ -395,6 +451,8 @@ def main():
         "roofline": roof,
         "spmv": spmv,
     }
+    if diff:
+        res["differentiable"] = diff
     if batch:
         res["batched"] = batch
         res["batched"]["mixed_batched_vs_plain_single"] = round(batch["mixed"]["images_per_s"] * t_plain, 3)

@@ -176,15 +176,15 @@ class Context:
         times every step-kernel shape and keeps the fastest for this size and
         precision.  Returns (best, trials) as dicts."""
         best = StepShape()
-        tried = (StepShape * 16)()
+        tried = (StepShape * 32)()
         nt = ctypes.c_int()
         check(lib().dmsc_autotune_eigensolver(self.handle, w, h, _p(m), m.shape[1], precision, steps,
-                                              ctypes.byref(best), tried, 16, ctypes.byref(nt)))
+                                              ctypes.byref(best), tried, 32, ctypes.byref(nt)))
 
         def d(x):
             return {"mixed_fused": x.mixed_fused, "tile": x.tile, "k1_threads": x.k1_threads,
                     "ms_per_step": x.ms_per_step}
-        return d(best), [d(tried[i]) for i in range(min(nt.value, 16))]
+        return d(best), [d(tried[i]) for i in range(min(nt.value, 32))]
 
     def autotune_spmv(self, w, h, m, reps=100):
         """dmsc_autotune_spmv: returns (rows per lane selected, {shape: us})."""

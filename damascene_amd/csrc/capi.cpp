@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <vector>
 
 #include "dmsc_internal.hpp"
@@ -231,9 +232,15 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "lanczos_tile")) {
-    DMSC_ARG(value == 6404 || value == 6408 || value == 6402 || value == 3208 || value == 3204,
-             "lanczos_tile must be 6404, 6408, 6402, 3208 or 3204 (tile width * 100 + height)");
+    DMSC_ARG(std::find(std::begin(kStepTiles), std::end(kStepTiles), value) != std::end(kStepTiles),
+             "lanczos_tile must be one of 3208, 6404, 1616, 3216, 1608, 6408, 6402, 3204 (width * 100 + height)");
     lanczos_tile = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_max_iter")) {
+    DMSC_ARG(value >= 25 && value <= kMaxIter && value % 25 == 0, "lanczos_max_iter must be a multiple of 25 in [25, %d]",
+             kMaxIter);
+    lanczos_max_iter = value;
     return DMSC_OK;
   }
   if (!std::strcmp(key, "batch_lanes")) {
@@ -263,6 +270,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "lanczos_tile")) {
     *value = lanczos_tile;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_max_iter")) {
+    *value = lanczos_max_iter;
     return DMSC_OK;
   }
   if (!std::strcmp(key, "batch_lanes")) {
@@ -385,6 +396,13 @@ int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* r
     DMSC_TRY(launch_kmeans(ctx->stream, w, h, k, nfilters, resp, labels, pdot, scr, max_iter));
   }
   return kmeans_read_state(ctx->stream, scr, changes, iterations);
+}
+
+int dmsc_kmeans_rechecks(dmsc_ctx* ctx, int* rechecks) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(rechecks, "null pointer");
+  DMSC_ARG(ctx->arena[2] && ctx->arena_bytes[2] >= kmeans_state_bytes(), "no k-means run on this context");
+  return kmeans_read_state(ctx->stream, ctx->arena[2], nullptr, nullptr, rechecks);
 }
 
 int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* grey, int* textons, int texton_choice,

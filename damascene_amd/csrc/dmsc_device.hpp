@@ -39,6 +39,30 @@ __device__ inline float cbrt_exact(float xf) {
   return (float)(x < 0 ? -r : r);
 }
 
+// exp of a float argument from basic IEEE double operations: Cody-Waite
+// reduction x = n ln2 + r (n the nearest integer to x / ln2, ln2 split in a
+// high part with 21 trailing zero bits and a low part, the split of fdlibm's
+// e_exp.c), exp(r) by the degree-12 Taylor polynomial (|r| <= ln2 / 2:
+// remainder below 2e-16 relative) in Horner form with fused multiply-adds,
+// times 2^n, rounded once to float.  The oracle runs the same operations,
+// so results are bit-identical to it.
+__device__ inline float exp_exact(float xf) {
+  if (xf != xf) return xf;
+  if (xf > 89.0f) return __int_as_float(0x7f800000);
+  if (xf < -104.0f) return 0.0f;
+  const double x = (double)xf;
+  const double n = rint(x * 1.4426950408889634);
+  double r = fma(-n, 6.93147180369123816490e-01, x);
+  r = fma(-n, 1.90821492927058770002e-10, r);
+  constexpr double f[13] = {1.0,       1.0,       1.0 / 2,    1.0 / 6,     1.0 / 24,      1.0 / 120,      1.0 / 720,
+                            1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600};
+  double p = f[12];
+#pragma unroll
+  for (int k = 11; k >= 0; k--) p = fma(p, r, f[k]);
+  const double scale = __longlong_as_double((long long)(((unsigned long long)((long long)n + 1023)) << 52));
+  return (float)(p * scale);
+}
+
 // In-register 32-slot tree sum in the order of the reference's warp
 // reduction (rotate.cu:378-389): (i, i+16), (i, i+8), ..., (0, 1).
 template <typename T>

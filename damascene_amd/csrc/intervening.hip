@@ -33,17 +33,18 @@ struct IcCtx {
   }
 };
 
-__device__ inline float c_ic_ss(float ic) {  // intervening.cu:32-42
+__device__ inline float c_ic_ss(float ic) {  // intervening.cu:32-42 (project exp, as above)
   float val = 1.8682f;
   val += (ic / 0.3130f) * -1.3113f;
-  const float post = 1.0f / (1.0f + expf(-val));
+  const float post = 1.0f / (1.0f + dev::exp_exact(-val));
   if (!isfinite(post)) return 0;
   if (post < 0) return 0;
   if (post > 1) return 1;
   return post;
 }
 
-__device__ inline float affinity(float c, float rs) { return expf(-(1 - c) * rs); }  // 28-30
+// intervening.cu:28-30, with the project exp in place of the device expf
+__device__ inline float affinity(float c, float rs) { return dev::exp_exact(-(1 - c) * rs); }
 
 // intervening.cu:44-283 ic_walk
 __device__ void ic_walk(const IcCtx& C, const DiagMap& dm, float rs, float* M, size_t pitch, int x0, int y0,

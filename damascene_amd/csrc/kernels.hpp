@@ -24,7 +24,7 @@ int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp,
 int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, int* labels, float* pdot,
                         void* state, int max_iter, void* work);
 size_t kmeans_mixed_work_bytes(int n);
-int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations);
+int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations, int* rechecks = nullptr);
 const int* kmeans_iterations_ptr(const void* state);  // device address of the iteration count
 
 // ------------------------------------------------------------- local cues
@@ -100,6 +100,7 @@ extern int lanczos_mixed_fused;
 // Tile of the tiled fused mixed step, TW * 100 + TH: 6404 (default), 6408,
 // 6402, 3208 or 3204.
 extern int lanczos_tile;
+extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25)
 extern int batch_lanes;
 
 // ------------------------------------------------------------- eigensolver
@@ -115,6 +116,9 @@ struct StepShape {
   int fused, tile, k1;
   float ms;
 };
+// Tiles of the tiled fused mixed step (width * 100 + height); the first is
+// the default.
+constexpr int kStepTiles[] = {3208, 6404, 1616, 3216, 1608, 6408, 6402, 3204};
 int autotune_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, int steps, StepShape* best,
                    StepShape* tried, int max_tried, int* n_tried);
 void autotune_clear();

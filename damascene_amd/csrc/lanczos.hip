@@ -52,7 +52,8 @@ namespace dmsc {
 int lanczos_trace = 0;
 int lanczos_k1_threads = 0;
 int lanczos_mixed_fused = 2;
-int lanczos_tile = 6404;
+int lanczos_tile = 3208;
+int lanczos_max_iter = kMaxIter;
 
 namespace {
 
@@ -112,6 +113,33 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int w = 1; w < NT / kWave; w++) t += red[w];
   __syncthreads();
   return t;
+}
+
+// Three block sums with one pair of barriers; each value is summed in
+// block_sum's order, so the results are bit-identical to three block_sum
+// calls.  red holds 3 * NT / kWave doubles.
+template <int NT>
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* red) {
+  constexpr int NW = NT / kWave;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[threadIdx.x / kWave] = a;
+    red[NW + threadIdx.x / kWave] = b;
+    red[2 * NW + threadIdx.x / kWave] = c;
+  }
+  __syncthreads();
+  a = red[0];
+  b = red[NW];
+  c = red[2 * NW];
+#pragma unroll
+  for (int w = 1; w < NW; w++) {
+    a += red[w];
+    b += red[NW + w];
+    c += red[2 * NW + w];
+  }
+  __syncthreads();
 }
 
 // Sum of nb partials, identical in every block (fixed assignment and order).
@@ -317,7 +345,7 @@ __global__ __launch_bounds__(kK2) void k_lzm2(LzmBufs b, int t) {
 template <int NT>
 __global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t) {
   extern __shared__ float sq[];
-  __shared__ double red[NT / kWave];
+  __shared__ double red[3 * NT / kWave];
   const int k = b.st->k + t;
   float* V = b.st->V;
   const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
@@ -345,8 +373,8 @@ __global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t
     v2 += b.p2[po + i];
     v3 += b.p3[po + i];
   }
-  const double alpha = block_sum<NT>(v1, red);
-  const double s2 = block_sum<NT>(v2, red), s3 = block_sum<NT>(v3, red);
+  block_sum3<NT>(v1, v2, v3, red);
+  const double alpha = v1, s2 = v2, s3 = v3;
   const double beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
   if (k > 0 && lb == 0 && threadIdx.x == 0) {
     b.alpha[k - 1] = alpha;
@@ -393,8 +421,8 @@ __global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t
 
 // Tiled fused mixed step: as k_lzmf, but each block owns a TW x TH tile of
 // image pixels instead of a run of consecutive rows, so the staged basis
-// values (tile plus a 5-pixel frame, (TW + 10) x (TH + 10)) are 4x the rows
-// for 64 x 4 tiles instead of 10x for 512 consecutive rows.  A staged slot
+// values (tile plus a 5-pixel frame, (TW + 10) x (TH + 10)) are 3x the rows
+// for 32 x 8 tiles instead of 10x for 512 consecutive rows.  A staged slot
 // (sy, sx) holds the basis value at linear index (ty0 - 5 + sy) * w +
 // (tx0 - 5 + sx) -- the reference's linear indexing, including its wrap
 // across image rows at the left and right edges -- or 0 outside [0, n), so
@@ -404,7 +432,7 @@ template <int TW, int TH>
 __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, int w, int h, int t) {
   constexpr int NT = TW * TH, SW = TW + 2 * kStencilRadius, SH = TH + 2 * kStencilRadius;
   __shared__ float sq[SW * SH];
-  __shared__ double red[NT / kWave];
+  __shared__ double red[3 * NT / kWave];
   const int k = b.st->k + t;
   float* V = b.st->V;
   const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
@@ -436,8 +464,8 @@ __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, 
     v2 += b.p2[po + i];
     v3 += b.p3[po + i];
   }
-  const double alpha = block_sum<NT>(v1, red);
-  const double s2 = block_sum<NT>(v2, red), s3 = block_sum<NT>(v3, red);
+  block_sum3<NT>(v1, v2, v3, red);
+  const double alpha = v1, s2 = v2, s3 = v3;
   const double beta = sqrt(fmax(fma(alpha * alpha, s3 - 2.0, s2), 0.0));
   if (k > 0 && lb == 0 && threadIdx.x == 0) {
     b.alpha[k - 1] = alpha;
@@ -470,9 +498,7 @@ __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, 
     d2 = zj * zj;
     d3 = qv * qv;
   }
-  d1 = block_sum<NT>(d1, red);
-  d2 = block_sum<NT>(d2, red);
-  d3 = block_sum<NT>(d3, red);
+  block_sum3<NT>(d1, d2, d3, red);
   if (threadIdx.x == 0) {
     b.p1[pc + lb] = d1;
     b.p2[pc + lb] = d2;
@@ -909,10 +935,10 @@ __global__ void k_set_m(LzState* st, int m) { st->m = m; }
 struct GraphKey {
   const void* A;
   const void* st;
-  int pitch, n, w, mixed, steps, k1, fused;
+  int pitch, n, w, mixed, steps, k1, fused_lds, tile;
   bool operator<(const GraphKey& o) const {
-    return std::tie(A, st, pitch, n, w, mixed, steps, k1, fused) <
-           std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps, o.k1, o.fused);
+    return std::tie(A, st, pitch, n, w, mixed, steps, k1, fused_lds, tile) <
+           std::tie(o.A, o.st, o.pitch, o.n, o.w, o.mixed, o.steps, o.k1, o.fused_lds, o.tile);
   }
 };
 
@@ -982,12 +1008,12 @@ int add_steps(hipGraph_t g, const Steps& p, int steps) {
     if (p.mixed && p.tile_h) {
       const int code = p.tile_w * 100 + p.tile_h;
       const void* fn = nullptr;
-      if (code == 6404) fn = (const void*)&k_lzmt<64, 4>;
-      else if (code == 6408) fn = (const void*)&k_lzmt<64, 8>;
-      else if (code == 6402) fn = (const void*)&k_lzmt<64, 2>;
-      else if (code == 3208) fn = (const void*)&k_lzmt<32, 8>;
-      else if (code == 3204) fn = (const void*)&k_lzmt<32, 4>;
-      else return fail(DMSC_ERR_INTERNAL, "no tiled step kernel for tile code %d", code);
+#define TILE_FN(TW, TH) \
+  if (code == TW * 100 + TH) fn = (const void*)&k_lzmt<TW, TH>;
+      TILE_FN(32, 8) TILE_FN(64, 4) TILE_FN(16, 16) TILE_FN(32, 16) TILE_FN(16, 8) TILE_FN(64, 8)
+      TILE_FN(64, 2) TILE_FN(32, 4)
+#undef TILE_FN
+      if (!fn) return fail(DMSC_ERR_INTERNAL, "no tiled step kernel for tile code %d", code);
       DMSC_TRY(add_node(g, &last, fn, dim3(p.mix.nb1), dim3(p.tile_w * p.tile_h), 0, p.mix, p.so, p.w, p.h, t));
     } else if (p.mixed && p.fused_lds) {
       const void* fn = p.k1 == 512 ? (const void*)&k_lzmf<512> : (const void*)&k_lzmf<256>;
@@ -1010,7 +1036,7 @@ int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps
   GraphCache* gc = graph_cache(ctx);
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
-                     steps, p.k1, (int)p.fused_lds + 1000000 * (p.tile_w * 100 + p.tile_h)};
+                     steps, p.k1, (int)p.fused_lds, p.tile_w * 100 + p.tile_h};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
@@ -1090,7 +1116,7 @@ StepShape default_shape(int w, int h, bool mixed) {
     auto it = tuned_shapes().find(std::make_tuple(w, h, mixed ? 1 : 0));
     if (it != tuned_shapes().end()) return it->second;
   }
-  return StepShape{mixed ? lanczos_mixed_fused : 0, lanczos_tile, lanczos_k1_threads};
+  return StepShape{mixed ? lanczos_mixed_fused : 0, lanczos_tile, lanczos_k1_threads, 0.f};
 }
 
 // time_steps > 0: run that many steps with no checks between two HIP
@@ -1282,9 +1308,11 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const LzState& hs = *hsp;
   int ahead = 0;  // steps already enqueued beyond `done`
   bool converged = false;
-  while (done < kMaxIter) {
+  // iteration cap: lanczos.cu:4 MAXITER, lowered by the "lanczos_max_iter" tuning value
+  const int max_it = std::min(kMaxIter, lanczos_max_iter);
+  while (done < max_it) {
     // the next check comes after max(period, ahead) steps
-    const int steps = std::min(std::max(period, ahead), kMaxIter - done);
+    const int steps = std::min(std::max(period, ahead), max_it - done);
     const int spec = mixed ? kSpecSteps : 0;
     if ((size_t)done + steps + spec + 2 > cap) {
       // grow the basis, keeping the zero row and every row already written:
@@ -1342,7 +1370,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       if (mixed) {
         DMSC_HIP(hipEventRecord(ev_chk, cs));
         if (ahead == 0) {  // steps past the check, overlapped with it
-          ahead = std::min(kSpecSteps, kMaxIter - done);
+          ahead = std::min(kSpecSteps, max_it - done);
           for (int t = 0; t < ahead; t += kGraphSteps)
             DMSC_TRY(run_steps(ctx, p, st, w, done + t, std::min(kGraphSteps, ahead - t)));
         }
@@ -1380,7 +1408,12 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     }
     if (mixed) period = next_period(done, hs, nvec, stop_tol, prev_m, prev_res);
   }
-  if (!converged) return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", kMaxIter);
+  // lanczos.cu:519-560: the reference checks once more at MAXITER - 1 and,
+  // converged or not, builds the Ritz vectors of that check; so does this
+  // solver when the last check screened nvec values (an iteration count at
+  // the cap tells the caller the tolerance was not reached)
+  if (!converged && !(hs.fail == 0 && hs.nscreened >= nvec && hs.m >= max_it - kGraphSteps))
+    return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", max_it);
   const int m = hs.m;
 
   // Ritz vectors, D^-1/2 scaling (stencilMVM.cu:182-198), min-max normalisation
@@ -1426,11 +1459,11 @@ int autotune_steps(Ctx* ctx, int w, int h, float* A, int pitch, int precision, i
   const bool mixed = precision == DMSC_PREC_MIXED;
   std::vector<StepShape> cand;
   if (mixed) {
-    for (int tile : {6404, 6408, 6402, 3208, 3204}) cand.push_back(StepShape{2, tile, 0});
-    for (int k1 : {512, 256}) cand.push_back(StepShape{1, 6404, k1});
-    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 6404, k1});
+    for (int tile : kStepTiles) cand.push_back(StepShape{2, tile, 0, 0.f});
+    for (int k1 : {512, 256}) cand.push_back(StepShape{1, 0, k1, 0.f});
+    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 0, k1, 0.f});
   } else {
-    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 6404, k1});
+    for (int k1 : {256, 64}) cand.push_back(StepShape{0, 0, k1, 0.f});
   }
   int nt = 0;
   StepShape top{};

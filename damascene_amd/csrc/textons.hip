@@ -83,7 +83,8 @@ struct KmState {
   unsigned int rechecks;    // mixed path: pixels re-labelled by the exact FP32 distances
   float cent[kNFilters][kMaxK];  // centroids of the current iteration (FP32), [filter][cluster]
   float cdot[kMaxK];             // |c|^2 in the reference order
-  alignas(16) uint16_t centb[kMaxK][32];  // mixed path: BF16 centroids, filters 0..31, [cluster][filter]
+  alignas(16) uint16_t centb[kMaxK][32];   // mixed path: BF16 centroids, filters 0..31, [cluster][filter]
+  alignas(16) uint16_t centbl[kMaxK][32];  // mixed path: BF16 of (centroid - its BF16), the low part
 };
 
 // kmeans.cu:14-35 assignInitialClusters (6x6 grid capped at 31) for K = 32,
@@ -163,6 +164,12 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {  // round to nearest even
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+// x ~ hi + lo with hi = bf16(x), lo = bf16(x - hi) (the subtraction is exact)
+__device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+  hi = to_bf16(x);
+  lo = to_bf16(x - from_bf16(hi));
+}
 
 // kmeans.cu:268-286 finishCentroids(_64) (an empty cluster gives NaN, which
 // never wins a comparison) and makeSelfDots for the centroids; one block.
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(256) void k_km_centroids(int nf, KmState* st, int i
   }
   for (int i = threadIdx.x; i < K * 32; i += 256) {
     const int c = i / 32, f = i - c * 32;
-    st->centb[c][f] = to_bf16(st->cent[f][c]);
+    split_bf16(st->cent[f][c], st->centb[c][f], st->centbl[c][f]);
   }
   for (int i = threadIdx.x; i < kNFilters * K; i += 256) st->mass[buf ^ 1][i] = 0;
   if (threadIdx.x < K) st->count[buf ^ 1][threadIdx.x] = 0;
@@ -257,18 +264,28 @@ __global__ __launch_bounds__(256) void k_km_assign(int n, const float* __restric
 }
 
 // ---------------------------------------------------------------- mixed path
-// BF16 copy of filters 0..31 of every pixel, [pixel][filter] (the MFMA B
-// operand); filter 32 stays FP32 and filter 33's response is zero.
+// Split BF16 copy of filters 0..31 of every pixel, [pixel][filter] (the MFMA
+// B operands): xb[p] the BF16 of the response, xb[n + p] the BF16 of the
+// remainder; filter 32 stays FP32 and filter 33's response is zero.
 __global__ void k_km_bf16(int n, const float* __restrict__ resp, uint16_t* __restrict__ xb) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  uint32_t pk[16];
+  uint32_t hi[16], lo[16];
 #pragma unroll
-  for (int f = 0; f < 32; f += 2)
-    pk[f / 2] = (uint32_t)to_bf16(resp[(size_t)f * n + p]) | ((uint32_t)to_bf16(resp[(size_t)(f + 1) * n + p]) << 16);
-  uint4* dst = reinterpret_cast<uint4*>(xb + (size_t)p * 32);
+  for (int f = 0; f < 32; f += 2) {
+    uint16_t h0, l0, h1, l1;
+    split_bf16(resp[(size_t)f * n + p], h0, l0);
+    split_bf16(resp[(size_t)(f + 1) * n + p], h1, l1);
+    hi[f / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    lo[f / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+  }
+  uint4* dh = reinterpret_cast<uint4*>(xb + (size_t)p * 32);
+  uint4* dl = reinterpret_cast<uint4*>(xb + ((size_t)n + p) * 32);
 #pragma unroll
-  for (int q = 0; q < 4; q++) dst[q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+  for (int q = 0; q < 4; q++) {
+    dh[q] = make_uint4(hi[4 * q], hi[4 * q + 1], hi[4 * q + 2], hi[4 * q + 3]);
+    dl[q] = make_uint4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
+  }
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -276,12 +293,17 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Distances from v_mfma_f32_16x16x32_bf16 products (clusters as rows, in
 // K/16 tiles of 16, pixels as columns, filters 0..31 as K) plus the FP32
-// filter-32 term.  Per (pixel, cluster) the approximation error against the
-// exact FP32 distance is below e = 2^-5 |x| |c| + 2^-20 (|x|^2 + |c|^2):
-// BF16 rounding of both operands perturbs each product by at most (2^-7 +
-// 2^-16)|x_f c_f|, summing to (2^-7 + 2^-16)|x||c| by Cauchy-Schwarz, times
-// 2 for the -2 factor, and FP32 rounding of both computations stays inside
-// the second term.  The label of the smallest approximate distance is the
+// filter-32 term.  Both operands are split, x ~ x_hi + x_lo and c ~ c_hi +
+// c_lo (BF16 each), and x.c is taken as x_lo c_hi + x_hi c_lo + x_hi c_hi
+// (three MFMAs into one FP32 accumulator).  Per (pixel, cluster) the error
+// against the exact FP32 distance is below e = 2^-14 |x| |c| + 2^-20 (|x|^2
+// + |c|^2): the split leaves at most 3.01 * 2^-18 |x_f c_f| per product
+// (the dropped x_lo c_lo and the BF16 rounding of the low parts), the 96
+// FP32 additions of the MFMAs at most 2^-17.4 sum |x_f c_f| and the
+// reference-order FP32 dot product at most 2^-19 sum |x_f c_f|, so with
+// sum |x_f c_f| <= |x||c| (Cauchy-Schwarz) and the factor 2 of the
+// distance below 2^-14.7 |x||c|; FP32 rounding of |x|^2 + |c|^2 - 2 x.c in
+// both computations stays inside the second term.  The label of the smallest approximate distance is the
 // exact label when its upper bound lies below every other cluster's lower
 // bound (no tie is possible then); any other pixel is re-labelled with the
 // exact FP32 distances in the reference order, so labels are bit-exact.
@@ -295,37 +317,53 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
   constexpr int NT = K / 16;  // cluster tiles
   if (st->done) return;
   __shared__ KmCent<K> sc;
+  __shared__ float cnorm[K];  // |c|
   __shared__ int schanges, srecheck;
   km_stage<K>(sc, st);
+  if (threadIdx.x < K) cnorm[threadIdx.x] = sqrtf(st->cdot[threadIdx.x]);
   if (threadIdx.x == 0) schanges = srecheck = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
   // A operands: centroid rows (clusters) 16*t + col, filters 8*grp..+7
-  bf16x8 a[NT];
+  bf16x8 a[NT], al[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) {
     const uint4 v = *reinterpret_cast<const uint4*>(&st->centb[16 * t + col][8 * grp]);
     __builtin_memcpy(&a[t], &v, 16);
+    const uint4 vl = *reinterpret_cast<const uint4*>(&st->centbl[16 * t + col][8 * grp]);
+    __builtin_memcpy(&al[t], &vl, 16);
   }
   int changed = 0, rech = 0;
-  // each wave labels 64 pixels, 16 per MFMA column tile
+  // each wave labels 64 pixels, 16 per MFMA column tile; every tile's
+  // operands are loaded before the first tile is computed
+  uint4 bv[4], blv[4];
+  float pdv[4], x32v[4];
+#pragma unroll
+  for (int tile = 0; tile < 4; tile++) {
+    const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
+    const int pp = p < n ? p : n - 1;
+    bv[tile] = *reinterpret_cast<const uint4*>(xb + (size_t)pp * 32 + 8 * grp);
+    blv[tile] = *reinterpret_cast<const uint4*>(xb + ((size_t)n + pp) * 32 + 8 * grp);
+    pdv[tile] = pdot[pp];
+    x32v[tile] = resp[(size_t)32 * n + pp];
+  }
+#pragma unroll
   for (int tile = 0; tile < 4; tile++) {
     const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
     const bool live = p < n;
-    const int pp = live ? p : n - 1;
-    bf16x8 b;
-    {
-      const uint4 v = *reinterpret_cast<const uint4*>(xb + (size_t)pp * 32 + 8 * grp);
-      __builtin_memcpy(&b, &v, 16);
-    }
+    bf16x8 b, bl;
+    __builtin_memcpy(&b, &bv[tile], 16);
+    __builtin_memcpy(&bl, &blv[tile], 16);
     f32x4 s[NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
+    for (int t = 0; t < NT; t++) {  // clusters 16t..16t+15; small terms first
       s[t] = f32x4{0, 0, 0, 0};
-      s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, s[t], 0, 0, 0);  // clusters 16t..16t+15
+      s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bl, s[t], 0, 0, 0);
+      s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], b, s[t], 0, 0, 0);
+      s[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, s[t], 0, 0, 0);
     }
-    const float pd = pdot[pp], x32 = resp[(size_t)32 * n + pp];
+    const float pd = pdv[tile], x32 = x32v[tile];
     const float xn = sqrtf(pd);
     // this lane's 4*NT clusters, in increasing order: 16t + 4*grp + i
     float dt[4 * NT];
@@ -352,14 +390,14 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
 #pragma unroll
     for (int i = 0; i < 4 * NT; i++)
       if (ci[i] != bc) {
-        const float e = 0.03125f * xn * sqrtf(sc.cdot[ci[i]]) + 9.5367431640625e-7f * (pd + sc.cdot[ci[i]]);
+        const float e = 6.103515625e-05f * xn * cnorm[ci[i]] + 9.5367431640625e-7f * (pd + sc.cdot[ci[i]]);
         lo_other = fminf(lo_other, dt[i] - e);
       }
 #pragma unroll
     for (int o = 16; o <= 32; o <<= 1) lo_other = fminf(lo_other, __shfl_xor(lo_other, o, 64));
     bool certain = false;
     if (bc < K) {
-      const float e = 0.03125f * xn * sqrtf(sc.cdot[bc]) + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
+      const float e = 6.103515625e-05f * xn * cnorm[bc] + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
       certain = best + e < lo_other && best + e < 10000000.0f;
     }
     if (grp == 0 && live) {
@@ -400,8 +438,8 @@ int launch_texton_conv(hipStream_t s, int w, int h, const float* grey, const flo
 
 size_t kmeans_state_bytes() { return sizeof(KmState); }
 
-// BF16 responses for the mixed path: 32 filters per pixel.
-size_t kmeans_mixed_work_bytes(int n) { return (size_t)n * 32 * sizeof(uint16_t) + 256; }
+// Split BF16 responses for the mixed path: 2 x 32 filters per pixel.
+size_t kmeans_mixed_work_bytes(int n) { return (size_t)n * 64 * sizeof(uint16_t) + 256; }
 
 namespace {
 template <int K>
@@ -453,13 +491,14 @@ const int* kmeans_iterations_ptr(const void* state) {
   return reinterpret_cast<const int*>(&static_cast<const KmState*>(state)->iterations);
 }
 
-int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations) {
+int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations, int* rechecks) {
   KmState h;
   DMSC_HIP(hipMemcpyAsync(&h, state, sizeof(KmState), hipMemcpyDeviceToHost, s));
   DMSC_HIP(hipStreamSynchronize(s));
   const int last = (int)h.iterations - 1;
   if (iterations) *iterations = (int)h.iterations;
   if (changes) *changes = last >= 0 ? h.changes[last & 1] : 0;
+  if (rechecks) *rechecks = (int)h.rechecks;
   return DMSC_OK;
 }
 

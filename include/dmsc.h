@@ -32,7 +32,7 @@ enum {
   DMSC_OK = 0,
   DMSC_ERR_ARG = 1,      /* invalid argument (sizes, null pointers, ranges) */
   DMSC_ERR_HIP = 2,      /* HIP runtime error */
-  DMSC_ERR_NOCONV = 3,   /* eigensolver reached the iteration cap */
+  DMSC_ERR_NOCONV = 3,   /* eigensolver screening failed, or reached the cap without nvec screened values */
   DMSC_ERR_NODEV = 4,    /* no usable GPU */
   DMSC_ERR_INTERNAL = 5
 };
@@ -112,6 +112,11 @@ int dmsc_texton_responses(dmsc_ctx* ctx, int w, int h, const float* d_grey, floa
 int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* d_resp,
                 int* d_labels, int max_iter, int conv_thresh, int precision, int* changes,
                 int* iterations);
+/* No reference counterpart: after a DMSC_PREC_MIXED dmsc_kmeans call on this
+ * context, the number of pixel assignments (summed over iterations) whose
+ * BF16 MFMA distances could not certify the winner and were recomputed with
+ * the exact FP32 distances. */
+int dmsc_kmeans_rechecks(dmsc_ctx* ctx, int* rechecks);
 /* texton.h:16 findTextons (texton_choice 1 = 32 textons, 2 = 64). */
 int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* d_grey, int* d_textons,
                       int texton_choice, int precision, int* iterations);
@@ -202,7 +207,14 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *                         or runs of consecutive rows (1, when the stencil
  *                         halo fits), or as two kernels (0).
  *   "lanczos_tile":       tile of the tiled fused step, width * 100 + height:
- *                         6404 (default), 6408, 6402, 3208, 3204.
+ *                         3208 (default), 6404, 1616, 3216, 1608, 6408,
+ *                         6402, 3204.
+ *   "batch_lanes":        default lane count of dmsc_gpb_batch (1-16, 4).
+ *   "lanczos_max_iter":   Lanczos iteration cap, a multiple of 25 up to 6000
+ *                         (lanczos.cu:4 MAXITER, the default).  At the cap
+ *                         the eigensolver returns the Ritz pairs of its last
+ *                         check, as the reference does, with the iteration
+ *                         count equal to the cap.
  *   "lanczos_trace":      non-zero prints one line per eigensolver check to
  *                         stderr (2: also the Ritz values of a failed
  *                         screening).
@@ -218,7 +230,8 @@ int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int ma
 /* Autotuner (no reference counterpart; the reference's launch shapes are
  * fixed).  dmsc_autotune_eigensolver converts d_matrix in place once, then
  * times every step-kernel shape of the given precision -- mixed: tiled
- * fused steps with tiles 64x4, 64x8, 64x2, 32x8, 32x4, linear fused steps
+ * fused steps with tiles 32x8, 64x4, 16x16, 32x16, 16x8, 64x8, 64x2, 32x4,
+ * linear fused steps
  * of 512 and 256 rows per block, two-kernel steps of 256 and 64; plain:
  * two-kernel steps of 256 and 64 -- with `steps` (a multiple of 25) timed
  * steps each, and records the fastest as the shape every later eigensolve

@@ -48,6 +48,7 @@ void orc_gamma_lut(float gamma, float* lut /*256*/);
 
 /* ---- deterministic math shared by contract with the GPU ---- */
 float orc_cbrtf(float x);
+float orc_expf(float x);  /* project exp (basic IEEE operations), bit-identical on CPU and GPU */
 
 /* ---- stages ---- */
 void orc_rgb_to_grey(int w, int h, const uint32_t* rgb, float* grey);

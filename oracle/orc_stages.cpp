@@ -507,13 +507,13 @@ struct IcCtx {
 inline float c_ic_ss(float ic) {  // intervening.cu:32-42
   float val = 1.8682f;
   val += (ic / 0.3130f) * -1.3113f;
-  const float post = 1.0f / (1.0f + expf(-val));
+  const float post = 1.0f / (1.0f + orc_expf(-val));
   if (!std::isfinite(post)) return 0;
   if (post < 0) return 0;
   if (post > 1) return 1;
   return post;
 }
-inline float affinity(float c, float rs) { return expf(-(1 - c) * rs); }  // 28-30
+inline float affinity(float c, float rs) { return orc_expf(-(1 - c) * rs); }  // 28-30
 
 void ic_walk(const IcCtx& C, int x0, int y0, int x1, int y1, int x2, int y2, int x3, int y3) {
   const int dx1 = x1 - x0, dy1 = y1 - y0, dx2 = x2 - x0, dy2 = y2 - y0, dx3 = x3 - x0, dy3 = y3 - y0;

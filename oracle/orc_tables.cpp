@@ -495,4 +495,33 @@ float orc_cbrtf(float xf) {
   return (float)(x < 0 ? -r : r);
 }
 
+// exp of a float argument from basic IEEE double operations, identical in
+// the kernels (dev::exp_exact): Cody-Waite reduction x = n ln2 + r with n the
+// nearest integer to x / ln2 and ln2 split in a high part with 21 trailing
+// zero bits and a low part (the split of fdlibm's e_exp.c), exp(r) by the
+// degree-12 Taylor polynomial (|r| <= ln2 / 2: remainder below 2e-16
+// relative) in Horner form with fused multiply-adds, times 2^n, rounded once
+// to float.  It stands in for the reference's device expf in the affinity
+// and its logistic (intervening.cu:28-42): a deviation of at most one float
+// rounding, which makes the affinity matrix bit-identical between the
+// kernels and the oracle.
+float orc_expf(float xf) {
+  if (xf != xf) return xf;
+  if (xf > 89.0f) return INFINITY;
+  if (xf < -104.0f) return 0.0f;
+  const double x = (double)xf;
+  const double n = std::nearbyint(x * 1.4426950408889634);
+  double r = std::fma(-n, 6.93147180369123816490e-01, x);
+  r = std::fma(-n, 1.90821492927058770002e-10, r);
+  static const double inv_fact[13] = {1.0,         1.0,          1.0 / 2,        1.0 / 6,         1.0 / 24,
+                                      1.0 / 120,   1.0 / 720,    1.0 / 5040,     1.0 / 40320,     1.0 / 362880,
+                                      1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600};
+  double p = inv_fact[12];
+  for (int k = 11; k >= 0; k--) p = std::fma(p, r, inv_fact[k]);
+  const uint64_t sb = (uint64_t)((int64_t)n + 1023) << 52;
+  double scale;
+  memcpy(&scale, &sb, 8);
+  return (float)(p * scale);
+}
+
 }  // extern "C"

@@ -101,6 +101,13 @@ def cbrtf(x):
     return np.array([o.orc_cbrtf(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
 
 
+def expf(x):
+    o = lib()
+    o.orc_expf.restype = ctypes.c_float
+    o.orc_expf.argtypes = [ctypes.c_float]
+    return np.array([o.orc_expf(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
+
+
 # ------------------------------------------------------------------ stages
 def rgb_to_grey(rgb):
     h, w = rgb.shape

@@ -1,8 +1,8 @@
 """Step-kernel shapes and the autotuner.
 
 Every Lanczos step shape the autotuner may pick (mixed: tiled fused with
-five tiles, linear fused with 512 or 256 rows per block, two-kernel with 256
-or 64; plain: two-kernel with 256 or 64) must give the default shape's
+eight tiles, linear fused with 512 or 256 rows per block, two-kernel with
+256 or 64; plain: two-kernel with 256 or 64) must give the default shape's
 results up to the rounding of its block partial sums: the pipeline up to the
 matrix is shape-independent and bit-exact, and eigenvalues and gPb agree
 within the tolerances below (the gPb bar is 1e-3).  Then the autotuner must
@@ -20,9 +20,10 @@ from damascene_amd._lib import check, lib
 pytestmark = pytest.mark.gpu
 
 W, H, CELLS, SEED = 160, 120, 12, 20261202
-MIXED_SHAPES = [(2, 6404, 0), (2, 6408, 0), (2, 6402, 0), (2, 3208, 0), (2, 3204, 0), (1, 6404, 512),
-                (1, 6404, 256), (0, 6404, 256), (0, 6404, 64)]
-PLAIN_SHAPES = [(0, 6404, 256), (0, 6404, 64)]
+TILES = [3208, 6404, 1616, 3216, 1608, 6408, 6402, 3204]
+# (mixed step form, tile, block size)
+MIXED_SHAPES = [(2, t, 0) for t in TILES] + [(1, 3208, 512), (1, 3208, 256), (0, 3208, 256), (0, 3208, 64)]
+PLAIN_SHAPES = [(0, 3208, 256), (0, 3208, 64)]
 # eigenvalue relative and gPb max-abs agreement with the default shape
 TOL = {dm.PREC_MIXED: (1e-5, 1e-3), dm.PREC_FP32: (1e-3, 1e-3)}
 
@@ -42,7 +43,7 @@ def image(ctx):
 @pytest.fixture(scope="module")
 def defaults(ctx, image):
     check(lib().dmsc_autotune_clear())
-    set_shape(2, 6404, 0)
+    set_shape(2, 3208, 0)
     return {p: ctx.gpb(image, precision=p) for p in (dm.PREC_FP32, dm.PREC_MIXED)}
 
 
@@ -67,7 +68,7 @@ def test_step_shape_matches_default(ctx, image, defaults, precision, shape):
     try:
         out = ctx.gpb(image, precision=precision)
     finally:
-        set_shape(2, 6404, 0)
+        set_shape(2, 3208, 0)
     compare(out, defaults[precision], precision, f"shape {shape}")
 
 
@@ -78,7 +79,7 @@ def test_autotune_picks_and_applies(ctx, image, defaults, precision):
     best, trials = ctx.autotune_eigensolver(W, H, m, precision=precision, steps=100)
     print("trials", [(t["mixed_fused"], t["tile"], t["k1_threads"], round(t["ms_per_step"] * 1e3, 2))
                      for t in trials], "best", best, flush=True)
-    assert len(trials) == (9 if precision == dm.PREC_MIXED else 2)
+    assert len(trials) == (len(TILES) + 4 if precision == dm.PREC_MIXED else 2)
     assert all(t["ms_per_step"] > 0 for t in trials)
     assert best["ms_per_step"] == min(t["ms_per_step"] for t in trials)
     try:

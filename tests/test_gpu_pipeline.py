@@ -165,3 +165,36 @@ def test_eigensolve_481x321_vs_truth(ctx, precision):
           f"subspace angle {ang:.3e}", flush=True)
     assert rel.max() < 1e-4
     assert ang < 1e-4
+
+
+@pytest.mark.parametrize("precision,cap", [(dm.PREC_MIXED, 200), (dm.PREC_FP32, 250)])
+def test_iteration_cap_returns_last_check(ctx, case, precision, cap):
+    """lanczos.cu:519-560: at MAXITER the reference checks once more and uses
+    that check's Ritz pairs whether or not they converged; a lowered cap
+    ("lanczos_max_iter") takes the same path: status OK, iteration count =
+    the cap, and the last check's values, which by Cauchy interlacing bound
+    the converged eigenvalues from above (the j-th smallest screened Ritz
+    value is at least the j-th smallest Ritz value, itself at least the j-th
+    eigenvalue)."""
+    import ctypes
+
+    from damascene_amd._lib import check, lib
+
+    W, H, rgb, ref, truth = case
+    print(f"case: iteration cap {cap}, precision {precision}", flush=True)
+    full = ctx.gpb(ctx.upload(rgb), precision=precision)
+    assert full["lanczos_iterations"] > cap
+    check(lib().dmsc_tune_set(b"lanczos_max_iter", cap))
+    try:
+        out = ctx.gpb(ctx.upload(rgb), precision=precision)
+    finally:
+        check(lib().dmsc_tune_set(b"lanczos_max_iter", 6000))
+    v = ctypes.c_int()
+    check(lib().dmsc_tune_get(b"lanczos_max_iter", ctypes.byref(v)))
+    assert v.value == 6000
+    assert out["lanczos_iterations"] == cap
+    ev = out["eigvals"]
+    assert np.all(np.isfinite(ev)) and np.all(np.isfinite(host(out["gpb"])))
+    print(f"capped eigenvalues {ev} vs converged {full['eigvals']}", flush=True)
+    assert np.all(ev[1:] >= full["eigvals"][1:] * (1 - 1e-4))
+    assert np.all(np.diff(ev) >= 0)

@@ -8,11 +8,14 @@ transform, SpMV, spectralPb, gPb) must match bit for bit.  The intervening
 contour uses the device expf: its matrix is compared with a relative
 tolerance of 1e-6 (1-ulp class differences of expf).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
 import damascene_amd as dm
+from damascene_amd._lib import check, lib
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -84,9 +87,15 @@ def test_kmeans_labels_mixed(ctx, responses):
     """BF16 MFMA distances with the certified-margin test and FP32 re-check
     of near ties: the same labels and iteration count as the exact path."""
     labels, it = ctx.kmeans(responses, precision=dm.PREC_MIXED)
+    rech = ctypes.c_int()
+    check(lib().dmsc_kmeans_rechecks(ctx.handle, ctypes.byref(rech)))
+    n = labels.numel()
+    print(f"mixed k-means: {it} iterations, {rech.value} exact re-checks of {it * n} assignments "
+          f"({rech.value / (it * n):.2%})", flush=True)
     ref, rit = O.kmeans(host(responses))
     assert it == rit, f"iterations gpu {it} oracle {rit}"
     np.testing.assert_array_equal(host(labels), ref)
+    assert 0 <= rech.value <= it * n
 
 
 @pytest.fixture(scope="module")
@@ -145,7 +154,7 @@ def test_intervene_convert_spmv(ctx, combined):
     n = W * H
     ref = O.intervene(host(nm))
     got = host(m)[:, :n]
-    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+    assert_bitexact(got, ref, "affinity matrix")
     # transform and SpMV from the GPU's own matrix: bit-exact
     src = got.copy()
     d, rs = ctx.convert_matrix(W, H, m)

@@ -138,6 +138,19 @@ def test_gamma_lut_and_cbrt():
     assert ulp[x > 0].max() <= 1.0
 
 
+def test_project_exp_is_correctly_rounded():
+    # the affinity's exp (intervening.cu:28-42) is the project's exp from
+    # basic IEEE operations (the kernels run the same operations); over the
+    # arguments the affinity and its logistic take, and beyond, it equals
+    # exp in float64 rounded once to float32
+    x = np.concatenate([RNG.uniform(-20, 10, 50000).astype(np.float32),
+                        np.float32([0, -0.0, 1, -1, 88, -103, 1e-8, -1e-8, -10, 3.2])])
+    got = O.expf(x)
+    want = np.exp(x.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert O.expf([100.0])[0] == np.inf and O.expf([-200.0])[0] == 0.0
+
+
 def test_blur_kernel_is_normalised_gaussian():
     for s in (2.0, 1.0):
         k, r = O.blur_kernel(s)

@@ -24,6 +24,10 @@ over 64 images of uneven size (sides uniform in [200, 1024]), 4 concurrent
 lanes, both precisions; image 0 of the 16 is the single-image workload and
 is checked bit for bit against the single-image call.
 
+large: the 1024x768 configuration (120 cells): images/s of both precisions,
+the step and SpMV rooflines at that size, and FP64 a posteriori residuals of
+the eigenpairs (no stored truth at that size).
+
 differentiable: 200 Adam steps (lr 1e-2) refitting the combine weights on
 16 images of 481x321 through the hand-written spectral Pb + combine backward
 (loss before and after, time per forward + backward over the 16 images).
@@ -308,6 +312,83 @@ def differentiable(ctx, steps=200, images=16, lr=1e-2):
                     "pre-non-max gPb against cell boundaries dilated 1 px"}
 
 
+def fp64_residuals(ctx, out, w, h):
+    """A posteriori FP64 check of the returned eigenpairs, needing no stored
+    truth: u_j = D^1/2 v_j with the D^1/2 direction projected out (undoing
+    the per-vector min-max map), normalised; the Rayleigh quotient
+    lam_j = u_j' L u_j and the residual |L u_j - lam_j u_j| in FP64, with L
+    the library's L_sym of the same image (FP32 entries, exact in FP64) and
+    a plain DIA product over the 81 stencil offsets; and the Davis-Kahan bound
+    sin(angle_j) <= |r_j| / gap_j against the computed neighbours (the gap
+    of the last vector to the unknown next eigenvalue is not bounded)."""
+    import torch
+
+    n = w * h
+    m = ctx.intervene(out["nmax"])
+    _, rs = ctx.convert_matrix(w, h, m)
+    # diagonal k of the DIA matrix: the k-th point of the radius-5 disc in scan order
+    offs = [dy * w + dx for dy in range(-5, 6) for dx in range(-5, 6) if dy * dy + dx * dx <= 25]
+    pad = 5 * w + 5
+    A = m[:, :n].double()
+    sq = 1.0 / rs.reshape(-1).double()
+    u0 = sq / sq.norm()
+    V = out["eigvecs"].reshape(NVEC, n).double()[1:] * sq[None, :]
+    V -= (V @ u0)[:, None] * u0[None, :]
+    V /= V.norm(dim=1, keepdim=True)
+    xp = torch.zeros(V.shape[0], n + 2 * pad, dtype=torch.float64, device=V.device)
+    xp[:, pad:pad + n] = V
+    LV = torch.zeros_like(V)
+    for k in range(81):
+        LV += A[k][None, :] * xp[:, pad + int(offs[k]):pad + int(offs[k]) + n]
+    lam = (V * LV).sum(dim=1)
+    res = (LV - lam[:, None] * V).norm(dim=1)
+    lam, res = lam.cpu().numpy(), res.cpu().numpy()
+    got = np.asarray(out["eigvals"], np.float64)[1:]
+    gaps = np.array([np.min(np.abs(lam[j] - np.delete(lam, j))) for j in range(len(lam))])
+    return {"residual_max": float(res.max()), "residuals": [float(f"{r:.3e}") for r in res],
+            "eigval_vs_rayleigh_max_rel": float(np.max(np.abs(got - lam) / lam)),
+            "davis_kahan_sin_bound_max": float(np.max(res[:-1] / gaps[:-1]))}
+
+
+def large(ctx, steps):
+    """The 1024x768 configuration (120 cells, seed SEED): images/s of both
+    precisions, the step and SpMV rooflines at that size, and the FP64
+    a posteriori residuals of the mixed eigenpairs."""
+    import ctypes
+
+    import torch
+
+    import damascene_amd as dm
+    from damascene_amd._lib import check, lib
+
+    w, h = 1024, 768
+    d_rgb = ctx.upload(dm.generate_image(w, h, 120, SEED))
+    res = {"size": f"{w}x{h}", "cells": 120}
+    outs = {}
+    for name, prec in (("plain_fp32", dm.PREC_FP32), ("mixed", dm.PREC_MIXED)):
+        t, out = time_pipeline(ctx, d_rgb, prec, steps, 1)
+        outs[name] = out
+        res[name] = {"images_per_s": round(1.0 / t, 3), "ms_per_image": round(t * 1e3, 3),
+                     "lanczos_iterations": out["lanczos_iterations"]}
+    res["mixed"]["fp64_check"] = fp64_residuals(ctx, outs["mixed"], w, h)
+    res["plain_fp32"]["fp64_check"] = fp64_residuals(ctx, outs["plain_fp32"], w, h)
+    m = ctx.intervene(outs["mixed"]["nmax"])
+    ms = ctypes.c_float()
+    check(lib().dmsc_lanczos_time_steps(ctx.handle, w, h, m.data_ptr(), m.shape[1], dm.PREC_MIXED, STEP_TIMED,
+                                        ctypes.byref(ms)))
+    n = w * h
+    res["step"] = {"kernel": STEP_KERNEL, "avg_us": round(ms.value * 1e3, 3),
+                   "achieved_gbs": round(STEP_BYTES_PER_ROW * n / (ms.value * 1e-3) / 1e9, 1)}
+    res["step"]["frac"] = round(res["step"]["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    best, us = ctx.autotune_spmv(w, h, m, reps=100)
+    res["spmv"] = {"best_rows_per_lane": best, "avg_us": round(us[best], 3),
+                   "achieved_gbs": round(332 * n / (us[best] * 1e-6) / 1e9, 1)}
+    res["spmv"]["frac"] = round(res["spmv"]["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    check(lib().dmsc_tune_set(b"spmv_rows_per_lane", 1))
+    torch.cuda.synchronize()
+    return res
+
+
 def spmv_sweep(ctx, nmax, reps):
     """Average duration of each SpMV launch shape (HIP events on the library
     stream, the stream every dmsc call is enqueued on); leaves the fastest
@@ -422,6 +503,7 @@ def main():
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
     batch = None if a.no_batch else batched(ctx, out_plain, out_mixed, max(3, a.steps), a.warmup)
     diff = None if a.no_batch else differentiable(ctx)
+    big = None if a.no_batch else large(ctx, 2)
     spmv = spmv_sweep(ctx, out_plain["nmax"], a.spmv_reps)
     roof = step_roofline(ctx, out_plain["nmax"])
     torch.cuda.synchronize()
@@ -453,6 +535,9 @@ def main():
     }
     if diff:
         res["differentiable"] = diff
+    if big:
+        res["large"] = big
+    res["mixed"]["fp64_check"] = fp64_residuals(ctx, out_mixed, W, H)
     if batch:
         res["batched"] = batch
         res["batched"]["mixed_batched_vs_plain_single"] = round(batch["mixed"]["images_per_s"] * t_plain, 3)

@@ -80,9 +80,43 @@ def parse():
     p.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                    help="summarise two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE passes of "
                         "--pmc-workload) into HBM bytes per launch of --kernel, calibrated on the 256 MiB copy")
+    p.add_argument("--pmc-mfma", metavar="DIR",
+                   help="summarise a rocprofv3 --pmc SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_VALU_MFMA_BUSY_CYCLES "
+                        "GRBM_GUI_ACTIVE pass of --pmc-workload into MFMA FLOPs and utilisation of --kernel")
     p.add_argument("--kernel", default=STEP_KERNEL,
                    help="kernel for --pmc-summarize: the Lanczos step kernel or an SpMV shape name")
     return p.parse_args()
+
+
+def pmc_mfma(d, kernel):
+    """MFMA work and utilisation of `kernel` from one rocprofv3 pass with
+    SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE:
+    BF16 MFMA FLOPs = 512 per MOP; utilisation = busy cycles summed over the
+    SIMDs / (GPU-active cycles x 1024 SIMDs), rocprofv3's MfmaUtil."""
+    import csv
+    import glob
+
+    rows = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if not re.search(re.escape(kernel) + r"\(", r["Kernel_Name"]):
+                continue
+            e = rows.setdefault(r["Dispatch_Id"], {"ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"])})
+            c, v = r["Counter_Name"], float(r["Counter_Value"])
+            # SQ counters are summed over their instances, GRBM_GUI_ACTIVE is the
+            # largest instance (rocprofv3's MfmaUtil: reduce(GRBM_GUI_ACTIVE, max))
+            e[c] = max(e.get(c, 0.0), v) if c == "GRBM_GUI_ACTIVE" else e.get(c, 0.0) + v
+    if not rows:
+        raise SystemExit(f"no counter rows for {kernel} under {d}")
+    n = len(rows)
+    flops = sum(e["SQ_INSTS_VALU_MFMA_MOPS_BF16"] for e in rows.values()) * 512 / n
+    busy = sum(e["SQ_VALU_MFMA_BUSY_CYCLES"] for e in rows.values()) / n
+    active = sum(e["GRBM_GUI_ACTIVE"] for e in rows.values()) / n
+    ns = sum(e["ns"] for e in rows.values()) / n
+    print(f"{kernel} ({n} dispatches): BF16 MFMA {flops / 1e6:.1f} MFLOP per dispatch, "
+          f"MFMA busy {busy:.0f} SIMD-cycles, GPU active {active:.0f} cycles, "
+          f"MfmaUtil {100 * busy / (active * 1024):.3f} %, {ns / 1e3:.1f} us under the profiler "
+          f"({flops / ns / 1e3:.2f} TFLOP/s)")
 
 
 def pmc_summarize(fetch_dir, write_dir, kernel):
@@ -140,7 +174,8 @@ def pmc_workload(shape):
     """Counter-pass workload: a 256 MiB copy whose FETCH_SIZE/WRITE_SIZE
     calibrate the counters (256 MiB read + 256 MiB written), then 20 stencil
     SpMVs on the 481x321 L_sym with the given launch shape, then 125 fused
-    mixed Lanczos steps (25 warm-up + 100 timed) on the same matrix."""
+    mixed Lanczos steps (25 warm-up + 100 timed) on the same matrix, then one
+    mixed-precision k-means (BF16 MFMA distances) on the image's responses."""
     import torch
 
     import damascene_amd as dm
@@ -166,7 +201,12 @@ def pmc_workload(shape):
     check(lib().dmsc_lanczos_time_steps(ctx.handle, W, H, m2.data_ptr(), m2.shape[1], dm.PREC_MIXED, 100,
                                         ctypes.byref(ms)))
     torch.cuda.synchronize()
-    print(f"pmc workload done: copy 256 MiB, 20 x {SPMV_SHAPES[shape]}, 125 mixed Lanczos steps", flush=True)
+    d_rgb = ctx.upload(dm.generate_image(W, H, CELLS, SEED))
+    resp = ctx.texton_responses(ctx.rgb_to_grey(d_rgb))
+    ctx.kmeans(resp, precision=dm.PREC_MIXED)
+    torch.cuda.synchronize()
+    print(f"pmc workload done: copy 256 MiB, 20 x {SPMV_SHAPES[shape]}, 125 mixed Lanczos steps, "
+          "one mixed k-means", flush=True)
 
 
 def step_roofline(ctx, nmax):
@@ -489,6 +529,9 @@ def main():
         return
     if a.pmc_summarize:
         pmc_summarize(*a.pmc_summarize, a.kernel)
+        return
+    if a.pmc_mfma:
+        pmc_mfma(a.pmc_mfma, a.kernel)
         return
     import torch
 

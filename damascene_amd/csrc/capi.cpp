@@ -237,6 +237,16 @@ int dmsc_tune_set(const char* key, int value) {
     lanczos_tile = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "lanczos_mixed_stop")) {
+    DMSC_ARG(value >= 0 && value <= 6, "lanczos_mixed_stop must be in [0, 6]");
+    lanczos_mixed_stop = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_check_max")) {
+    DMSC_ARG(value == 0 || (value >= 25 && value % 25 == 0), "lanczos_check_max must be 0 or a multiple of 25");
+    lanczos_check_max = value;
+    return DMSC_OK;
+  }
   if (!std::strcmp(key, "lanczos_max_iter")) {
     DMSC_ARG(value >= 25 && value <= kMaxIter && value % 25 == 0, "lanczos_max_iter must be a multiple of 25 in [25, %d]",
              kMaxIter);
@@ -270,6 +280,14 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "lanczos_tile")) {
     *value = lanczos_tile;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_mixed_stop")) {
+    *value = lanczos_mixed_stop;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_check_max")) {
+    *value = lanczos_check_max;
     return DMSC_OK;
   }
   if (!std::strcmp(key, "lanczos_max_iter")) {

@@ -100,6 +100,8 @@ extern int lanczos_mixed_fused;
 // Tile of the tiled fused mixed step, TW * 100 + TH: 6404 (default), 6408,
 // 6402, 3208 or 3204.
 extern int lanczos_tile;
+extern int lanczos_check_max;  // mixed path: most steps between checks (0 = a quarter of the steps so far)
+extern int lanczos_mixed_stop;  // mixed path: stop when every residual is <= tol * 10^-this (default 4)
 extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25)
 extern int batch_lanes;
 

@@ -14,7 +14,10 @@
 //  MIXED: FP32 matrix and FP32-stored basis, FP64 SpMV accumulation,
 //    three-term vector and inner products; tested on an adaptive schedule
 //    (multiples of 25 steps) and stopped when every residual
-//    |beta_m s_{m,j}| is at most tol * 1e-3.
+//    |beta_m s_{m,j}| is at most tol * 1e-4 (the "lanczos_mixed_stop"
+//    exponent; at 481x321 a bound of tol * 1e-3 left one of three test
+//    images at a subspace error of 1.7e-4, tol * 1e-4 kept all three
+//    below 2.1e-5).
 //
 // A plain step is two kernels and no host round trip:
 //   K1: beta_{k-1} from the |z|^2 partials (every block reduces them in the
@@ -54,6 +57,8 @@ int lanczos_k1_threads = 0;
 int lanczos_mixed_fused = 2;
 int lanczos_tile = 3208;
 int lanczos_max_iter = kMaxIter;
+int lanczos_check_max = 0;
+int lanczos_mixed_stop = 4;
 
 namespace {
 
@@ -1095,6 +1100,7 @@ int next_period(int m, const LzState& hs, int nvec, double tol, int& prev_m, dou
     prev_res = r;
   }
   steps = std::min(std::max(steps, kGraphSteps), std::max(kGraphSteps, m / 4));
+  if (lanczos_check_max > 0) steps = std::min(steps, lanczos_check_max);
   return (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
 }
 
@@ -1176,7 +1182,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   const int nb1 = tile_h ? div_up(w, tile_w) * div_up(h, tile_h) : div_up(n, k1);
   const int nb2 = div_up(n, kK2 * kK2Rows);
   int period = mixed ? kGraphSteps : kCheckIter;  // the plain path checks every 500 (lanczos.cu:519)
-  const double stop_tol = mixed ? (double)tol * 1e-3 : (double)tol;
+  const double stop_tol = mixed ? (double)tol * std::pow(10.0, -lanczos_mixed_stop) : (double)tol;
   const int nblk = std::min(div_up(n, 256), 1024);
   const size_t zb = mixed ? sizeof(double) : sizeof(float);
 

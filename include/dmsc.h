@@ -210,6 +210,12 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *                         3208 (default), 6404, 1616, 3216, 1608, 6408,
  *                         6402, 3204.
  *   "batch_lanes":        default lane count of dmsc_gpb_batch (1-16, 4).
+ *   "lanczos_mixed_stop": the mixed eigensolver stops when every Ritz
+ *                         residual is at most tol * 10^-value (0-6, default
+ *                         4).
+ *   "lanczos_check_max":  the mixed eigensolver's most steps between two
+ *                         checks (0 = a quarter of the steps so far, the
+ *                         default; else a multiple of 25).
  *   "lanczos_max_iter":   Lanczos iteration cap, a multiple of 25 up to 6000
  *                         (lanczos.cu:4 MAXITER, the default).  At the cap
  *                         the eigensolver returns the Ritz pairs of its last

@@ -1,5 +1,6 @@
 """FP64 ground truth for the 481x321 workload (synthetic image: 40 Voronoi
-cells, seed 20261202), written to tests/golden/truth_481x321.npz.
+cells, seed 20261202), written to tests/golden/truth_481x321.npz; with a
+seed argument (e.g. 20261204) to tests/golden/truth_481x321_s<seed>.npz.
 
 The CPU oracle produces nmax, the affinity matrix and L_sym (bit-identical to
 the GPU path).  Then:
@@ -12,7 +13,7 @@ the GPU path).  Then:
 Saved: eigenvalues (10, FP64); u_1..u_8 (the non-trivial eigenvectors of
 L_sym, unit norm, FP32); sqrt(d) (row sums, FP32); gPb (FP32).
 
-    python tests/golden/make_truth.py      (about a minute on 8 cores)
+    python tests/golden/make_truth.py [seed]      (about 6 minutes on 8 cores)
 """
 import os
 import sys
@@ -61,11 +62,14 @@ def main():
     vecs = O.finish_eigvecs(W, H, rs, U[:, :NVEC].T.astype(np.float32).copy())
     spb = O.spectral_pb(ev[:NVEC].astype(np.float32), vecs, W, H)
     _, gpb = O.gpb(cg, spb, nm)
-    np.savez_compressed(os.path.join(HERE, "truth_481x321.npz"), eigvals=ev, u=U[:, 1:NVEC].T.astype(np.float32),
+    name = "truth_481x321.npz" if SEED == 20261202 else f"truth_481x321_s{SEED}.npz"
+    np.savez_compressed(os.path.join(HERE, name), eigvals=ev, u=U[:, 1:NVEC].T.astype(np.float32),
                         sqrt_d=np.sqrt(d.astype(np.float64)).astype(np.float32), gpb=gpb.astype(np.float32),
                         meta=np.array([W, H, CELLS, SEED, NVEC]))
     print(f"eigenvalues {ev}\nmax residual {res:.2e}; {time.time() - t0:.1f} s")
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        SEED = int(sys.argv[1])
     main()

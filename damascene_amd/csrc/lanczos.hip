@@ -736,8 +736,9 @@ __global__ __launch_bounds__(kWave) void k_tri_screen_mixed(LzState* st, const d
   st->fail = ns < nvec ? 1 : 0;
 }
 
-// Ritz vector j of T by inverse iteration (block j; lane 0 runs the
-// recurrences) and its residual |beta_m s_{m,j}| (lanczos.cu:114-140: FP32
+// Ritz vector j of T by inverse iteration (block j, one wave: lane 0 runs
+// the recurrences, the wave shares the passes without serial dependence)
+// and its residual |beta_m s_{m,j}| (lanczos.cu:114-140: FP32
 // on the plain path).  When they fit, alpha, beta and the work arrays are
 // staged in LDS; otherwise the work arrays live in global memory.
 // Plain path: vector j of the screened value st->vals[j], residual to
@@ -756,6 +757,7 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
   // one solve for a candidate (its residual and, if selected, its vector:
   // the other eigenvectors stay below eps |T| / gap), two for the plain path
   const int iters = cand ? 1 : kInvitIters;
+  __shared__ double nrm_bc[1];
   int bad;
   if (lds) {
     double* sa = dyn;
@@ -765,16 +767,15 @@ __global__ __launch_bounds__(kWave) void k_tri_vec(const double* __restrict__ al
       sb[i] = beta[i];
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
     double* wk = dyn + 2 * m;
-    bad = tri::invit(m, sa, sb, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
-                     (unsigned char*)(wk + 5 * m), iters);
+    bad = tri::invit_wave(m, sa, sb, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
+                          (unsigned char*)(wk + 5 * m), iters, nrm_bc);
   } else {
-    if (threadIdx.x != 0) return;
     double* wk = gwork + (size_t)j * 6 * m;
-    bad = tri::invit(m, alpha, beta, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
-                     (unsigned char*)(wk + 5 * m), iters);
+    bad = tri::invit_wave(m, alpha, beta, lam, sj, wk, wk + m, wk + 2 * m, wk + 3 * m, wk + 4 * m,
+                          (unsigned char*)(wk + 5 * m), iters, nrm_bc);
   }
+  if (threadIdx.x != 0) return;
   const double bm = beta[m - 1];
   const double r = mixed ? fabs(bm * sj[m - 1]) : (double)fabsf((float)bm * (float)sj[m - 1]);
   if (cand) {

@@ -255,5 +255,148 @@ __host__ __device__ inline int invit(int n, const double* __restrict__ a, const 
   return 0;
 }
 
+#ifdef __HIPCC__
+// invit on one 64-lane wave (blockDim.x == 64, every lane calls it): lane 0
+// runs the three recurrences that carry a serial dependence (factorisation,
+// forward and backward substitution); the passes with none (|T| row sums,
+// reciprocal pivots and start vector, normalisation, largest entry and
+// sign) are shared by the 64 lanes.  Every value is formed by the same
+// operations as in invit (the row-sum maximum and the first largest entry
+// do not depend on the order of the comparisons), so the results are
+// bit-identical to it.  All arrays must be visible to every lane (LDS or
+// global memory of this block).
+__device__ inline int invit_wave(int n, const double* __restrict__ a, const double* __restrict__ b, double lam,
+                                 double* __restrict__ v, double* __restrict__ dl, double* __restrict__ d,
+                                 double* __restrict__ du, double* __restrict__ du2, double* __restrict__ x,
+                                 unsigned char* __restrict__ piv, int iters, double* __restrict__ shared4) {
+  const int lane = threadIdx.x;
+  double tnorm = 0;
+  for (int i = lane; i < n; i += 64)
+    tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0));
+  for (int o = 32; o >= 1; o >>= 1) tnorm = fmax(tnorm, __shfl_xor(tnorm, o, 64));
+  const double eps = 2.2e-16 * fmax(tnorm, 1e-300);
+  if (lane == 0) {  // factorisation (dgttrf), as in invit
+    auto rowin = [&](int i, double& li, double& dn, double& un) {
+      li = b[i];
+      dn = a[i + 1] - lam;
+      un = i < n - 2 ? b[i + 1] : 0.0;
+    };
+    double di = a[0] - lam, ui = n > 1 ? b[0] : 0.0;
+    double l1 = 0, d1 = 0, u1 = 0, l2 = 0, d2 = 0, u2 = 0;
+    if (n > 1) rowin(0, l1, d1, u1);
+    if (n > 2) rowin(1, l2, d2, u2);
+    for (int i = 0; i < n - 1; i++) {
+      const double li = l1, dn = d1, un = u1;
+      l1 = l2; d1 = d2; u1 = u2;
+      if (i + 2 < n - 1) rowin(i + 2, l2, d2, u2);
+      if (fabs(di) >= fabs(li)) {
+        piv[i] = 0;
+        if (di == 0) di = eps;
+        const double f = li * recip(di);
+        dl[i] = f;
+        d[i] = di;
+        du[i] = ui;
+        du2[i] = 0;
+        di = dn - f * ui;
+        ui = un;
+      } else {
+        piv[i] = 1;
+        const double f = di * recip(li);
+        dl[i] = f;
+        d[i] = li;
+        du[i] = dn;
+        du2[i] = un;
+        di = ui - f * dn;
+        ui = -f * un;
+      }
+    }
+    d[n - 1] = di;
+    piv[n - 1] = 0;
+  }
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) {  // reciprocal pivots and the hashed start vector
+    double t = d[i];
+    if (fabs(t) < eps) t = t < 0 ? -eps : eps;
+    d[i] = recip(t);
+    unsigned long long h = (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 27;
+    x[i] = ((double)(h >> 11) / 9007199254740992.0) - 0.5;
+  }
+  __syncthreads();
+  for (int iter = 0; iter < iters; iter++) {
+    if (lane == 0) {
+      double xi = x[0];
+      {  // forward: L with interchanges
+        double xa = n > 1 ? x[1] : 0, fa = dl[0], xb = n > 2 ? x[2] : 0, fb = n > 1 ? dl[1] : 0;
+        int pa = piv[0], pb = n > 1 ? piv[1] : 0;
+        for (int i = 0; i < n - 1; i++) {
+          const double xn = xa, f = fa;
+          const int pv = pa;
+          xa = xb; fa = fb; pa = pb;
+          if (i + 2 < n - 1) {
+            xb = x[i + 3];
+            fb = dl[i + 2];
+            pb = piv[i + 2];
+          }
+          if (pv) {
+            x[i] = xn;
+            xi = xi - f * xn;
+          } else {
+            x[i] = xi;
+            xi = xn - f * xi;
+          }
+        }
+      }
+      double x1 = xi * d[n - 1], x2 = 0;  // backward: U (three bands)
+      x[n - 1] = x1;
+      if (n > 1) {
+        x2 = x1;
+        x1 = (x[n - 2] - du[n - 2] * x2) * d[n - 2];
+        x[n - 2] = x1;
+      }
+      double nrm = x1 * x1 + x2 * x2;
+      if (n > 2) {
+        double ra[4], rb[4] = {0, 0, 0, 0};
+        ra[0] = x[n - 3]; ra[1] = du[n - 3]; ra[2] = du2[n - 3]; ra[3] = d[n - 3];
+        if (n > 3) { rb[0] = x[n - 4]; rb[1] = du[n - 4]; rb[2] = du2[n - 4]; rb[3] = d[n - 4]; }
+        for (int i = n - 3; i >= 0; i--) {
+          const double xv = ra[0], u = ra[1], u2 = ra[2], dv = ra[3];
+          ra[0] = rb[0]; ra[1] = rb[1]; ra[2] = rb[2]; ra[3] = rb[3];
+          if (i - 2 >= 0) { rb[0] = x[i - 2]; rb[1] = du[i - 2]; rb[2] = du2[i - 2]; rb[3] = d[i - 2]; }
+          const double xi0 = (xv - u * x1 - u2 * x2) * dv;
+          x[i] = xi0;
+          x2 = x1;
+          x1 = xi0;
+          nrm += xi0 * xi0;
+        }
+      }
+      shared4[0] = sqrt(nrm);
+    }
+    __syncthreads();
+    const double nrm = shared4[0];
+    if (!(nrm > 0) || !isfinite(nrm)) return -1;
+    const double inv = 1.0 / nrm;
+    for (int i = lane; i < n; i += 64) x[i] = x[i] * inv;
+    __syncthreads();
+  }
+  double big = 0;
+  int jm = n;  // first index of the largest |x|
+  for (int i = lane; i < n; i += 64)
+    if (fabs(x[i]) > big) { big = fabs(x[i]); jm = i; }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double ob = __shfl_xor(big, o, 64);
+    const int oj = __shfl_xor(jm, o, 64);
+    if (ob > big || (ob == big && oj < jm)) { big = ob; jm = oj; }
+  }
+  if (jm >= n) jm = 0;
+  const double sg = x[jm] < 0 ? -1.0 : 1.0;
+  for (int i = lane; i < n; i += 64) v[i] = x[i] * sg;
+  __syncthreads();
+  return 0;
+}
+#endif
+
 }  // namespace tri
 }  // namespace dmsc

@@ -339,6 +339,7 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
   // operands are loaded before the first tile is computed
   uint4 bv[4], blv[4];
   float pdv[4], x32v[4];
+  int oldl[4];  // the labels being replaced (for the change count)
 #pragma unroll
   for (int tile = 0; tile < 4; tile++) {
     const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
@@ -347,6 +348,7 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
     blv[tile] = *reinterpret_cast<const uint4*>(xb + ((size_t)n + pp) * 32 + 8 * grp);
     pdv[tile] = pdot[pp];
     x32v[tile] = resp[(size_t)32 * n + pp];
+    oldl[tile] = labels[pp];
   }
 #pragma unroll
   for (int tile = 0; tile < 4; tile++) {
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
         bl = km_exact_label<K>(x, pd, sc);
         rech++;
       }
-      changed += bl != labels[p];
+      changed += bl != oldl[tile];
       labels[p] = bl;
     }
   }

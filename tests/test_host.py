@@ -261,3 +261,33 @@ def test_result_file_formats(tmp_path):
     assert raw.startswith(head)
     px = np.frombuffer(raw[len(head):], np.uint8).reshape(h, w)
     np.testing.assert_array_equal(px, (np.clip(g, 0, 1) * np.float32(255)).astype(np.uint8))
+
+
+def test_combine_weights_match_reference_tables():
+    """dmsc_combine_weights (the kernels' tables) against the reference's own
+    combine.cu:16-21 arrays (fixture read from the source text)."""
+    t = np.load(os.path.join(ROOT, "tests", "golden", "combine_tables.npz"))
+    for choice, suffix in ((dm.TEXTON32, ""), (dm.TEXTON64, "_64")):
+        coef, wts = (ctypes.c_float * 12)(), (ctypes.c_float * 12)()
+        assert lib().dmsc_combine_weights(choice, coef, wts) == 0
+        np.testing.assert_array_equal(np.array(coef[:], np.float32), t["hostCoefficients" + suffix].astype(np.float32))
+        np.testing.assert_array_equal(np.array(wts[:], np.float32), t["hostweights" + suffix][:12].astype(np.float32))
+    assert lib().dmsc_combine_weights(3, coef, wts) == 1  # DMSC_ERR_ARG
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks come before any device work: a null context or a bad
+    tuning value is an error status on a machine without a GPU."""
+    L = lib()
+    assert L.dmsc_gpb_batch(None, 0, None, None, 0) == 1
+    assert L.dmsc_spb_combine_forward(None, 64, 64, 4096, None, None, None, None, 9, None, None, None, 0, None,
+                                      None) == 1
+    assert L.dmsc_autotune_eigensolver(None, 64, 64, None, 4096, 0, 100, None, None, 0, None) == 1
+    assert L.dmsc_kmeans_rechecks(None, None) == 1
+    for key, bad in ((b"lanczos_tile", 6405), (b"lanczos_max_iter", 10), (b"lanczos_mixed_stop", 9),
+                     (b"lanczos_check_max", 30), (b"batch_lanes", 0)):
+        assert L.dmsc_tune_set(key, bad) == 1, key
+    v = ctypes.c_int()
+    for key, want in ((b"lanczos_tile", 3208), (b"lanczos_max_iter", 6000), (b"lanczos_mixed_stop", 4),
+                      (b"lanczos_check_max", 0), (b"batch_lanes", 4)):
+        assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key

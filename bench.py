@@ -83,6 +83,10 @@ def parse():
     p.add_argument("--pmc-mfma", metavar="DIR",
                    help="summarise a rocprofv3 --pmc SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_VALU_MFMA_BUSY_CYCLES "
                         "GRBM_GUI_ACTIVE pass of --pmc-workload into MFMA FLOPs and utilisation of --kernel")
+    p.add_argument("--pmc-sq", metavar="DIR",
+                   help="summarise a rocprofv3 --pmc pass of SQ wave-state counters (SQ_WAVES SQ_WAVE_CYCLES "
+                        "SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VMEM "
+                        "SQ_ACTIVE_INST_LDS) of --pmc-workload for --kernel")
     p.add_argument("--kernel", default=STEP_KERNEL,
                    help="kernel for --pmc-summarize: the Lanczos step kernel or an SpMV shape name")
     return p.parse_args()
@@ -117,6 +121,32 @@ def pmc_mfma(d, kernel):
           f"MFMA busy {busy:.0f} SIMD-cycles, GPU active {active:.0f} cycles, "
           f"MfmaUtil {100 * busy / (active * 1024):.3f} %, {ns / 1e3:.1f} us under the profiler "
           f"({flops / ns / 1e3:.2f} TFLOP/s)")
+
+
+def pmc_sq(d, kernel):
+    """Wave-state counters of `kernel` from one rocprofv3 pass with SQ_WAVES,
+    SQ_WAVE_CYCLES, SQ_BUSY_CYCLES, SQ_WAIT_ANY, SQ_WAIT_INST_ANY,
+    SQ_ACTIVE_INST_VALU, SQ_ACTIVE_INST_VMEM and SQ_ACTIVE_INST_LDS: the
+    share of the waves' lifetime spent waiting (on memory counters) and
+    issuing each instruction kind."""
+    import csv
+    import glob
+
+    rows = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if re.search(re.escape(kernel) + r"\(", r["Kernel_Name"]):
+                e = rows.setdefault(r["Dispatch_Id"], {})
+                e[r["Counter_Name"]] = e.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    if not rows:
+        raise SystemExit(f"no counter rows for {kernel} under {d}")
+    n = len(rows)
+    avg = {k: sum(e.get(k, 0.0) for e in rows.values()) / n for k in next(iter(rows.values()))}
+    wc = avg["SQ_WAVE_CYCLES"]
+    print(f"{kernel} ({n} dispatches): {avg['SQ_WAVES']:.0f} waves, wave-cycles {wc:.0f}; share of wave "
+          f"lifetime: waiting {avg['SQ_WAIT_ANY'] / wc:.3f}, waiting to issue {avg['SQ_WAIT_INST_ANY'] / wc:.3f}, "
+          f"VALU {avg['SQ_ACTIVE_INST_VALU'] / wc:.3f}, LDS {avg['SQ_ACTIVE_INST_LDS'] / wc:.3f}, "
+          f"VMEM {avg['SQ_ACTIVE_INST_VMEM'] / wc:.3f}")
 
 
 def pmc_summarize(fetch_dir, write_dir, kernel):
@@ -532,6 +562,9 @@ def main():
         return
     if a.pmc_mfma:
         pmc_mfma(a.pmc_mfma, a.kernel)
+        return
+    if a.pmc_sq:
+        pmc_sq(a.pmc_sq, a.kernel)
         return
     import torch
 

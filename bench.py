@@ -382,6 +382,32 @@ def differentiable(ctx, steps=200, images=16, lr=1e-2):
                     "pre-non-max gPb against cell boundaries dilated 1 px"}
 
 
+def stage_roofline(stage, iterations, kmeans_iterations, mixed):
+    """Per-stage roofline fractions of one 481x321 image from the stage times
+    (HIP events): each stage's algorithmic work over its whole time against
+    the bound that stage is written for (a lower bound where a stage mixes
+    kinds of work).  FP32 VALU peak 157.3 TFLOP/s, HBM 8 TB/s."""
+    n = W * H
+    valu = 157.3e12
+    t = [v * 1e-3 for v in stage]  # seconds
+    textons = 2 * 9010 * n + kmeans_iterations * 2 * n * 32 * 34   # filter bank + k-means distances
+    parabola = 2 * 58720 * n
+    hist = 890.3e6
+    eig = iterations * (STEP_BYTES_PER_ROW if mixed else 352) * n
+    spb = 6272 * n
+    return {
+        "textons": {"bound": "fp32 valu", "work": "filter bank 2*9010 flop/px + k-means 2*N*32*34 flop/iteration",
+                    "achieved_tflops": round(textons / t[0] / 1e12, 2), "frac": round(textons / t[0] / valu, 4)},
+        "localcues": {"bound": "fp32 valu (parabola) and hbm (histograms)",
+                      "parabola_tflops": round(parabola / t[1] / 1e12, 2), "parabola_frac": round(parabola / t[1] / valu, 4),
+                      "histogram_gbs": round(hist / t[1] / 1e9, 1), "histogram_frac": round(hist / t[1] / 8e12, 4)},
+        "eigensolve": {"bound": "hbm", "work": f"{iterations} Lanczos steps x {STEP_BYTES_PER_ROW if mixed else 352} B/row",
+                       "achieved_gbs": round(eig / t[4] / 1e9, 1), "frac": round(eig / t[4] / 8e12, 4)},
+        "spectralpb_gpb": {"bound": "fp32 valu", "work": "6272 flop/px",
+                           "achieved_tflops": round(spb / t[5] / 1e12, 2), "frac": round(spb / t[5] / valu, 4)},
+    }
+
+
 def fp64_residuals(ctx, out, w, h):
     """A posteriori FP64 check of the returned eigenpairs, needing no stored
     truth: u_j = D^1/2 v_j with the D^1/2 direction projected out (undoing
@@ -600,10 +626,14 @@ def main():
                    "n_eig": NVEC, "tol": TOL, "textons": 32},
         "plain_fp32": {"images_per_s": round(1.0 / t_plain, 3), "ms_per_image": round(t_plain * 1e3, 3),
                        "stage_ms": [round(v, 3) for v in stage_plain["stage_ms"]],
-                       "lanczos_iterations": stage_plain["lanczos_iterations"]},
+                       "lanczos_iterations": stage_plain["lanczos_iterations"],
+                       "stage_roofline": stage_roofline(stage_plain["stage_ms"], stage_plain["lanczos_iterations"],
+                                                        stage_plain["kmeans_iterations"], False)},
         "mixed": {"images_per_s": round(1.0 / t_mixed, 3), "ms_per_image": round(t_mixed * 1e3, 3),
                   "stage_ms": [round(v, 3) for v in stage_mixed["stage_ms"]],
-                  "lanczos_iterations": stage_mixed["lanczos_iterations"]},
+                  "lanczos_iterations": stage_mixed["lanczos_iterations"],
+                  "stage_roofline": stage_roofline(stage_mixed["stage_ms"], stage_mixed["lanczos_iterations"],
+                                                   stage_mixed["kmeans_iterations"], True)},
         "speedup_mixed_vs_plain": round(t_plain / t_mixed, 3),
         "stage_names": ["textons", "localcues", "combine+nonmax", "intervene", "eigensolve", "spectralPb+gPb"],
         "roofline": roof,

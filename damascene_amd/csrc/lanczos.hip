@@ -28,8 +28,8 @@
 // A mixed step is one kernel by default (k_lzmt: pixel tiles whose basis
 // values and stencil halo are staged in LDS; see k_lzmf and k_lzmt), with
 // the two-kernel form k_lzm1/k_lzm2 kept as a tuning alternative.
-// Each node of a captured chunk carries its step offset and the host sets
-// the chunk's first step in device memory, so a whole chunk is captured once
+// Each node of a chunk's graph carries its step offset and the host sets
+// the chunk's first step in device memory, so a whole chunk is built once
 // into a HIP graph and replayed.  At a check point one block per wanted
 // value finds the FP64 Ritz values of T and of T without its first
 // row/column by 256-way multisection over division-free Sturm counts, one
@@ -62,7 +62,7 @@ int lanczos_mixed_stop = 4;
 
 namespace {
 
-constexpr int kGraphSteps = 25;   // steps per captured graph; periods are multiples of it
+constexpr int kGraphSteps = 25;   // steps per graph; periods are multiples of it
 constexpr int kMaxEig = 25;
 constexpr int kCullumExtra = 20;  // eigCheck = nvec + 20 (lanczos.cu:456)
 constexpr int kCullumMax = 1000;  // lanczos.cu:531
@@ -158,7 +158,7 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
-// Step index of a kernel inside a captured chunk: the host sets st->k to
+// Step index of a kernel inside a chunk's graph: the host sets st->k to
 // the chunk's first step before each replay and node t of the chunk carries
 // t as an argument, so no kernel updates the index while others read it.
 
@@ -1275,7 +1275,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
       cap = ncap;
       hipLaunchKernelGGL(k_set_basis, dim3(1), dim3(1), 0, s, st, V);
     }
-    // steps 0..24 capture the graph and warm up; steps 25.. are timed
+    // steps 0..24 build the graph and warm up; steps 25.. are timed
     DMSC_TRY(run_steps(ctx, p, st, w, 0, kGraphSteps));
     hipEvent_t e0, e1;
     DMSC_HIP(hipEventCreate(&e0));

@@ -562,16 +562,42 @@ def errors_vs_truth(out, truth):
 
 
 def cpu_baseline(rgb):
+    """The oracle's plain-FP32 pipeline in reference order on the host cores
+    (OpenMP): one warm-up, then the median of three runs of the 481x321 image
+    with every thread; beside it one thread on the 160x120 image (the
+    reference's CPU-runnable configuration, 12 cells), after a warm-up."""
+    import ctypes
+    import statistics
+
+    import damascene_amd as dm
     from oracle import pyoracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
+    lib = O.lib()
+    lib.orc_max_threads.restype = ctypes.c_int
+    threads = lib.orc_max_threads()
     r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
+        runs.append(time.perf_counter() - t0)
+    dt = statistics.median(runs)
+    small = dm.generate_image(160, 120, 12, SEED)
+    lib.orc_set_threads(1)
+    try:
+        O.pipeline(small, nvec=NVEC, tol=TOL)
+        t0 = time.perf_counter()
+        O.pipeline(small, nvec=NVEC, tol=TOL)
+        dt1 = time.perf_counter() - t0
+    finally:
+        lib.orc_set_threads(threads)
     return {
         "value": round(1.0 / dt, 4), "unit": "images/s", "cores": threads, "kind": "port",
-        "sample": f"one {W}x{H} image, whole plain-FP32 gPb in reference order (oracle, OpenMP), {dt:.2f} s",
+        "sample": f"one {W}x{H} image, whole plain-FP32 gPb in reference order (oracle, OpenMP, {threads} "
+                  f"threads): median of 3 runs after a warm-up, {dt:.2f} s",
         "stage_ms": [round(v, 1) for v in r["stage_ms"]],
+        "one_thread_160x120": {"images_per_s": round(1.0 / dt1, 4), "s_per_image": round(dt1, 3),
+                               "nproc": os.cpu_count()},
     }, r
 
 

@@ -115,6 +115,8 @@ int  orc_postprocess(int w, int h, float* gpb, const float* nmax, float* thin);
 void orc_normalize_gpb_all(int npix, int norient, int pitch, float* all);
 
 /* whole pipeline (plain FP32 reference order); returns 0 on success */
+void orc_set_threads(int n);  /* OpenMP threads of the parallel loops */
+int  orc_max_threads(void);
 int  orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol,
                       int* textons, float* mpb8, float* cg8, float* nmax, float* eigvals,
                       float* eigvecs, float* spb8, float* gpbAll, float* gpb, double* stage_ms,

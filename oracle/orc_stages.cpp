@@ -5,6 +5,7 @@
 // output element is computed by one iteration in the reference's order, so
 // the result does not depend on the thread count.
 #include "oracle.h"
+#include <omp.h>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -20,6 +21,11 @@ inline int mirror_index(int i, int border, int n) {  // convert.cu:128-141
 }  // namespace
 
 extern "C" {
+
+// OpenMP threads of the oracle's parallel loops (timing of the CPU baseline).
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 
 // convert.cu:11-25 rgbUtoGreyF_kernel; R in bits 0-7, G 8-15, B 16-23.
 void orc_rgb_to_grey(int w, int h, const uint32_t* rgb, float* grey) {

@@ -97,8 +97,8 @@ extern int lanczos_k1_threads;
 // (default), 1 = one fused kernel over runs of consecutive rows (when
 // the stencil halo fits in LDS), 0 = two kernels per step.
 extern int lanczos_mixed_fused;
-// Tile of the tiled fused mixed step, TW * 100 + TH: 6404 (default), 6408,
-// 6402, 3208 or 3204.
+// Tile of the tiled fused mixed step, TW * 100 + TH: one of kStepTiles
+// (3208, the default, first).
 extern int lanczos_tile;
 extern int lanczos_check_max;  // mixed path: most steps between checks (0 = a quarter of the steps so far)
 extern int lanczos_mixed_stop;  // mixed path: stop when every residual is <= tol * 10^-this (default 4)

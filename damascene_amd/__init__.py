@@ -8,13 +8,14 @@ combine, non_max_suppression, intervene, generalized_eigensolve,
 spectral_pb, calc_gpb, plus gpb() for the whole pipeline.
 """
 import ctypes
+import os
 
 import numpy as np
 
 from ._lib import BatchImage, DmscError, GpbOutputs, GpbParams, StepShape, check, lib
 
 __all__ = [
-    "DmscError", "Context", "generate_image", "boundary_mask", "pitch_in_floats", "gpb",
+    "DmscError", "Context", "generate_image", "boundary_mask", "load_ppm", "pitch_in_floats", "gpb",
     "PREC_FP32", "PREC_MIXED",
 ]
 
@@ -43,6 +44,16 @@ def generate_image(w, h, ncells, seed, with_cells=False):
     cells = np.zeros((h, w), np.int32) if with_cells else None
     check(lib().dmsc_generate_image(w, h, ncells, ctypes.c_uint64(seed), _p(rgb), _p(cells)))
     return (rgb, cells) if with_cells else rgb
+
+
+def load_ppm(path):
+    """convert.cu:88-95 loadPPM_rgbU: a binary PPM (P6, maxval 255) as a
+    [h, w] uint32 array, R in bits 0-7, G 8-15, B 16-23."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    check(lib().dmsc_load_ppm(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), None))
+    rgb = np.zeros((h.value, w.value), np.uint32)
+    check(lib().dmsc_load_ppm(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), _p(rgb)))
+    return rgb
 
 
 def boundary_mask(cells):

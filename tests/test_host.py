@@ -168,6 +168,9 @@ def test_ppm_loader_packs_rgb(tmp_path):
     bad = tmp_path / "bad.ppm"
     bad.write_bytes(b"P5\n7 5\n255\n" + px.tobytes())
     assert lib().dmsc_load_ppm(str(bad).encode(), ctypes.byref(W), ctypes.byref(H), None) != 0
+    np.testing.assert_array_equal(dm.load_ppm(path), want)  # the Python wrapper
+    with pytest.raises(dm.DmscError):
+        dm.load_ppm(bad)
 
 
 STAGE_API_PROGRAM = r"""

@@ -38,6 +38,7 @@ int make_lanes(Ctx* ctx, int lanes) {
     c->device = ctx->device;
     c->stream = s;
     c->own_stream = true;
+    c->overlap_stages = false;
     ctx->lanes.push_back(c);
   }
   return DMSC_OK;

@@ -696,6 +696,20 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   }
   // damascene.cu:269-285 grey + textons; 309-317 Lab (gamma 2.5), normalised
   DMSC_TRY(launch_colour_fused(s, n, 2.5f, rgb, grey, L, A, Bc));
+  // damascene.cu:352 localCues, L/a/b channels: they need no textons, so they
+  // run on the context's second stream while the textons are computed
+  // (on the main stream after the textons when the context does not overlap)
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  DMSC_TRY(local_cues_prepare(ctx, w, h, ntex));
+  if (ctx->overlap_stages) {
+    DMSC_TRY(ctx->aux_stream(&aux, &fork, &join));
+    DMSC_HIP(hipEventRecord(fork, s));
+    DMSC_HIP(hipStreamWaitEvent(aux, fork, 0));
+    DMSC_TRY(launch_local_cues_part(ctx, aux, 1, w, h, L, A, Bc, tex, ntex, cues, cues + 24 * P, cues + 48 * P,
+                                    cues + 72 * P, pitch, nullptr));
+    DMSC_HIP(hipEventRecord(join, aux));
+  }
   const float* coeffs;
   DMSC_TRY(ctx->texton_coeffs(&coeffs));
   DMSC_TRY(launch_texton_conv(s, w, h, grey, coeffs, resp));
@@ -715,9 +729,10 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
     DMSC_HIP(hipMemcpyAsync(kit, kmeans_iterations_ptr(scr), sizeof(int), hipMemcpyDeviceToHost, s));
   }
   if (timing) DMSC_HIP(hipEventRecord(ev[1], s));
-  // damascene.cu:352 localCues
-  DMSC_TRY(launch_local_cues(ctx, w, h, L, A, Bc, tex, ntex, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P,
-                             pitch, nullptr));
+  // damascene.cu:352 localCues, texton channel; then join the L/a/b part
+  DMSC_TRY(launch_local_cues_part(ctx, s, ctx->overlap_stages ? 2 : 3, w, h, L, A, Bc, tex, ntex, cues,
+                                  cues + 24 * P, cues + 48 * P, cues + 72 * P, pitch, nullptr));
+  if (ctx->overlap_stages) DMSC_HIP(hipStreamWaitEvent(s, join, 0));
   if (timing) DMSC_HIP(hipEventRecord(ev[2], s));
   // damascene.cu:388-404 combine + nonMaxSuppression
   DMSC_TRY(launch_combine(s, n, pitch, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, ntex, mpb, cg));

@@ -36,6 +36,10 @@ struct Ctx {
   hipStream_t aux = nullptr;  // second stream (default priority) for work overlapped with `stream`
   hipEvent_t aux_ev[2] = {};
   std::vector<Ctx*> lanes;  // child contexts of batch calls, each with its own stream
+  // dmsc_gpb runs the L/a/b local cues on `aux` beside the textons; batch
+  // lanes do not (the lanes already overlap whole images, and a second
+  // stream per lane makes them contend for the hardware queues)
+  bool overlap_stages = true;
   ~Ctx();
   int pinned(int** out);
   int aux_stream(hipStream_t* s, hipEvent_t* ev0, hipEvent_t* ev1);

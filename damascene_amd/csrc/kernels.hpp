@@ -31,6 +31,12 @@ const int* kmeans_iterations_ptr(const void* state);  // device address of the i
 size_t local_cues_scratch_bytes(int w, int h);
 int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B, const int* tex,
                       int ntex, float* bg, float* cga, float* cgb, float* tg, int pitch, float* grads_out);
+// The local cues in two parts for two streams: part 1 the L, a, b channels
+// (needs no textons), part 2 the texton channel, 3 both; prepare first.
+int local_cues_prepare(Ctx* ctx, int w, int h, int ntex);
+int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, const float* L, const float* A,
+                           const float* B, const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg,
+                           int pitch, float* grads_out);
 
 // ------------------------------------------------------------- combine / nonmax / gPb
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga, const float* cgb,

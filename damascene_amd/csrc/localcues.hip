@@ -58,12 +58,12 @@ __global__ void k_fill_u8(uint8_t* p, size_t n) {
 }
 
 // gradient.cu:210-223 (quantizeImage, mirrorImage) + rotate.cu:247-259
-// (bresenhamRotate): one mirrored pixel per thread, all 16 frames.
-__global__ void k_cue_labels(FrameGeom g, const float* __restrict__ L, const float* __restrict__ A,
+// (bresenhamRotate): one mirrored pixel per thread, frames f0, f0 + 1, ...
+__global__ void k_cue_labels(FrameGeom g, int f0, const float* __restrict__ L, const float* __restrict__ A,
                              const float* __restrict__ B, const int* __restrict__ tex,
                              const int* __restrict__ tabs, uint8_t* __restrict__ frames) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y;
-  const int f = blockIdx.z, ch = f >> 2, o = f & 3;
+  const int f = f0 + blockIdx.z, ch = f >> 2, o = f & 3;
   if (X >= g.bw) return;
   const size_t si = (size_t)dev::mirror(Y, kBorder, g.h) * g.w + dev::mirror(X, kBorder, g.w);
   int lab;
@@ -111,8 +111,8 @@ __global__ __launch_bounds__(256) void k_integral_rows(FrameGeom g, int f0, cons
 }
 
 // gradient.cu:89-103: column pass, two bins per thread, modulo 2^16.
-__global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, uint16_t* __restrict__ I) {
-  const int f = blockIdx.y, o = f & 3;
+__global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, int f0, uint16_t* __restrict__ I) {
+  const int f = f0 + blockIdx.y, o = f & 3;
   const int nw = g.nw[o], nh = g.nh[o], pairs = bin_pitch(g, f) / 2;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (x, bin pair)
   if (t >= nw * pairs) return;
@@ -558,12 +558,27 @@ size_t local_cues_scratch_bytes(int w, int h) {
   return lab + ints + grads + 4096;
 }
 
-// localcues.cu:257-300 localCues for all four cues.
-int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B,
-                      const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg, int pitch,
-                      float* grads_out) {
-  hipStream_t s = ctx->stream;
+// localcues.cu:257-300 localCues, in two parts that may run on two streams:
+// part 1 the L, a and b channels (frames 0..11: labels, integral images,
+// Bg and Cg gradients and their parabola fits), which do not need the
+// textons; part 2 the texton channel (frames 12..15: Tg); part 3 both.
+// local_cues_prepare builds the plan and sizes the scratch arena first, so
+// the parts only launch kernels and never reallocate while the other part
+// runs.
+int local_cues_prepare(Ctx* ctx, int w, int h, int ntex) {
   if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "local cues support 32 or 64 textons");
+  CuePlan* P;
+  DMSC_TRY(get_plan(ctx, w, h, ntex, &P));
+  void* scratch;
+  const size_t gbytes = (size_t)4 * 24 * w * h * sizeof(float);
+  return ctx->scratch(1, P->lab_bytes + P->int_elems * sizeof(uint16_t) + gbytes + 1024, &scratch);
+}
+
+int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, const float* L, const float* A,
+                           const float* B, const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg,
+                           int pitch, float* grads_out) {
+  if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "local cues support 32 or 64 textons");
+  if (part < 1 || part > 3) return fail(DMSC_ERR_INTERNAL, "bad local-cue part %d", part);
   const int tbp = ntex;
   CuePlan* P;
   DMSC_TRY(get_plan(ctx, w, h, tbp, &P));
@@ -576,38 +591,53 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
   uint8_t* frames = (uint8_t*)scratch;
   uint16_t* I = (uint16_t*)((char*)scratch + P->lab_bytes);
   float* grads = grads_out ? grads_out : (float*)((char*)scratch + P->lab_bytes + ibytes);
-  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(P->lab_bytes, 256)), dim3(256), 0, s, frames, P->lab_bytes);
-  hipLaunchKernelGGL(k_cue_labels, dim3(div_up(g.bw, 256), g.bh, kFrames), dim3(256), 0, s, g, L, A, B,
-                     tex, P->d_tabs, frames);
+  const bool colour = part & 1, texture = part & 2;
+  // frames of channel ch are f = 4 ch + o; the texture frames 12..15 come last
+  const size_t split = (size_t)g.lab_off[12];
+  const int f0 = colour ? 0 : 12, nf = (colour ? 12 : 0) + (texture ? 4 : 0);
+  const size_t fill0 = colour ? 0 : split, fill1 = texture ? P->lab_bytes : split;
+  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(fill1 - fill0, 256)), dim3(256), 0, s, frames + fill0, fill1 - fill0);
+  hipLaunchKernelGGL(k_cue_labels, dim3(div_up(g.bw, 256), g.bh, nf), dim3(256), 0, s, g, f0, L, A, B, tex,
+                     P->d_tabs, frames);
   int maxh = 0, maxw = 0;
   for (int o = 0; o < 4; o++) { maxh = std::max(maxh, g.nh[o]); maxw = std::max(maxw, g.nw[o]); }
-  if (tbp == 32) {
-    hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, kFrames), dim3(256), (size_t)maxw, s, g, 0, frames, I);
-  } else {
+  if (colour) {
     hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 12), dim3(256), (size_t)maxw, s, g, 0, frames, I);
-    hipLaunchKernelGGL(k_integral_rows<64>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+    hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 32 / 2, 256), 12), dim3(256), 0, s, g, 0, I);
   }
-  hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * tbp / 2, 256), kFrames), dim3(256), 0, s, g, I);
+  if (texture) {
+    if (tbp == 32)
+      hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+    else
+      hipLaunchKernelGGL(k_integral_rows<64>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+    hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * tbp / 2, 256), 4), dim3(256), 0, s, g, 12, I);
+  }
   const size_t plane = (size_t)n;
   BlurK none;
   memset(&none, 0, sizeof(none));
   // Bg: channel 0, radii {3,5,10}; Cga/Cgb: channels 1-2, radii {5,10,20};
   // Tg: channel 3, radii {5,10,20}.
-  hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 0, 0,
-                     P->d_tabs, I, P->blur_bg, grads, plane);
-  hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 3, 8), dim3(256), 0, s, g, 1, 1,
-                     P->d_tabs, I, P->blur_cg, grads, plane);
-  if (ntex == 64)
-    hipLaunchKernelGGL(k_gradients64, dim3(div_up(2 * n, 256), 3, 4), dim3(256), 0, s, g, 1, P->d_tabs, I, grads,
-                       plane);
-  else
-    hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
-                       P->d_tabs, I, none, grads, plane);
+  if (colour) {
+    hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 0, 0,
+                       P->d_tabs, I, P->blur_bg, grads, plane);
+    hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 3, 8), dim3(256), 0, s, g, 1, 1,
+                       P->d_tabs, I, P->blur_cg, grads, plane);
+  }
+  if (texture) {
+    if (ntex == 64)
+      hipLaunchKernelGGL(k_gradients64, dim3(div_up(2 * n, 256), 3, 4), dim3(256), 0, s, g, 1, P->d_tabs, I, grads,
+                         plane);
+    else
+      hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
+                         P->d_tabs, I, none, grads, plane);
+  }
   CueOut out{{bg, cga, cgb, tg}};
   for (int scale = 0; scale < 3; scale++) {
-    for (int grp = 0; grp < 2; grp++) {
-      const int ri = scale + grp;
-      const int ch0 = grp == 0 ? 0 : 1, nch = grp == 0 ? 1 : 3;
+    // (first channel, channels, radius index): Bg; Cga and Cgb; Tg
+    const int groups[3][3] = {{0, 1, scale}, {1, 2, scale + 1}, {3, 1, scale + 1}};
+    for (int gi = 0; gi < 3; gi++) {
+      if ((gi < 2 && !colour) || (gi == 2 && !texture)) continue;
+      const int ch0 = groups[gi][0], nch = groups[gi][1], ri = groups[gi][2];
       const dim3 grid(div_up(w, QTX), div_up(h, QTY), nch * 8);
       switch (kRadii[ri]) {
 #define PARABOLA_CASE(R)                                                                                   \
@@ -623,6 +653,13 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
   }
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
+}
+
+int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, const float* B,
+                      const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg, int pitch,
+                      float* grads_out) {
+  DMSC_TRY(local_cues_prepare(ctx, w, h, ntex));
+  return launch_local_cues_part(ctx, ctx->stream, 3, w, h, L, A, B, tex, ntex, bg, cga, cgb, tg, pitch, grads_out);
 }
 
 }  // namespace dmsc

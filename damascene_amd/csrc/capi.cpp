@@ -46,9 +46,10 @@ Ctx::~Ctx() {
 }
 
 // Pinned block layout: bytes [0, 256) the pipeline's k-means iteration
-// count, [1024, 4096) the eigensolver's status record.
+// count, [1024, 4096) the eigensolver's status record, [8192, 40960) the
+// block eigensolver's Gram matrices and rotation.
 int Ctx::pinned(int** out) {
-  if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 4096, hipHostMallocDefault));
+  if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 65536, hipHostMallocDefault));
   *out = h_pinned;
   return DMSC_OK;
 }
@@ -258,6 +259,16 @@ int dmsc_tune_set(const char* key, int value) {
     batch_lanes = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "eigensolver")) {
+    DMSC_ARG(value == 0 || value == 1, "eigensolver must be 0 (Lanczos) or 1 (Chebyshev block)");
+    eigensolver_method = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_shape")) {
+    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 22 or 42");
+    chebyshev_shape = value;
+    return DMSC_OK;
+  }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
 }
 int dmsc_tune_get(const char* key, int* value) {
@@ -296,6 +307,14 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "batch_lanes")) {
     *value = batch_lanes;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "eigensolver")) {
+    *value = eigensolver_method;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_shape")) {
+    *value = chebyshev_shape;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);

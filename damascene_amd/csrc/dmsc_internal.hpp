@@ -32,7 +32,7 @@ struct Ctx {
   void* arena[kArenas] = {};
   size_t arena_bytes[kArenas] = {};
   float* d_texton_coeffs = nullptr;
-  int* h_pinned = nullptr;  // pinned host block (4 KiB) for asynchronous read-backs
+  int* h_pinned = nullptr;  // pinned host block (64 KiB) for asynchronous read-backs
   hipStream_t aux = nullptr;  // second stream (default priority) for work overlapped with `stream`
   hipEvent_t aux_ev[2] = {};
   std::vector<Ctx*> lanes;  // child contexts of batch calls, each with its own stream

@@ -110,6 +110,14 @@ extern int lanczos_check_max;  // mixed path: most steps between checks (0 = a q
 extern int lanczos_mixed_stop;  // mixed path: stop when every residual is <= tol * 10^-this (default 4)
 extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25)
 extern int batch_lanes;
+// Mixed path eigensolver: 0 Lanczos, 1 Chebyshev-filtered block subspace
+// iteration (chebyshev.hip); its filter tile (rows per lane * 10 + float4
+// planes per lane: 22 or 42).
+extern int eigensolver_method;
+extern int chebyshev_shape;
+bool chebyshev_shape_valid(int v);
+int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec, float tol, float* d_out,
+                    double* vals, int* products);
 
 // ------------------------------------------------------------- eigensolver
 int run_eigensolve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, int precision,

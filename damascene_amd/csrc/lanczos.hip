@@ -1250,6 +1250,25 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
                   st};
 
   if (convert) DMSC_TRY(launch_convert_matrix(s, w, h, A, pitch, nullptr, rs));
+  float* mm = (float*)(B + o_mm);
+  float* fin = (float*)(B + o_fin);
+  // D^-1/2 scaling (stencilMVM.cu:182-198) and min-max normalisation of the unit eigenvectors
+  auto finish = [&]() -> int {
+    const dim3 g(div_up(n, 256));
+    hipLaunchKernelGGL(k_scale_rs, g, dim3(256), 0, s, n, nvec, rs, d_eigvecs);
+    hipLaunchKernelGGL(k_minmax1, dim3(nblk, nvec), dim3(256), 0, s, n, d_eigvecs, mm, nblk);
+    hipLaunchKernelGGL(k_minmax2, dim3(nvec), dim3(256), 0, s, mm, nblk, fin);
+    hipLaunchKernelGGL(k_normalize, g, dim3(256), 0, s, n, nvec, fin, d_eigvecs);
+    DMSC_HIP(hipGetLastError());
+    return DMSC_OK;
+  };
+  if (mixed && eigensolver_method == 1 && time_steps == 0 && d_eigvecs) {
+    double vals[kMaxEig];
+    DMSC_TRY(chebyshev_solve(ctx, w, h, A, pitch, nvec, tol, d_eigvecs, vals, iterations));
+    DMSC_TRY(finish());
+    for (int j = 0; j < nvec; j++) h_eigvals[j] = (float)vals[j];
+    return DMSC_OK;
+  }
   const float q0 = (float)(1 / std::sqrt((double)n));  // lanczos.cu:64-72
   const dim3 gi(div_up(std::max(n, nb1), 256));
   hipLaunchKernelGGL(k_zero_row, dim3(div_up(n, 256)), dim3(256), 0, s, V - vstride, n);
@@ -1425,14 +1444,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
 
   // Ritz vectors, D^-1/2 scaling (stencilMVM.cu:182-198), min-max normalisation
   DMSC_TRY(launch_ritz(s, mixed, nvec, n, st, vstride, m, S, d_eigvecs));
-  const dim3 g(div_up(n, 256));
-  hipLaunchKernelGGL(k_scale_rs, g, dim3(256), 0, s, n, nvec, rs, d_eigvecs);
-  float* mm = (float*)(B + o_mm);
-  float* fin = (float*)(B + o_fin);
-  hipLaunchKernelGGL(k_minmax1, dim3(nblk, nvec), dim3(256), 0, s, n, d_eigvecs, mm, nblk);
-  hipLaunchKernelGGL(k_minmax2, dim3(nvec), dim3(256), 0, s, mm, nblk, fin);
-  hipLaunchKernelGGL(k_normalize, g, dim3(256), 0, s, n, nvec, fin, d_eigvecs);
-  DMSC_HIP(hipGetLastError());
+  DMSC_TRY(finish());
   for (int j = 0; j < nvec; j++) h_eigvals[j] = (float)hs.vals[j];
   if (iterations) *iterations = m;
   return DMSC_OK;

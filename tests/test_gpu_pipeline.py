@@ -17,6 +17,7 @@ import pytest
 import torch
 
 import damascene_amd as dm
+from damascene_amd._lib import check, lib
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -135,16 +136,19 @@ def test_pipeline_texton64(ctx):
     assert gerr < 2e-3
 
 
+@pytest.mark.parametrize("solver", [0, 1])
 @pytest.mark.parametrize("precision,seed,fixture", [(dm.PREC_MIXED, 20261202, "truth_481x321.npz"),
                                                     (dm.PREC_MIXED, 20261204, "truth_481x321_s20261204.npz")])
-def test_eigensolve_481x321_vs_truth(ctx, precision, seed, fixture):
+def test_eigensolve_481x321_vs_truth(ctx, precision, seed, fixture, solver):
     """The full-size workload (481x321, 656 blocks per Lanczos step) on the
     oracle's own L_sym against committed FP64 truths (shift-invert eigsh,
-    tests/golden/make_truth.py).  Seed 20261204 is the image on which a stop
-    bound of tol * 1e-3 left a subspace error of 1.7e-4."""
+    tests/golden/make_truth.py), with the mixed path's Lanczos solver
+    (solver 0) and its Chebyshev block solver (1).  Seed 20261204 is the
+    image on which a Lanczos stop bound of tol * 1e-3 left a subspace error
+    of 1.7e-4."""
     import os
     W, H, CELLS, SEED = 481, 321, 40, seed
-    print(f"case: eigensolve {W}x{H} precision {precision} seed {seed}", flush=True)
+    print(f"case: eigensolve {W}x{H} precision {precision} seed {seed} solver {solver}", flush=True)
     truth = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", fixture))
     assert list(truth["meta"]) == [W, H, CELLS, SEED, NVEC]
     rgb = dm.generate_image(W, H, CELLS, SEED)
@@ -154,7 +158,11 @@ def test_eigensolve_481x321_vs_truth(ctx, precision, seed, fixture):
     Wm = O.intervene(O.nonmax(mpb))
     n = W * H
     m = torch.from_numpy(np.pad(Wm, ((0, 0), (0, dm.pitch_in_floats(n) - n)))).cuda()
-    vals, vecs, it = ctx.generalized_eigensolve(W, H, m, nvec=NVEC, tol=TOL, precision=precision)
+    check(lib().dmsc_tune_set(b"eigensolver", solver))
+    try:
+        vals, vecs, it = ctx.generalized_eigensolve(W, H, m, nvec=NVEC, tol=TOL, precision=precision)
+    finally:
+        check(lib().dmsc_tune_set(b"eigensolver", 0))
     tv = truth["eigvals"][:NVEC]
     rel = np.abs(vals[1:] - tv[1:]) / tv[1:]
     sq = truth["sqrt_d"].astype(np.float64)
@@ -164,7 +172,7 @@ def test_eigensolve_481x321_vs_truth(ctx, precision, seed, fixture):
     R = truth["u"].astype(np.float64)
     R -= np.outer(R @ u0, u0)
     ang = subspace_angle(G.T, R.T)
-    print(f"481x321 precision {precision}: iterations {it}, eigval max rel {rel.max():.3e}, "
+    print(f"481x321 precision {precision} solver {solver}: iterations {it}, eigval max rel {rel.max():.3e}, "
           f"subspace angle {ang:.3e}", flush=True)
     assert rel.max() < 1e-4
     assert ang < 1e-4

@@ -224,6 +224,16 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *   "lanczos_trace":      non-zero prints one line per eigensolver check to
  *                         stderr (2: also the Ritz values of a failed
  *                         screening).
+ *   "eigensolver":        mixed precision only: 0 Lanczos (the default), 1
+ *                         Chebyshev-filtered subspace iteration on a block of
+ *                         32 vectors with an FP64 Rayleigh-Ritz step (no
+ *                         reference counterpart; nvec <= 16).  It stops when no
+ *                         wanted Ritz vector moved by more than 0.03 tol
+ *                         (sine of the angle) since the previous outer
+ *                         iteration; the iteration count it reports is the
+ *                         number of block products with the matrix.
+ *   "chebyshev_shape":    its filter kernel's tile, rows per lane * 10 +
+ *                         float4 planes per lane: 22 (the default) or 42.
  * Returns DMSC_ERR_ARG for an unknown key or value. */
 /* Measurement (no reference counterpart; it backs the roofline figure of
  * the benchmark): converts d_matrix in place as the eigensolver does, runs

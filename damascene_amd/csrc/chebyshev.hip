@@ -5,13 +5,15 @@
 // wanted eigenvalues sit in a narrow cluster at 0 (481x321: nine values in
 // [0, 1.5e-4], neighbours 1e-5 apart) of a spectrum that reaches 2, so one
 // Krylov vector needs about 2,000 steps, each a grid-wide reduction.  This
-// solver filters a block of kB = 32 vectors with a Chebyshev polynomial of
-// degree kDeg that damps [a, 2] (a = the block's largest Ritz value): the
+// solver filters a block of kB = 32 vectors with a Chebyshev polynomial that
+// damps [a, 2] (a = the block's largest Ritz value): the
 // wanted vectors converge at a rate set by their gap to the 33rd eigenvalue
 // rather than to the 10th, and a filter step has no reduction.  One outer
 // iteration:
-//   1. kDeg filter steps y = alpha (A x - c x) - beta z, one kernel each
-//      (the scaled three-term recurrence, so the block stays O(1));
+//   1. deg filter steps y = alpha (A x - c x) - beta z, one kernel each
+//      (the scaled three-term recurrence, so the block stays O(1)); deg is
+//      50 at first, then whatever cuts the slowest wanted vector's error
+//      tenfold given the current Ritz values (30..300);
 //   2. W = A Y; the Gram matrices G = Y'Y, H = Y'W and C = X0'Y in FP64
 //      (X0: the previous iteration's Ritz vectors, the filter's input);
 //   3. on the host, H s = theta G s in FP64 (Cholesky of G, cyclic Jacobi);
@@ -44,8 +46,9 @@ namespace {
 
 constexpr int kB = 32;             // block vectors
 constexpr int kPlanes = kB / 4;    // float4 planes
-constexpr int kDeg = 50;           // filter degree per outer iteration
-constexpr int kMaxOuter = 40;
+constexpr int kDeg = 50;           // filter degree of the first outer iteration
+constexpr int kMinDeg = 30, kMaxDeg = 300;
+constexpr int kMaxOuter = 60;
 constexpr double kFirstCut = 1e-2;  // damped interval of the first filter: [kFirstCut, 2]
 constexpr double kUpper = 2.0;      // spectrum of L_sym lies in [0, 2]
 constexpr int kGramPx = 64;         // pixels staged per Gram round
@@ -425,15 +428,16 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   std::vector<double> theta, S;
   const double bound = 0.03 * (double)tol;  // sin of the largest per-vector move between iterations
   double cut = kFirstCut;
+  int deg = kDeg, total = 0;
   for (int outer = 0; outer < kMaxOuter; outer++) {
-    // scaled Chebyshev filter of degree kDeg damping [cut, kUpper], normalised at 0
+    // scaled Chebyshev filter of degree deg damping [cut, kUpper], normalised at 0
     const double e = (kUpper - cut) / 2, c = (kUpper + cut) / 2;
     double sigma = e / (0.0 - c);
     const double tau = 2 / sigma;
     DMSC_TRY(step(shape, s, w, h, A, pitchz, X0, nullptr, ring[0], vs, sigma / e, c, 0.0));
     int cur = 0;
     const float4* prev = X0;
-    for (int i = 2; i <= kDeg; i++) {
+    for (int i = 2; i <= deg; i++) {
       const double sn = 1 / (tau - sigma);
       const int nx = (cur + 1) % 3;
       DMSC_TRY(step(shape, s, w, h, A, pitchz, ring[cur], prev, ring[nx], vs, 2 * sn / e, c, sigma * sn));
@@ -460,8 +464,8 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       }
     }
     if (lanczos_trace)
-      std::fprintf(stderr, "[chebyshev] outer %d cut %.4e theta %.6e .. %.6e move %.3e\n", outer, cut, theta[0],
-                   theta[nvec - 1], move);
+      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e\n", outer, deg, cut,
+                   theta[0], theta[nvec - 1], move);
     std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
     DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
     const bool done = outer > 0 && move <= bound;
@@ -469,12 +473,19 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, nvec, nullptr, d_out);
       DMSC_HIP(hipGetLastError());
       for (int j = 0; j < nvec; j++) vals[j] = theta[j];
-      if (products) *products = (outer + 1) * (kDeg + 1);
+      if (products) *products = total + deg + 1;
       return DMSC_OK;
     }
     hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, -1, X0, nullptr);
     DMSC_HIP(hipGetLastError());
+    total += deg + 1;
     cut = std::max(theta[kB - 1], 1e-6);
+    // next degree: a tenfold reduction of the slowest wanted vector's error,
+    // whose components grow by exp(acosh(t)) per degree against the damped
+    // interval (t = its distance from the interval's centre in half-widths)
+    const double tj = ((kUpper + cut) / 2 - theta[nvec - 1]) / ((kUpper - cut) / 2);
+    deg = tj > 1.0 ? (int)std::ceil(std::log(10.0) / std::acosh(tj)) : kMaxDeg;
+    deg = std::min(kMaxDeg, std::max(kMinDeg, deg));
   }
   return fail(DMSC_ERR_NOCONV, "block eigensolver did not converge in %d outer iterations", kMaxOuter);
 }

@@ -229,3 +229,26 @@ def test_large_image_fp64_residuals(ctx):
     assert out["lanczos_iterations"] < 6000
     assert chk["residual_max"] < 5e-7
     assert chk["eigval_vs_rayleigh_max_rel"] < 1e-5
+
+
+def test_block_solver_large_image(ctx):
+    """The opt-in block eigensolver (tune key "eigensolver" = 1) on a
+    780x1001 image of the uneven batch, whose denser spectrum needs filter
+    degrees of about 150: its eigenvalues and gPb against the Lanczos
+    solver's (both mixed precision)."""
+    w, h, seed = 780, 1001, 20261202 + 142
+    print(f"case: block solver {w}x{h} seed {seed}", flush=True)
+    img = ctx.upload(dm.generate_image(w, h, w * h // 4000, seed))
+    ref = ctx.gpb(img, precision=dm.PREC_MIXED)
+    check(lib().dmsc_tune_set(b"eigensolver", 1))
+    try:
+        out = ctx.gpb(img, precision=dm.PREC_MIXED)
+    finally:
+        check(lib().dmsc_tune_set(b"eigensolver", 0))
+    ev, rv = out["eigvals"][1:].astype(np.float64), ref["eigvals"][1:].astype(np.float64)
+    rel = float(np.max(np.abs(ev - rv) / rv))
+    g = float(np.abs(out["gpb"].cpu().numpy() - ref["gpb"].cpu().numpy()).max())
+    print(f"{w}x{h}: Lanczos {ref['lanczos_iterations']} steps, block solver {out['lanczos_iterations']} block "
+          f"products; eigval rel {rel:.2e}, gPb max-abs {g:.2e}", flush=True)
+    assert rel < 1e-4
+    assert g < 1e-3

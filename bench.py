@@ -71,6 +71,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-batch", action="store_true", help="skip the batched and differentiable measurements")
     p.add_argument("--spmv-reps", type=int, default=200)
+    p.add_argument("--block-profile", action="store_true",
+                   help="only run the mixed pipeline with the block eigensolver (eigensolver = 1) three times, "
+                        "for a kernel trace")
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the workload of the rocprofv3 counter passes: one 256 MiB device copy "
                         "(calibration), 20 launches of the selected SpMV shape and 100 mixed Lanczos steps, "
@@ -601,6 +604,21 @@ def cpu_baseline(rgb):
     }, r
 
 
+def block_solver(ctx, d_rgb, steps, warmup):
+    """The mixed pipeline with the block eigensolver (tune key "eigensolver"
+    = 1): seconds per image, the outputs, and one timed run's stage times."""
+    import damascene_amd as dm
+    from damascene_amd._lib import check, lib
+
+    check(lib().dmsc_tune_set(b"eigensolver", 1))
+    try:
+        t, out = time_pipeline(ctx, d_rgb, dm.PREC_MIXED, steps, warmup)
+        stage = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
+    finally:
+        check(lib().dmsc_tune_set(b"eigensolver", 0))
+    return t, out, stage
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -625,8 +643,14 @@ def main():
     rgb = dm.generate_image(W, H, CELLS, SEED)
     ctx = dm.Context(0)
     d_rgb = ctx.upload(rgb)
+    if a.block_profile:
+        t, _, st = block_solver(ctx, d_rgb, 3, 1)
+        print(json.dumps({"block_solver_ms_per_image": round(t * 1e3, 3), "block_products": st["lanczos_iterations"],
+                          "stage_ms": [round(v, 3) for v in st["stage_ms"]]}))
+        return
     t_plain, out_plain = time_pipeline(ctx, d_rgb, dm.PREC_FP32, a.steps, a.warmup)
     t_mixed, out_mixed = time_pipeline(ctx, d_rgb, dm.PREC_MIXED, a.steps, a.warmup)
+    t_block, out_block, stage_block = block_solver(ctx, d_rgb, a.steps, a.warmup)
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
     batch = None if a.no_batch else batched(ctx, out_plain, out_mixed, max(3, a.steps), a.warmup)
@@ -665,6 +689,12 @@ def main():
         "roofline": roof,
         "spmv": spmv,
     }
+    res["mixed_block_solver"] = {
+        "what": "mixed pipeline with the opt-in block eigensolver (tune key eigensolver = 1: Chebyshev-filtered "
+                "subspace iteration on 32 vectors, FP64 Rayleigh-Ritz); not the default",
+        "images_per_s": round(1.0 / t_block, 3), "ms_per_image": round(t_block * 1e3, 3),
+        "stage_ms": [round(v, 3) for v in stage_block["stage_ms"]],
+        "block_products": stage_block["lanczos_iterations"]}
     if diff:
         res["differentiable"] = diff
     if big:
@@ -677,6 +707,7 @@ def main():
         truth = np.load(TRUTH)
         res["plain_fp32"]["err_vs_fp64"] = errors_vs_truth(out_plain, truth)
         res["mixed"]["err_vs_fp64"] = errors_vs_truth(out_mixed, truth)
+        res["mixed_block_solver"]["err_vs_fp64"] = errors_vs_truth(out_block, truth)
     if not a.no_cpu:
         res["cpu_baseline"], ref = cpu_baseline(rgb)
         for name, out in (("plain_fp32", out_plain), ("mixed", out_mixed)):

@@ -265,7 +265,7 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_shape")) {
-    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 22 or 42");
+    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22 or 42");
     chebyshev_shape = value;
     return DMSC_OK;
   }

@@ -40,7 +40,7 @@
 namespace dmsc {
 
 int eigensolver_method = 0;  // mixed path: 0 Lanczos, 1 Chebyshev block (tune key "eigensolver")
-int chebyshev_shape = 22;    // filter tile: rows per lane * 10 + float4 planes per lane (22 or 42)
+int chebyshev_shape = 24;    // filter tile: rows per lane * 10 + float4 planes per lane (24, 28, 22, 42)
 
 namespace {
 
@@ -380,12 +380,14 @@ int launch_step(hipStream_t s, int w, int h, const float* A, size_t pitch, const
 
 int step(int shape, hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, const float4* z, float4* y, size_t vs, double alpha, double c, double beta) {
   if (shape == 42) return launch_step<4, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 24) return launch_step<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 28) return launch_step<2, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   return launch_step<2, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
 }
 
 }  // namespace
 
-bool chebyshev_shape_valid(int v) { return v == 22 || v == 42; }
+bool chebyshev_shape_valid(int v) { return v == 22 || v == 42 || v == 24 || v == 28; }
 
 // The nvec smallest eigenpairs of L_sym (A, already converted): unit
 // eigenvectors to d_out[j * n + p], eigenvalues to vals; *products = the
@@ -447,7 +449,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     }
     float4* Y = ring[cur];
     float4* Wb = ring[(cur + 1) % 3];
-    DMSC_TRY((launch_step<4, 1, true>(s, w, h, A, pitchz, Y, nullptr, Wb, vs, 1.0, 0.0, 0.0)));
+    DMSC_TRY((launch_step<2, 4, true>(s, w, h, A, pitchz, Y, nullptr, Wb, vs, 1.0, 0.0, 0.0)));
     hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, 3), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
     hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(3 * kB * kB, 256)), dim3(256), 0, s, nblk, part, gram);
     DMSC_HIP(hipMemcpyAsync(h_gram, gram, 3 * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -112,7 +112,7 @@ extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25
 extern int batch_lanes;
 // Mixed path eigensolver: 0 Lanczos, 1 Chebyshev-filtered block subspace
 // iteration (chebyshev.hip); its filter tile (rows per lane * 10 + float4
-// planes per lane: 22 or 42).
+// planes per lane: 24, 28, 22 or 42).
 extern int eigensolver_method;
 extern int chebyshev_shape;
 bool chebyshev_shape_valid(int v);

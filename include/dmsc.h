@@ -233,7 +233,8 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *                         iteration; the iteration count it reports is the
  *                         number of block products with the matrix.
  *   "chebyshev_shape":    its filter kernel's tile, rows per lane * 10 +
- *                         float4 planes per lane: 22 (the default) or 42.
+ *                         float4 planes per lane: 24 (the default), 28, 22
+ *                         or 42.
  * Returns DMSC_ERR_ARG for an unknown key or value. */
 /* Measurement (no reference counterpart; it backs the roofline figure of
  * the benchmark): converts d_matrix in place as the eigensolver does, runs

@@ -294,5 +294,5 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     v = ctypes.c_int()
     for key, want in ((b"lanczos_tile", 3208), (b"lanczos_max_iter", 6000), (b"lanczos_mixed_stop", 4),
                       (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 0),
-                      (b"chebyshev_shape", 22)):
+                      (b"chebyshev_shape", 24)):
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key

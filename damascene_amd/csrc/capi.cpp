@@ -46,7 +46,7 @@ Ctx::~Ctx() {
 }
 
 // Pinned block layout: bytes [0, 256) the pipeline's k-means iteration
-// count, [1024, 4096) the eigensolver's status record, [8192, 40960) the
+// count, [1024, 4096) the eigensolver's status record, [8192, 49152) the
 // block eigensolver's Gram matrices and rotation.
 int Ctx::pinned(int** out) {
   if (!h_pinned) DMSC_HIP(hipHostMalloc((void**)&h_pinned, 65536, hipHostMallocDefault));

@@ -21,7 +21,10 @@
 //   4. stop when every wanted Ritz vector moved by less than the bound
 //      since the previous iteration: sin of the angle between
 //      X0 e_j and Y s_j, from C.  Convergence is linear, so that change
-//      bounds the previous iterate's error and exceeds the new one's.
+//      bounds the previous iterate's error and exceeds the new one's.  Or
+//      stop when every wanted residual |A y - theta y| (from K = W'W) is at
+//      most tol * 1e-4, the mixed Lanczos bound: vectors of a degenerate
+//      eigenvalue need not stop moving.
 // Blocks are FP32 in HBM as kB/4 planes of float4 per pixel; the filter
 // sums in FP32, the Gram matrices and the rotation in FP64.  Matrix products
 // use the reference's stencil semantics: the neighbour at offset (dy, dx)
@@ -52,8 +55,9 @@ constexpr int kMaxOuter = 60;
 constexpr double kFirstCut = 1e-2;  // damped interval of the first filter: [kFirstCut, 2]
 constexpr double kUpper = 2.0;      // spectrum of L_sym lies in [0, 2]
 constexpr int kGramPx = 64;         // pixels staged per Gram round
-// pinned host block: Gram matrices at [8 KiB, 32 KiB), the rotation at [32 KiB, 40 KiB)
-constexpr size_t kPinGram = 8192, kPinRot = 32768;
+// pinned host block: Gram matrices at [8 KiB, 40 KiB), the rotation at [40 KiB, 48 KiB)
+constexpr size_t kPinGram = 8192, kPinRot = 40960;
+constexpr int kGrams = 4;  // Y'Y, Y'W, X0'Y, W'W
 
 // rows of the radius-5 disc at column offset dx: |dy| <= half_height(dx)
 __host__ __device__ constexpr int half_height(int dx) {
@@ -181,7 +185,7 @@ __global__ void k_cheb_init(int n, size_t vs, float4* __restrict__ x) {
 }
 
 // Partial Gram matrices over one chunk of pixels, FP64: blockIdx.y = 0
-// G = Y'Y, 1 H = Y'W, 2 C = X0'Y (entry [i][j] = U_i . V_j).  Thread t
+// G = Y'Y, 1 H = Y'W, 2 C = X0'Y, 3 K = W'W (entry [i][j] = U_i . V_j).  Thread t
 // owns row i = t / 8 and columns 4 (t % 8) .. +3.
 __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, const float4* __restrict__ Y,
                                                    const float4* __restrict__ Wb, const float4* __restrict__ X0,
@@ -189,8 +193,8 @@ __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, 
   __shared__ float su[kGramPx][kB + 1];
   __shared__ float4 sv[kGramPx][kPlanes];
   const int which = blockIdx.y;
-  const float4* U = which == 2 ? X0 : Y;
-  const float4* V = which == 1 ? Wb : Y;
+  const float4* U = which == 2 ? X0 : which == 3 ? Wb : Y;
+  const float4* V = which == 1 || which == 3 ? Wb : Y;
   const int i = threadIdx.x >> 3, j4 = threadIdx.x & 7;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int p0 = blockIdx.x * chunk, p1 = min(n, p0 + chunk);
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, 
 // Sums the partials of each Gram entry in block order (deterministic).
 __global__ void k_cheb_gram_sum(int nblk, const double* __restrict__ part, double* __restrict__ gram) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * kB * kB) return;
+  if (e >= kGrams * kB * kB) return;
   const int which = e / (kB * kB), idx = e % (kB * kB);
   const double* p = part + (size_t)which * nblk * (kB * kB) + idx;
   double s = 0.0;
@@ -401,8 +405,8 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   const int chunk = std::max(kGramPx, div_up(div_up(n, 512), kGramPx) * kGramPx);
   const int nblk = div_up(n, chunk);
   const size_t blk = (size_t)kPlanes * vs * sizeof(float4);
-  const size_t o_part = 4 * blk, o_gram = o_part + (size_t)3 * nblk * kB * kB * sizeof(double),
-               o_rot = o_gram + (size_t)3 * kB * kB * sizeof(double),
+  const size_t o_part = 4 * blk, o_gram = o_part + (size_t)kGrams * nblk * kB * kB * sizeof(double),
+               o_rot = o_gram + (size_t)kGrams * kB * kB * sizeof(double),
                bytes = o_rot + (size_t)kB * kB * sizeof(double);
   void* base;
   DMSC_TRY(ctx->scratch(3, bytes, &base));
@@ -430,7 +434,8 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   std::vector<double> theta, S;
   const double bound = 0.03 * (double)tol;  // sin of the largest per-vector move between iterations
   double cut = kFirstCut;
-  int deg = kDeg, total = 0;
+  int deg = kDeg, total = 0, flat = 0;
+  double best = HUGE_VAL;
   for (int outer = 0; outer < kMaxOuter; outer++) {
     // scaled Chebyshev filter of degree deg damping [cut, kUpper], normalised at 0
     const double e = (kUpper - cut) / 2, c = (kUpper + cut) / 2;
@@ -450,12 +455,28 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     float4* Y = ring[cur];
     float4* Wb = ring[(cur + 1) % 3];
     DMSC_TRY((launch_step<2, 4, true>(s, w, h, A, pitchz, Y, nullptr, Wb, vs, 1.0, 0.0, 0.0)));
-    hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, 3), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
-    hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(3 * kB * kB, 256)), dim3(256), 0, s, nblk, part, gram);
-    DMSC_HIP(hipMemcpyAsync(h_gram, gram, 3 * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, kGrams), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
+    hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(kGrams * kB * kB, 256)), dim3(256), 0, s, nblk, part, gram);
+    DMSC_HIP(hipMemcpyAsync(h_gram, gram, kGrams * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));
     DMSC_HIP(hipStreamSynchronize(s));
     if (!rayleigh_ritz(h_gram, h_gram + kB * kB, theta, S))
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: filtered block lost rank at outer iteration %d", outer);
+    // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j:
+    // s'K s - theta^2 (s'G s = 1, s'H s = theta), exact to FP64 rounding
+    // of terms of size theta^2
+    double resid = 0.0;
+    {
+      const double* K = h_gram + 3 * kB * kB;
+      for (int j = 0; j < nvec; j++) {
+        double q = 0.0;
+        for (int a = 0; a < kB; a++) {
+          double r = 0.0;
+          for (int b = 0; b < kB; b++) r += K[a * kB + b] * S[b * kB + j];
+          q += S[a * kB + j] * r;
+        }
+        resid = std::max(resid, std::sqrt(std::max(0.0, q - theta[j] * theta[j])));
+      }
+    }
     double move = 0.0;
     if (outer > 0) {
       const double* C = h_gram + 2 * kB * kB;
@@ -466,11 +487,20 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       }
     }
     if (lanczos_trace)
-      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e\n", outer, deg, cut,
-                   theta[0], theta[nvec - 1], move);
+      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e residual %.3e\n",
+                   outer, deg, cut, theta[0], theta[nvec - 1], move, resid);
     std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
     DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
-    const bool done = outer > 0 && move <= bound;
+    // done when the wanted vectors stopped moving, or (a degenerate
+    // eigenvalue leaves its vectors free to turn) when every residual meets
+    // the mixed Lanczos bound, tol * 1e-4
+    // or when the residuals stopped falling (three outer iterations without
+    // halving) below 1e-6: the floor that FP32 storage of the block sets
+    // (about 2^-24 |A| per vector entry), reached on small images whose
+    // larger eigenvalues put it above tol * 1e-4
+    flat = resid < 0.5 * best ? 0 : flat + 1;
+    best = std::min(best, resid);
+    const bool done = outer > 0 && (move <= bound || resid <= 1e-4 * (double)tol || (flat >= 3 && resid <= 1e-6));
     if (done) {
       hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, nvec, nullptr, d_out);
       DMSC_HIP(hipGetLastError());

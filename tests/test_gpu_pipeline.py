@@ -231,14 +231,15 @@ def test_large_image_fp64_residuals(ctx):
     assert chk["eigval_vs_rayleigh_max_rel"] < 1e-5
 
 
-def test_block_solver_large_image(ctx):
-    """The opt-in block eigensolver (tune key "eigensolver" = 1) on a
-    780x1001 image of the uneven batch, whose denser spectrum needs filter
-    degrees of about 150: its eigenvalues and gPb against the Lanczos
-    solver's (both mixed precision)."""
-    w, h, seed = 780, 1001, 20261202 + 142
+@pytest.mark.parametrize("w,h,seed", [(780, 1001, 20261202 + 142), (160, 120, 20261202), (61, 37, 7)])
+def test_block_solver_large_image(ctx, w, h, seed):
+    """The opt-in block eigensolver (tune key "eigensolver" = 1) against the
+    Lanczos solver (both mixed precision): eigenvalues and gPb on a 780x1001
+    image of the uneven batch, whose denser spectrum needs filter degrees of
+    about 150, on the 160x120 test image, and on a 61x37 image narrower than
+    one 64-column filter tile."""
     print(f"case: block solver {w}x{h} seed {seed}", flush=True)
-    img = ctx.upload(dm.generate_image(w, h, w * h // 4000, seed))
+    img = ctx.upload(dm.generate_image(w, h, max(4, w * h // 4000), seed))
     ref = ctx.gpb(img, precision=dm.PREC_MIXED)
     check(lib().dmsc_tune_set(b"eigensolver", 1))
     try:

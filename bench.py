@@ -4,7 +4,7 @@
 
 A step is one full gPb pass (RGB8 resident in HBM -> oriented gPb) over one
 481x321 image.  `value` is images/s of the mixed-precision path (FP64-
-accumulated eigensolver converged to residual tol*1e-3); the plain FP32 path
+accumulated eigensolver converged to residual tol*1e-4); the plain FP32 path
 (the reference algorithm) is timed beside it, and both report their errors
 against the FP64 ground truth in tests/golden/truth_481x321.npz (eigenvalue
 relative error, subspace angle of eigenvectors 1..8, gPb max-abs error,
@@ -284,11 +284,18 @@ def batch_images(ctx):
     import damascene_amd as dm
 
     same = [ctx.upload(dm.generate_image(W, H, CELLS, SEED + i)) for i in range(BATCH)]
+    specs = uneven_specs()
+    uneven = [ctx.upload(dm.generate_image(*s)) for s in specs]
+    return same, uneven, [s[:2] for s in specs]
+
+
+def uneven_specs():
+    """(w, h, cells, seed) of the uneven 64-image batch: width and height
+    each uniform in [200, 1024] (drawn with seed SEED), cells = area / 4000
+    (at least 4), image seeds SEED + 100 + i."""
     rng = np.random.default_rng(SEED)
     sizes = [(int(rng.integers(200, 1025)), int(rng.integers(200, 1025))) for _ in range(UNEVEN)]
-    uneven = [ctx.upload(dm.generate_image(w, h, max(4, w * h // 4000), SEED + 100 + i))
-              for i, (w, h) in enumerate(sizes)]
-    return same, uneven, sizes
+    return [(w, h, max(4, w * h // 4000), SEED + 100 + i) for i, (w, h) in enumerate(sizes)]
 
 
 def time_batch(ctx, rgbs, precision, steps, warmup):
@@ -386,29 +393,57 @@ def differentiable(ctx, steps=200, images=16, lr=1e-2):
 
 
 def stage_roofline(stage, iterations, kmeans_iterations, mixed):
-    """Per-stage roofline fractions of one 481x321 image from the stage times
-    (HIP events): each stage's algorithmic work over its whole time against
-    the bound that stage is written for (a lower bound where a stage mixes
-    kinds of work).  FP32 VALU peak 157.3 TFLOP/s, HBM 8 TB/s."""
+    """Per-stage roofline fractions of one 481x321 image from a timed run with
+    every stage in order on one stream (tune key overlap_stages = 0), so each
+    window holds exactly its own kernels: [6] colour + texton filter bank,
+    [0] - [6] k-means, [7] rotated labels + integral images + chi-squared
+    gradients (the histogram stage), [1] - [7] parabola fits, [4] eigensolve,
+    [5] spectral Pb + gPb.  Each stage's algorithmic work over its window,
+    against the bound it is written for: FP32 VALU 157.3 TFLOP/s, BF16 MFMA
+    2.5 PFLOP/s (the mixed k-means distances), HBM 8 TB/s."""
     n = W * H
-    valu = 157.3e12
+    valu, bf16 = 157.3e12, 2.5e15
     t = [v * 1e-3 for v in stage]  # seconds
-    textons = 2 * 9010 * n + kmeans_iterations * 2 * n * 32 * 34   # filter bank + k-means distances
+    bank = 2 * 9010 * n
+    km = kmeans_iterations * 2 * n * 32 * 34
     parabola = 2 * 58720 * n
     hist = 890.3e6
     eig = iterations * (STEP_BYTES_PER_ROW if mixed else 352) * n
     spb = 6272 * n
+    t_bank, t_km = t[6], t[0] - t[6]
+    t_hist, t_par = t[7], t[1] - t[7]
+    km_peak = bf16 if mixed else valu
     return {
-        "textons": {"bound": "fp32 valu", "work": "filter bank 2*9010 flop/px + k-means 2*N*32*34 flop/iteration",
-                    "achieved_tflops": round(textons / t[0] / 1e12, 2), "frac": round(textons / t[0] / valu, 4)},
-        "localcues": {"bound": "fp32 valu (parabola) and hbm (histograms)",
-                      "parabola_tflops": round(parabola / t[1] / 1e12, 2), "parabola_frac": round(parabola / t[1] / valu, 4),
-                      "histogram_gbs": round(hist / t[1] / 1e9, 1), "histogram_frac": round(hist / t[1] / 8e12, 4)},
-        "eigensolve": {"bound": "hbm", "work": f"{iterations} Lanczos steps x {STEP_BYTES_PER_ROW if mixed else 352} B/row",
+        "timing": "stages in order on one stream (overlap_stages 0); HIP events between stages",
+        "texton_bank": {"bound": "fp32 valu", "work": "2*9010 flop/px", "ms": round(t_bank * 1e3, 3),
+                        "achieved_tflops": round(bank / t_bank / 1e12, 2), "frac": round(bank / t_bank / valu, 4)},
+        "kmeans": {"bound": "bf16 mfma" if mixed else "fp32 valu", "ms": round(t_km * 1e3, 3),
+                   "work": f"{kmeans_iterations} iterations x 2*N*32*34 flop",
+                   "achieved_tflops": round(km / t_km / 1e12, 2), "frac": round(km / t_km / km_peak, 5)},
+        "histogram": {"bound": "hbm", "work": "890.3 MB (u16 integral images written once and read once, labels, "
+                                              "chi-squared planes)", "ms": round(t_hist * 1e3, 3),
+                      "achieved_gbs": round(hist / t_hist / 1e9, 1), "frac": round(hist / t_hist / 8e12, 4)},
+        "parabola": {"bound": "fp32 valu", "work": "2*58720 flop/px", "ms": round(t_par * 1e3, 3),
+                     "achieved_tflops": round(parabola / t_par / 1e12, 2),
+                     "frac": round(parabola / t_par / valu, 4)},
+        "eigensolve": {"bound": "hbm", "ms": round(t[4] * 1e3, 3),
+                       "work": f"{iterations} Lanczos steps x {STEP_BYTES_PER_ROW if mixed else 352} B/row",
                        "achieved_gbs": round(eig / t[4] / 1e9, 1), "frac": round(eig / t[4] / 8e12, 4)},
-        "spectralpb_gpb": {"bound": "fp32 valu", "work": "6272 flop/px",
+        "spectralpb_gpb": {"bound": "fp32 valu", "work": "6272 flop/px", "ms": round(t[5] * 1e3, 3),
                            "achieved_tflops": round(spb / t[5] / 1e12, 2), "frac": round(spb / t[5] / valu, 4)},
     }
+
+
+def sequential_stages(ctx, d_rgb, precision):
+    """One timed gPb with the stages in order on one stream."""
+    from damascene_amd._lib import check, lib
+
+    check(lib().dmsc_tune_set(b"overlap_stages", 0))
+    try:
+        ctx.gpb(d_rgb, precision=precision, timing=True)
+        return ctx.gpb(d_rgb, precision=precision, timing=True)
+    finally:
+        check(lib().dmsc_tune_set(b"overlap_stages", 1))
 
 
 def fp64_residuals(ctx, out, w, h):
@@ -566,10 +601,17 @@ def errors_vs_truth(out, truth):
 
 def cpu_baseline(rgb):
     """The oracle's plain-FP32 pipeline in reference order on the host cores
-    (OpenMP): one warm-up, then the median of three runs of the 481x321 image
-    with every thread; beside it one thread on the 160x120 image (the
-    reference's CPU-runnable configuration, 12 cells), after a warm-up."""
+    (OpenMP), in its timing mode: the texton bank, parabola and spectral Pb
+    correlations and the interior rows of the SpMV run as register-tiled
+    SIMD loops (AVX-512 where the host has it, else AVX2; the technique of
+    the reference's cpu_convolve2d.cc) with every output's operation
+    sequence unchanged, so its results equal the scalar oracle's bit for
+    bit.  One warm-up, then the median of five runs of the 481x321 image
+    with every thread; beside it one run of the scalar loops, and one thread
+    on the 160x120 image (the reference's CPU-runnable configuration, 12
+    cells) after a warm-up."""
     import ctypes
+    import platform
     import statistics
 
     import damascene_amd as dm
@@ -578,29 +620,49 @@ def cpu_baseline(rgb):
     lib = O.lib()
     lib.orc_max_threads.restype = ctypes.c_int
     threads = lib.orc_max_threads()
-    r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
-    runs = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
-        runs.append(time.perf_counter() - t0)
-    dt = statistics.median(runs)
-    small = dm.generate_image(160, 120, 12, SEED)
-    lib.orc_set_threads(1)
+    lib.orc_set_simd(0)
+    t0 = time.perf_counter()
+    scalar = O.pipeline(rgb, nvec=NVEC, tol=TOL)
+    dt_scalar = time.perf_counter() - t0
+    lib.orc_set_simd(1)
     try:
-        O.pipeline(small, nvec=NVEC, tol=TOL)
-        t0 = time.perf_counter()
-        O.pipeline(small, nvec=NVEC, tol=TOL)
-        dt1 = time.perf_counter() - t0
+        r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
+        runs = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r = O.pipeline(rgb, nvec=NVEC, tol=TOL)
+            runs.append(time.perf_counter() - t0)
+        dt = statistics.median(runs)
+        same = all(np.array_equal(np.asarray(r[k]).view(np.uint32), np.asarray(scalar[k]).view(np.uint32))
+                   for k in ("textons", "nmax", "gpb"))
+        small = dm.generate_image(160, 120, 12, SEED)
+        lib.orc_set_threads(1)
+        try:
+            O.pipeline(small, nvec=NVEC, tol=TOL)
+            t0 = time.perf_counter()
+            O.pipeline(small, nvec=NVEC, tol=TOL)
+            dt1 = time.perf_counter() - t0
+        finally:
+            lib.orc_set_threads(threads)
     finally:
-        lib.orc_set_threads(threads)
+        lib.orc_set_simd(0)
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), cpu)
+    except OSError:
+        pass
     return {
         "value": round(1.0 / dt, 4), "unit": "images/s", "cores": threads, "kind": "port",
-        "sample": f"one {W}x{H} image, whole plain-FP32 gPb in reference order (oracle, OpenMP, {threads} "
-                  f"threads): median of 3 runs after a warm-up, {dt:.2f} s",
+        "sample": f"one {W}x{H} image, whole plain-FP32 gPb in reference order (oracle, OpenMP, {threads} threads, "
+                  f"register-tiled SIMD correlations and SpMV, {lib.orc_simd_width()} columns per tile): median of "
+                  f"5 runs after a warm-up, {dt:.2f} s",
         "stage_ms": [round(v, 1) for v in r["stage_ms"]],
+        "simd_results_identical_to_scalar": bool(same),
+        "scalar_loops_s_per_image": round(dt_scalar, 3),
         "one_thread_160x120": {"images_per_s": round(1.0 / dt1, 4), "s_per_image": round(dt1, 3),
                                "nproc": os.cpu_count()},
+        "host_cpu": cpu,
     }, r
 
 
@@ -653,6 +715,8 @@ def main():
     t_block, out_block, stage_block = block_solver(ctx, d_rgb, a.steps, a.warmup)
     stage_plain = ctx.gpb(d_rgb, precision=dm.PREC_FP32, timing=True)
     stage_mixed = ctx.gpb(d_rgb, precision=dm.PREC_MIXED, timing=True)
+    seq_plain = sequential_stages(ctx, d_rgb, dm.PREC_FP32)
+    seq_mixed = sequential_stages(ctx, d_rgb, dm.PREC_MIXED)
     batch = None if a.no_batch else batched(ctx, out_plain, out_mixed, max(3, a.steps), a.warmup)
     diff = None if a.no_batch else differentiable(ctx)
     big = None if a.no_batch else large(ctx, 2)
@@ -675,17 +739,18 @@ def main():
         "config": {"workload": f"gpb-{W}x{H}-single-image", "width": W, "height": H, "cells": CELLS,
                    "n_eig": NVEC, "tol": TOL, "textons": 32},
         "plain_fp32": {"images_per_s": round(1.0 / t_plain, 3), "ms_per_image": round(t_plain * 1e3, 3),
-                       "stage_ms": [round(v, 3) for v in stage_plain["stage_ms"]],
+                       "stage_ms": [round(v, 3) for v in stage_plain["stage_ms"][:6]],
                        "lanczos_iterations": stage_plain["lanczos_iterations"],
-                       "stage_roofline": stage_roofline(stage_plain["stage_ms"], stage_plain["lanczos_iterations"],
-                                                        stage_plain["kmeans_iterations"], False)},
+                       "stage_roofline": stage_roofline(seq_plain["stage_ms"], seq_plain["lanczos_iterations"],
+                                                        seq_plain["kmeans_iterations"], False)},
         "mixed": {"images_per_s": round(1.0 / t_mixed, 3), "ms_per_image": round(t_mixed * 1e3, 3),
-                  "stage_ms": [round(v, 3) for v in stage_mixed["stage_ms"]],
+                  "stage_ms": [round(v, 3) for v in stage_mixed["stage_ms"][:6]],
                   "lanczos_iterations": stage_mixed["lanczos_iterations"],
-                  "stage_roofline": stage_roofline(stage_mixed["stage_ms"], stage_mixed["lanczos_iterations"],
-                                                   stage_mixed["kmeans_iterations"], True)},
+                  "stage_roofline": stage_roofline(seq_mixed["stage_ms"], seq_mixed["lanczos_iterations"],
+                                                   seq_mixed["kmeans_iterations"], True)},
         "speedup_mixed_vs_plain": round(t_plain / t_mixed, 3),
-        "stage_names": ["textons", "localcues", "combine+nonmax", "intervene", "eigensolve", "spectralPb+gPb"],
+        "stage_names": ["textons (+ overlapped L/a/b local cues)", "local cues (rest)", "combine+nonmax",
+                        "intervene", "eigensolve", "spectralPb+gPb"],
         "roofline": roof,
         "spmv": spmv,
     }

@@ -21,6 +21,7 @@ __all__ = [
 
 PREC_FP32 = 0
 PREC_MIXED = 1
+STOP_RESIDUAL, STOP_STEADY, STOP_FLOOR, STOP_CAP = 1, 2, 3, 4
 TEXTON32 = 1
 TEXTON64 = 2
 
@@ -119,6 +120,30 @@ class Context:
         if normalize:
             check(lib().dmsc_normalize_lab(self.handle, w, h, _p(L), _p(a), _p(b)))
         return L, a, b
+
+    def mirror_image(self, img, border):
+        """convert.h:75/97 mirrorImage: symmetric-reflect padding of a float32
+        or int32 [h, w] image by `border` pixels on every side."""
+        h, w = img.shape
+        out = self.empty(h + 2 * border, w + 2 * border, dtype=img.dtype)
+        fn = lib().dmsc_mirror_image_i if img.dtype == self._torch.int32 else lib().dmsc_mirror_image_f
+        check(fn(self.handle, w, h, border, _p(img), _p(out)))
+        return out
+
+    def unmirror_image(self, img, border):
+        """convert.h:120 unMirrorImage: the interior of a mirrored float32
+        image."""
+        bh, bw = img.shape
+        out = self.empty(bh - 2 * border, bw - 2 * border)
+        check(lib().dmsc_unmirror_image_f(self.handle, bw, bh, border, _p(img), _p(out)))
+        return out
+
+    def quantize_image(self, img, nbins):
+        """convert.h:140 quantizeImage: floor(v * nbins), nbins -> nbins - 1."""
+        h, w = img.shape
+        out = self.empty(h, w, dtype=self._torch.int32)
+        check(lib().dmsc_quantize_image(self.handle, w, h, nbins, _p(img), _p(out)))
+        return out
 
     def texton_responses(self, grey):
         h, w = grey.shape
@@ -220,6 +245,15 @@ class Context:
                                                 _p(vals), _p(vecs), ctypes.byref(it)))
         return vals, vecs, it.value
 
+    def eigensolve_report(self):
+        """How the last eigensolve on this context stopped: (stop rule, the
+        largest residual of its wanted pairs); rules as DMSC_STOP_* in
+        dmsc.h (1 residual bound met, 2 vectors steady, 3 FP32 floor, 4
+        iteration cap)."""
+        rule, res = ctypes.c_int(), ctypes.c_double()
+        check(lib().dmsc_eigensolve_report(self.handle, ctypes.byref(rule), ctypes.byref(res)))
+        return rule.value, res.value
+
     def spectral_pb(self, eigvals, eigvecs):
         nvec, h, w = eigvecs.shape
         pitch = pitch_in_floats(w * h)
@@ -274,7 +308,7 @@ class Context:
         out["eigvals"] = np.array(o.h_eigvals[:n_eig], np.float32)
         out["kmeans_iterations"] = o.kmeans_iterations
         out["lanczos_iterations"] = o.lanczos_iterations
-        out["stage_ms"] = list(o.stage_ms[:6])
+        out["stage_ms"] = list(o.stage_ms[:8])
         return out
 
     def gpb(self, rgb, n_eig=9, tol=1e-3, precision=PREC_FP32, timing=False, texton_choice=TEXTON32):

@@ -18,6 +18,8 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+int overlap_stages = 1;
+
 int fail(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -269,6 +271,16 @@ int dmsc_tune_set(const char* key, int value) {
     chebyshev_shape = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "lanczos_step_loads")) {
+    DMSC_ARG(value >= 0 && value <= 2, "lanczos_step_loads must be 0, 1 or 2");
+    lanczos_step_loads = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "overlap_stages")) {
+    DMSC_ARG(value == 0 || value == 1, "overlap_stages must be 0 or 1");
+    overlap_stages = value;
+    return DMSC_OK;
+  }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
 }
 int dmsc_tune_get(const char* key, int* value) {
@@ -315,6 +327,14 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "chebyshev_shape")) {
     *value = chebyshev_shape;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "lanczos_step_loads")) {
+    *value = lanczos_step_loads;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "overlap_stages")) {
+    *value = overlap_stages;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -440,6 +460,14 @@ int dmsc_kmeans_rechecks(dmsc_ctx* ctx, int* rechecks) {
   DMSC_ARG(rechecks, "null pointer");
   DMSC_ARG(ctx->arena[2] && ctx->arena_bytes[2] >= kmeans_state_bytes(), "no k-means run on this context");
   return kmeans_read_state(ctx->stream, ctx->arena[2], nullptr, nullptr, rechecks);
+}
+
+int dmsc_eigensolve_report(dmsc_ctx* ctx, int* stop_rule, double* max_residual) {
+  CHECK_CTX(ctx);
+  DMSC_ARG(stop_rule && max_residual, "null pointer");
+  *stop_rule = ctx->eig_stop;
+  *max_residual = ctx->eig_residual;
+  return DMSC_OK;
 }
 
 int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* grey, int* textons, int texton_choice,
@@ -708,9 +736,13 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   float* gpb = out->d_gpb ? out->d_gpb : (float*)(B0 + o_gpb);
 
   int* kit = nullptr;
-  hipEvent_t ev[7] = {};
+  // ev[0..6]: stage boundaries; ev[7]: after the texton filter bank, ev[8]:
+  // after the chi-squared gradients (finer splits, timed when the stages run
+  // in order on one stream)
+  hipEvent_t ev[9] = {};
+  const bool overlap = ctx->overlap_stages && overlap_stages;
   if (timing) {
-    for (int i = 0; i < 7; i++) DMSC_HIP(hipEventCreate(&ev[i]));
+    for (int i = 0; i < 9; i++) DMSC_HIP(hipEventCreate(&ev[i]));
     DMSC_HIP(hipEventRecord(ev[0], s));
   }
   // damascene.cu:269-285 grey + textons; 309-317 Lab (gamma 2.5), normalised
@@ -721,7 +753,7 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   DMSC_TRY(local_cues_prepare(ctx, w, h, ntex));
-  if (ctx->overlap_stages) {
+  if (overlap) {
     DMSC_TRY(ctx->aux_stream(&aux, &fork, &join));
     DMSC_HIP(hipEventRecord(fork, s));
     DMSC_HIP(hipStreamWaitEvent(aux, fork, 0));
@@ -732,6 +764,7 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   const float* coeffs;
   DMSC_TRY(ctx->texton_coeffs(&coeffs));
   DMSC_TRY(launch_texton_conv(s, w, h, grey, coeffs, resp));
+  if (timing) DMSC_HIP(hipEventRecord(ev[7], s));
   {
     void* scr;
     const size_t st = (kmeans_state_bytes() + 255) & ~(size_t)255;
@@ -749,9 +782,9 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   }
   if (timing) DMSC_HIP(hipEventRecord(ev[1], s));
   // damascene.cu:352 localCues, texton channel; then join the L/a/b part
-  DMSC_TRY(launch_local_cues_part(ctx, s, ctx->overlap_stages ? 2 : 3, w, h, L, A, Bc, tex, ntex, cues,
-                                  cues + 24 * P, cues + 48 * P, cues + 72 * P, pitch, nullptr));
-  if (ctx->overlap_stages) DMSC_HIP(hipStreamWaitEvent(s, join, 0));
+  DMSC_TRY(launch_local_cues_part(ctx, s, overlap ? 2 : 3, w, h, L, A, Bc, tex, ntex, cues, cues + 24 * P,
+                                  cues + 48 * P, cues + 72 * P, pitch, nullptr, timing ? ev[8] : nullptr));
+  if (overlap) DMSC_HIP(hipStreamWaitEvent(s, join, 0));
   if (timing) DMSC_HIP(hipEventRecord(ev[2], s));
   // damascene.cu:388-404 combine + nonMaxSuppression
   DMSC_TRY(launch_combine(s, n, pitch, cues, cues + 24 * P, cues + 48 * P, cues + 72 * P, ntex, mpb, cg));
@@ -773,7 +806,11 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
     DMSC_HIP(hipEventSynchronize(ev[6]));
     for (int i = 0; i < 6; i++) DMSC_HIP(hipEventElapsedTime(&out->stage_ms[i], ev[i], ev[i + 1]));
     out->stage_ms[6] = out->stage_ms[7] = 0.f;
-    for (int i = 0; i < 7; i++) (void)hipEventDestroy(ev[i]);
+    if (!overlap) {
+      DMSC_HIP(hipEventElapsedTime(&out->stage_ms[6], ev[0], ev[7]));
+      DMSC_HIP(hipEventElapsedTime(&out->stage_ms[7], ev[1], ev[8]));
+    }
+    for (int i = 0; i < 9; i++) (void)hipEventDestroy(ev[i]);
   }
   return DMSC_OK;
 }

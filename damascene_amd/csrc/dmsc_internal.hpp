@@ -40,6 +40,10 @@ struct Ctx {
   // lanes do not (the lanes already overlap whole images, and a second
   // stream per lane makes them contend for the hardware queues)
   bool overlap_stages = true;
+  // how the last eigensolve on this context stopped (DMSC_STOP_*) and the
+  // largest residual |A u - lambda u| of its wanted pairs at that point
+  int eig_stop = 0;
+  double eig_residual = 0.0;
   ~Ctx();
   int pinned(int** out);
   int aux_stream(hipStream_t* s, hipEvent_t* ev0, hipEvent_t* ev1);

@@ -34,9 +34,12 @@ int launch_local_cues(Ctx* ctx, int w, int h, const float* L, const float* A, co
 // The local cues in two parts for two streams: part 1 the L, a, b channels
 // (needs no textons), part 2 the texton channel, 3 both; prepare first.
 int local_cues_prepare(Ctx* ctx, int w, int h, int ntex);
+// hist_done (may be null) is recorded on s between the chi-squared
+// gradients and the parabola fits.
 int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, const float* L, const float* A,
                            const float* B, const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg,
-                           int pitch, float* grads_out);
+                           int pitch, float* grads_out, hipEvent_t hist_done = nullptr);
+extern int overlap_stages;  // dmsc_gpb: L/a/b local cues beside the textons (tuning key "overlap_stages")
 
 // ------------------------------------------------------------- combine / nonmax / gPb
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga, const float* cgb,
@@ -106,6 +109,10 @@ extern int lanczos_mixed_fused;
 // Tile of the tiled fused mixed step, TW * 100 + TH: one of kStepTiles
 // (3208, the default, first).
 extern int lanczos_tile;
+// Tiled fused mixed step: when a lane issues its 81 matrix loads (0 in
+// groups of 9, 1 all after the staging barrier, 2 all before the partial
+// reduction); results are identical.
+extern int lanczos_step_loads;
 extern int lanczos_check_max;  // mixed path: most steps between checks (0 = a quarter of the steps so far)
 extern int lanczos_mixed_stop;  // mixed path: stop when every residual is <= tol * 10^-this (default 4)
 extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25)

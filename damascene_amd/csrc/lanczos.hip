@@ -59,6 +59,7 @@ int lanczos_tile = 3208;
 int lanczos_max_iter = kMaxIter;
 int lanczos_check_max = 0;
 int lanczos_mixed_stop = 4;
+int lanczos_step_loads = 0;
 
 namespace {
 
@@ -433,7 +434,13 @@ __global__ __launch_bounds__(NT) void k_lzmf(LzmBufs b, StencilOffsets so, int t
 // across image rows at the left and right edges -- or 0 outside [0, n), so
 // every product is the one k_lzmf forms.  Blocks are numbered XCD-aware
 // over the row-major tile order.
-template <int TW, int TH>
+//
+// LOADS selects when the 81 matrix loads of a lane are issued: 0 in groups
+// of 9 inside the multiply-add loop, 1 all at once after the staging
+// barrier, 2 all at once before the partial-sum reduction (so their
+// latency overlaps the reduction and the staging).  The FP64 sums run in
+// diagonal order in all three, so the results are bit-identical.
+template <int TW, int TH, int LOADS>
 __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, int w, int h, int t) {
   constexpr int NT = TW * TH, SW = TW + 2 * kStencilRadius, SH = TH + 2 * kStencilRadius;
   __shared__ float sq[SW * SH];
@@ -451,20 +458,51 @@ __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, 
   const int x = tx0 + lx, y = ty0 + ly;
   const bool own = x < w && y < h;
   const int row = own ? y * w + x : 0;
+  // Every load of the prologue is issued before the first one is used, with
+  // clamped (in-range) indices instead of branches, in the order their
+  // values are needed: the previous step's partial sums (the critical
+  // path: alpha and beta), then the staged inputs, then (LOADS 2) the
+  // matrix.  The hardware returns loads in order, so each use waits only
+  // for the loads issued before it.
+  constexpr int kPart = 4;  // partials per thread kept in registers; larger grids loop for the rest
+  double pv[kPart][3];
+#pragma unroll
+  for (int j = 0; j < kPart; j++) {
+    const int i = (int)threadIdx.x + j * NT;
+    const size_t ic = po + (size_t)(i < b.nb1 ? i : 0);
+    pv[j][0] = b.p1[ic];
+    pv[j][1] = b.p2[ic];
+    pv[j][2] = b.p3[ic];
+  }
   constexpr int kPre = (SW * SH + NT - 1) / NT;
   double zr[kPre];
   float qr[kPre];
 #pragma unroll
-  for (int j = 0; j < kPre; j++) {  // staged inputs first: their latency overlaps the reduction
+  for (int j = 0; j < kPre; j++) {
     const int i = (int)threadIdx.x + j * NT;
     const int sy = i / SW, sx = i - sy * SW;
     const int c = (ty0 - kStencilRadius + sy) * w + (tx0 - kStencilRadius + sx);
-    const bool ok = i < SW * SH && (unsigned)c < (unsigned)b.n;
-    zr[j] = ok ? zp[c] : 0.0;
-    qr[j] = ok ? qm[c] : 0.0f;
+    const int cc = (i < SW * SH && (unsigned)c < (unsigned)b.n) ? c : 0;  // selected away below
+    zr[j] = zp[cc];
+    qr[j] = qm[cc];
   }
+  const float qmo = qm[row];  // q_{k-1} at the lane's own pixel
+  float am[LOADS ? kStencilArea : 1];
+  if constexpr (LOADS == 2) {
+#pragma unroll
+    for (int d = 0; d < kStencilArea; d++) am[d] = b.A[(size_t)d * b.pitch + row];
+  }
+  // sums in the order of a plain strided loop from 0: bit-identical to it
   double v1 = 0, v2 = 0, v3 = 0;
-  for (int i = threadIdx.x; i < b.nb1; i += NT) {
+#pragma unroll
+  for (int j = 0; j < kPart; j++) {
+    if ((int)threadIdx.x + j * NT < b.nb1) {
+      v1 += pv[j][0];
+      v2 += pv[j][1];
+      v3 += pv[j][2];
+    }
+  }
+  for (int i = threadIdx.x + kPart * NT; i < b.nb1; i += NT) {
     v1 += b.p1[po + i];
     v2 += b.p2[po + i];
     v3 += b.p3[po + i];
@@ -493,11 +531,20 @@ __global__ __launch_bounds__(TW * TH) void k_lzmt(LzmBufs b, StencilOffsets so, 
     const float qf = xs[0];
     V[(size_t)k * b.vstride + row] = qf;
     double acc = 0.0;  // FP64 accumulation of exact FP32 products, diagonal order
+    if constexpr (LOADS == 0) {
 #pragma unroll 9
-    for (int d = 0; d < kStencilArea; d++)
-      acc = fma((double)b.A[(size_t)d * b.pitch + row], (double)xs[so.dy[d] * SW + so.dx[d]], acc);
+      for (int d = 0; d < kStencilArea; d++)
+        acc = fma((double)b.A[(size_t)d * b.pitch + row], (double)xs[so.dy[d] * SW + so.dx[d]], acc);
+    } else {
+      if constexpr (LOADS == 1) {
+#pragma unroll
+        for (int d = 0; d < kStencilArea; d++) am[d] = b.A[(size_t)d * b.pitch + row];
+      }
+#pragma unroll
+      for (int d = 0; d < kStencilArea; d++) acc = fma((double)am[d], (double)xs[so.dy[d] * SW + so.dx[d]], acc);
+    }
     const double qv = (double)qf;
-    const double zj = fma(-beta, (double)qm[row], acc);
+    const double zj = fma(-beta, (double)qmo, acc);
     zc[row] = zj;
     d1 = qv * zj;
     d2 = zj * zj;
@@ -980,6 +1027,7 @@ struct Steps {
   int k1;            // K1 block size
   size_t fused_lds;  // mixed path: LDS bytes of the linear fused step kernel, 0 = not used
   int tile_w, tile_h;  // mixed path: tile of the tiled fused step, 0 = not used
+  int loads;           // tiled fused step: matrix load schedule (k_lzmt's LOADS)
   int w, h;
   LzBufs plain;
   LzmBufs mix;
@@ -1014,8 +1062,11 @@ int add_steps(hipGraph_t g, const Steps& p, int steps) {
     if (p.mixed && p.tile_h) {
       const int code = p.tile_w * 100 + p.tile_h;
       const void* fn = nullptr;
-#define TILE_FN(TW, TH) \
-  if (code == TW * 100 + TH) fn = (const void*)&k_lzmt<TW, TH>;
+#define TILE_FN(TW, TH)                                                        \
+  if (code == TW * 100 + TH)                                                   \
+    fn = p.loads == 2   ? (const void*)&k_lzmt<TW, TH, 2>                      \
+         : p.loads == 1 ? (const void*)&k_lzmt<TW, TH, 1>                      \
+                        : (const void*)&k_lzmt<TW, TH, 0>;
       TILE_FN(32, 8) TILE_FN(64, 4) TILE_FN(16, 16) TILE_FN(32, 16) TILE_FN(16, 8) TILE_FN(64, 8)
       TILE_FN(64, 2) TILE_FN(32, 4)
 #undef TILE_FN
@@ -1042,7 +1093,7 @@ int run_steps(Ctx* ctx, const Steps& p, LzState* st, int w, int first, int steps
   GraphCache* gc = graph_cache(ctx);
   const GraphKey key{p.mixed ? (const void*)p.mix.A : (const void*)p.plain.A, st,
                      (int)(p.mixed ? p.mix.pitch : p.plain.pitch), p.mixed ? p.mix.n : p.plain.n, w, p.mixed ? 1 : 0,
-                     steps, p.k1, (int)p.fused_lds, p.tile_w * 100 + p.tile_h};
+                     steps, p.k1, (int)p.fused_lds, p.loads * 10000 + p.tile_w * 100 + p.tile_h};
   auto it = gc->execs.find(key);
   if (it == gc->execs.end()) {
     if ((int)gc->execs.size() >= kMaxGraphs) gc->clear();
@@ -1229,6 +1280,7 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   p.fused_lds = fused_lds;
   p.tile_w = tile_w;
   p.tile_h = tile_h;
+  p.loads = lanczos_step_loads;
   p.w = w;
   p.h = h;
   p.so = make_stencil_offsets(w);
@@ -1441,6 +1493,12 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
   if (!converged && !(hs.fail == 0 && hs.nscreened >= nvec && hs.m >= max_it - kGraphSteps))
     return fail(DMSC_ERR_NOCONV, "Lanczos did not converge in %d iterations", max_it);
   const int m = hs.m;
+  {
+    double r = 0;
+    for (int j = 0; j < nvec; j++) r = std::max(r, hs.res[j]);
+    ctx->eig_stop = converged ? DMSC_STOP_RESIDUAL : DMSC_STOP_CAP;
+    ctx->eig_residual = r;
+  }
 
   // Ritz vectors, D^-1/2 scaling (stencilMVM.cu:182-198), min-max normalisation
   DMSC_TRY(launch_ritz(s, mixed, nvec, n, st, vstride, m, S, d_eigvecs));

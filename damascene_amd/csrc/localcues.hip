@@ -576,7 +576,7 @@ int local_cues_prepare(Ctx* ctx, int w, int h, int ntex) {
 
 int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, const float* L, const float* A,
                            const float* B, const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg,
-                           int pitch, float* grads_out) {
+                           int pitch, float* grads_out, hipEvent_t hist_done) {
   if (ntex != 32 && ntex != 64) return fail(DMSC_ERR_ARG, "local cues support 32 or 64 textons");
   if (part < 1 || part > 3) return fail(DMSC_ERR_INTERNAL, "bad local-cue part %d", part);
   const int tbp = ntex;
@@ -631,6 +631,7 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
       hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
                          P->d_tabs, I, none, grads, plane);
   }
+  if (hist_done) DMSC_HIP(hipEventRecord(hist_done, s));
   CueOut out{{bg, cga, cgb, tg}};
   for (int scale = 0; scale < 3; scale++) {
     // (first channel, channels, radius index): Bg; Cga and Cgb; Tg

@@ -83,6 +83,8 @@ Stencil::Stencil(int radiusIn, int widthIn, int heightIn, int matrixPitchInFloat
   const int area = dmsc_stencil(radius, width, hostOffsets, diagonalOffsets);
   if (area <= 0) throw std::runtime_error(std::string("Stencil: ") + dmsc_last_error_string());
   stencilArea = (uint)area;
+  for (int i = area; i < STENCILAREAMAX; i++) hostOffsets[i] = -2000000000;  // Stencil.cpp:29-31
+  for (int i = diameter * diameter; i < CONSTSPACE; i++) diagonalOffsets[i] = -1;
 }
 uint Stencil::getStencilArea() { return stencilArea; }
 // Index of matrix entry (row, col) in the DIA storage: pitch * diagonal +
@@ -103,7 +105,7 @@ int Stencil::matrixIndex(int row, int col) {
   const int d = diagonalOffsets[(dy + radius) * diameter + dx + radius];
   return d < 0 ? -1 : matrixPitchInFloats * d + row;
 }
-void Stencil::copyOffsets(int* destOffsets) { std::memcpy(destOffsets, hostOffsets, sizeof(int) * stencilArea); }
+void Stencil::copyOffsets(int* destOffsets) { std::memcpy(destOffsets, hostOffsets, sizeof(int) * STENCILAREAMAX); }
 int Stencil::getMatrixPitch() { return matrixPitchInFloats * (int)sizeof(float); }
 int Stencil::getMatrixPitchInFloats() { return matrixPitchInFloats; }
 int Stencil::getHeight() { return height; }
@@ -112,6 +114,43 @@ int Stencil::getRadius() { return radius; }
 int Stencil::getDiameter() { return diameter; }
 void Stencil::copyDiagonalOffsets(int* destOffsets) {
   std::memcpy(destOffsets, diagonalOffsets, sizeof(int) * diameter * diameter);
+}
+
+// Stencil.cpp:92-145 readStencilMatrix (.sma: n, nnz, then per row the
+// count, the FP64 values and the int columns), each value rounded to float
+// at its DIA slot.
+float* Stencil::readStencilMatrix(char* filename) {
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(filename, "rb"), std::fclose);
+  if (!fp) throw std::runtime_error(std::string("readStencilMatrix: cannot open ") + filename);
+  const size_t total = (size_t)stencilArea * (size_t)matrixPitchInFloats;
+  std::unique_ptr<float, void (*)(void*)> out((float*)std::calloc(total ? total : 1, sizeof(float)), std::free);
+  if (!out) throw std::runtime_error("readStencilMatrix: out of host memory");
+  int n = 0, nnz = 0;
+  if (std::fread(&n, sizeof(int), 1, fp.get()) != 1 || std::fread(&nnz, sizeof(int), 1, fp.get()) != 1)
+    throw std::runtime_error("readStencilMatrix: truncated header");
+  if (n != width * height)
+    throw std::runtime_error("readStencilMatrix: " + std::to_string(n) + " rows for a " + std::to_string(width) +
+                             "x" + std::to_string(height) + " stencil");
+  if (nnz < 0 || (long long)nnz >= (long long)stencilArea * n)
+    throw std::runtime_error("readStencilMatrix: bad entry count " + std::to_string(nnz));
+  std::vector<double> z(stencilArea);
+  std::vector<int> x(stencilArea);
+  for (int row = 0; row < n; row++) {
+    int nz = 0;
+    if (std::fread(&nz, sizeof(int), 1, fp.get()) != 1 || nz < 0 || nz > (int)stencilArea)
+      throw std::runtime_error("readStencilMatrix: bad entry count in row " + std::to_string(row));
+    if (std::fread(z.data(), sizeof(double), nz, fp.get()) != (size_t)nz ||
+        std::fread(x.data(), sizeof(int), nz, fp.get()) != (size_t)nz)
+      throw std::runtime_error("readStencilMatrix: truncated row " + std::to_string(row));
+    for (int k = 0; k < nz; k++) {
+      const int idx = (x[k] >= 0 && x[k] < n) ? matrixIndex(row, x[k]) : -1;
+      if (idx < 0 || (size_t)idx >= total)
+        throw std::runtime_error("readStencilMatrix: entry (" + std::to_string(row) + ", " + std::to_string(x[k]) +
+                                 ") is outside the stencil");
+      out.get()[idx] = (float)z[k];
+    }
+  }
+  return out.release();
 }
 
 // ------------------------------------------------------------------ convert

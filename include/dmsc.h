@@ -161,11 +161,26 @@ int dmsc_stencil_spmv(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int ma
  * reorthogonalisation, Cullum screening every 500 steps, residual test at
  * tol); DMSC_PREC_MIXED = FP32 matrix and FP32-stored basis with FP64 SpMV
  * accumulation, three-term vector and inner products, Cullum screening on
- * an adaptive schedule, residuals at tol * 1e-3.  The FP64 tridiagonal
+ * an adaptive schedule, residuals at tol * 1e-4 (the "lanczos_mixed_stop"
+ * tuning value).  The FP64 tridiagonal
  * eigensolve runs on the GPU in both. */
 int dmsc_generalized_eigensolve(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch,
                                 int nvec, float tol, int precision, float* h_eigvals,
                                 float* d_eigvecs, int* iterations);
+/* No reference counterpart: how the last dmsc_generalized_eigensolve (or
+ * dmsc_gpb) on this context stopped, and the largest residual
+ * |L_sym u - lambda u| of its wanted pairs at that point (the Lanczos
+ * estimate |beta_m s_mj|, or the block solver's FP64 Gram residual).
+ *   DMSC_STOP_RESIDUAL: every residual met the tolerance (plain: tol; mixed:
+ *                       tol * 10^-lanczos_mixed_stop);
+ *   DMSC_STOP_STEADY:   block solver: no wanted Ritz vector moved by more
+ *                       than 0.03 tol since the previous outer iteration;
+ *   DMSC_STOP_FLOOR:    block solver: the residuals stagnated above the
+ *                       tolerance at the FP32 storage floor (about 4.8e-7);
+ *   DMSC_STOP_CAP:      Lanczos reached its iteration cap and returned the
+ *                       last check's Ritz pairs (lanczos.cu:519-560). */
+enum { DMSC_STOP_NONE = 0, DMSC_STOP_RESIDUAL = 1, DMSC_STOP_STEADY = 2, DMSC_STOP_FLOOR = 3, DMSC_STOP_CAP = 4 };
+int dmsc_eigensolve_report(dmsc_ctx* ctx, int* stop_rule, double* max_residual);
 /* spectralPb.h:19 spectralPb -> [8][pitch] (accumulates into d_spb after
  * zeroing it). */
 int dmsc_spectral_pb(dmsc_ctx* ctx, const float* h_eigvals, const float* d_eigvecs, int w,
@@ -227,14 +242,23 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  *   "eigensolver":        mixed precision only: 0 Lanczos (the default), 1
  *                         Chebyshev-filtered subspace iteration on a block of
  *                         32 vectors with an FP64 Rayleigh-Ritz step (no
- *                         reference counterpart; nvec <= 16).  It stops when no
- *                         wanted Ritz vector moved by more than 0.03 tol
- *                         (sine of the angle) since the previous outer
- *                         iteration; the iteration count it reports is the
- *                         number of block products with the matrix.
+ *                         reference counterpart; nvec <= 16).  It stops when
+ *                         every wanted residual is at most
+ *                         tol * 10^-lanczos_mixed_stop, when no wanted Ritz
+ *                         vector moved by more than 0.03 tol (sine of the
+ *                         angle) since the previous outer iteration, or when
+ *                         the residuals stagnate at the FP32 floor
+ *                         (dmsc_eigensolve_report tells which); the iteration
+ *                         count it reports is the number of block products
+ *                         with the matrix.
  *   "chebyshev_shape":    its filter kernel's tile, rows per lane * 10 +
  *                         float4 planes per lane: 24 (the default), 28, 22
  *                         or 42.
+ *   "overlap_stages":     dmsc_gpb runs the L, a and b local cues on the
+ *                         context's second stream beside the textons (1, the
+ *                         default) or every stage in order on one stream (0;
+ *                         results are bit-identical, and per-stage timing
+ *                         then splits the textons and the local cues).
  * Returns DMSC_ERR_ARG for an unknown key or value. */
 /* Measurement (no reference counterpart; it backs the roofline figure of
  * the benchmark): converts d_matrix in place as the eigensolver does, runs
@@ -298,8 +322,14 @@ typedef struct {
   float h_eigvals[32];
   int kmeans_iterations;
   int lanczos_iterations;
-  float stage_ms[8];     /* textons, localcues, combine+nonmax, intervene,
-                            eigensolve, spectralPb+gPb (when timing is on) */
+  float stage_ms[8];     /* when timing is on: [0] colour + textons (+ the
+                            L/a/b local cues when they overlap), [1] local
+                            cues (the rest), [2] combine + nonmax, [3]
+                            intervene, [4] eigensolve, [5] spectralPb + gPb;
+                            when the stages do not overlap ("overlap_stages"
+                            0): [6] colour + texton filter bank (part of
+                            [0]) and [7] rotated labels, integral images and
+                            chi-squared gradients (part of [1]); else 0 */
 } dmsc_gpb_outputs;
 
 void dmsc_gpb_default_params(dmsc_gpb_params* p);

@@ -94,6 +94,8 @@ int  orc_tridiag_eigvec(int n, const double* alpha, const double* beta, double l
 int  orc_lanczos_fp32(int w, int h, const float* lsym, int nvec, float tol,
                       float* eigvals, float* eigvecs /*[nvec][N], u = D^1/2 v basis*/,
                       int* iterations);
+/* the Cullum-Willoughby screening step of the above on a given T (size i+1) */
+int  orc_cullum_screen(int i, const float* alpha, const float* beta, int eigCheck, double* vals);
 /* FP64 oracle: Lanczos with full reorthogonalisation on the same L_sym */
 int  orc_lanczos_fp64(int w, int h, const float* lsym, int nvec, double tol, int maxiter,
                       double* eigvals, double* eigvecs, int* iterations);
@@ -116,6 +118,17 @@ void orc_normalize_gpb_all(int npix, int norient, int pitch, float* all);
 
 /* whole pipeline (plain FP32 reference order); returns 0 on success */
 void orc_set_threads(int n);  /* OpenMP threads of the parallel loops */
+/* Timing mode of the CPU baseline: the texton bank, parabola and spectral Pb
+ * correlations run as register-tiled SIMD loops (orc_simd.cpp, the
+ * technique of the reference's cpu_convolve2d.cc) with every output's
+ * operation sequence unchanged, so results are identical to the default. */
+void orc_set_simd(int on);
+int  orc_simd_width(void);  /* columns per register tile: 32 (AVX-512) or 16 (AVX2) */
+void orc_spmv_interior_simd(const float* M, int n, const float* x, float* y, int r0, int r1,
+                            const int* lin, int nd);
+void orc_correlate_simd(const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int fh,
+                        int fw, int order /* 0: filter rows outer, 1: filter columns outer */,
+                        const float* f /* fh x fw, row-major */);
 int  orc_max_threads(void);
 int  orc_gpb_pipeline(int w, int h, const uint32_t* rgb, int nvec, float tol,
                       int* textons, float* mpb8, float* cg8, float* nmax, float* eigvals,

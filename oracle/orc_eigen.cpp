@@ -178,8 +178,7 @@ int tridiag_vectors(int n, const double* a, const double* b, const double* lam, 
 // (first row/column removed); merge values closer than LUMPTOL=1e-5, drop
 // non-duplicate values that are also eigenvalues of T-hat (relative distance
 // <= 1e-10); take the first nEig survivors and their Ritz vectors.
-bool cullum(int i, const float* alpha, const float* beta, int eigCheck, int nEig, double* vals,
-            std::vector<double>& S) {
+bool screen(int i, const float* alpha, const float* beta, int eigCheck, std::vector<double>& scr) {
   int tn = i;
   std::vector<double> ta(i + 1), tb(i + 1), wc(eigCheck), wr(eigCheck);
   for (int j = 0; j < tn; j++) ta[j] = alpha[j + 1];
@@ -200,7 +199,7 @@ bool cullum(int i, const float* alpha, const float* beta, int eigCheck, int nEig
     }
     if (accept) { acc.push_back(wr[j]); dup.push_back(0); }
   }
-  std::vector<double> scr;
+  scr.clear();
   for (size_t j = 0; j < acc.size(); j++) {
     bool accept = true;
     if (!dup[j])
@@ -208,8 +207,19 @@ bool cullum(int i, const float* alpha, const float* beta, int eigCheck, int nEig
         if (std::fabs((acc[j] - wc[k]) / acc[j]) <= 1e-10) accept = false;
     if (accept) scr.push_back(acc[j]);
   }
+  return true;
+}
+
+bool cullum(int i, const float* alpha, const float* beta, int eigCheck, int nEig, double* vals,
+            std::vector<double>& S) {
+  std::vector<double> scr;
+  if (!screen(i, alpha, beta, eigCheck, scr)) return false;
   if ((int)scr.size() < nEig) return false;
   for (int j = 0; j < nEig; j++) vals[j] = scr[j];
+  const int tn = i + 1;
+  std::vector<double> ta(tn), tb(tn, 0.0);
+  for (int j = 0; j < tn; j++) ta[j] = alpha[j];
+  for (int j = 0; j < tn - 1; j++) tb[j] = beta[j];
   S.assign((size_t)nEig * tn, 0.0);
   return tridiag_vectors(tn, ta.data(), tb.data(), vals, nEig, S.data()) == 0;
 }
@@ -217,6 +227,18 @@ bool cullum(int i, const float* alpha, const float* beta, int eigCheck, int nEig
 }  // namespace
 
 extern "C" {
+
+// lanczos.cu:279-384 CullumDevice's screening alone: T = tridiag(alpha[0..i],
+// beta[0..i-1]) and T-hat = T without its first row and column; the
+// screened Ritz values of the first eigCheck values of T, ascending, into
+// vals (at most eigCheck); returns their number, or -1 if a tridiagonal
+// eigenvalue solve failed.
+int orc_cullum_screen(int i, const float* alpha, const float* beta, int eigCheck, double* vals) {
+  std::vector<double> scr;
+  if (!screen(i, alpha, beta, eigCheck, scr)) return -1;
+  for (size_t j = 0; j < scr.size(); j++) vals[j] = scr[j];
+  return (int)scr.size();
+}
 
 // lanczos.cu:386-733 lanczos (no reorthogonalisation) on L_sym.  FP32
 // vectors and SpMV; inner products accumulate in double and are rounded to

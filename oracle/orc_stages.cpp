@@ -13,6 +13,7 @@
 
 namespace {
 inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+int g_simd = 0;  // orc_set_simd: register-tiled SIMD correlations (same results)
 inline int mirror_index(int i, int border, int n) {  // convert.cu:128-141
   if (i < border) return border - i - 1;
   if (i < border + n) return i - border;
@@ -24,6 +25,7 @@ extern "C" {
 
 // OpenMP threads of the oracle's parallel loops (timing of the CPU baseline).
 void orc_set_threads(int n) { omp_set_num_threads(n); }
+void orc_set_simd(int on) { g_simd = on ? 1 : 0; }
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
 
@@ -109,6 +111,22 @@ void orc_texton_convolve(int w, int h, const float* grey, float* resp) {
   static int radii[34];
   int nbound = orc_texton_filters(coeffs, radii);
   const int n = w * h;
+  if (g_simd) {  // the same sums over a clamp-padded plane, register-tiled
+    const int R = 9, pw = w + 2 * R, ph = h + 2 * R;
+    std::vector<float> pad((size_t)pw * ph), f(19 * 19);
+    for (int yy = 0; yy < ph; yy++)
+      for (int xx = 0; xx < pw; xx++)
+        pad[(size_t)yy * pw + xx] = grey[clampi(yy - R, 0, h - 1) * w + clampi(xx - R, 0, w - 1)];
+    int ci = 0;
+    for (int fi = 0; fi < 34; fi++) {
+      const int r = radii[fi], L = 2 * r + 1;
+      for (int i = 0; i < L; i++)  // coefficients run x offset outer, y offset inner
+        for (int j = 0; j < L; j++, ci++) f[j * L + i] = (ci < nbound) ? coeffs[ci] : 0.0f;
+      orc_correlate_simd(pad.data() + (size_t)(R - r) * pw + (R - r), pw, resp + (size_t)fi * n, w, w, h, L, L, 1,
+                         f.data());
+    }
+    return;
+  }
 #pragma omp parallel for schedule(dynamic, 4)
   for (int y = 0; y < h; y++) {
     for (int x = 0; x < w; x++) {
@@ -354,6 +372,20 @@ void orc_channel_gradients(int w, int h, int nbins, int blur, float sigma, const
 void orc_parabola(int w, int h, int border, int r, const float* g8, const float* filt,
                   float* out8) {
   const int len = 2 * r + 1, n = w * h;
+  if (g_simd) {  // the same sums over the mirrored plane, register-tiled
+    const int pw = w + 2 * r, ph = h + 2 * r;
+    std::vector<float> pad((size_t)pw * ph);
+    for (int o = 0; o < 8; o++) {
+      const float* g = g8 + (size_t)o * n;
+#pragma omp parallel for schedule(static)
+      for (int yy = 0; yy < ph; yy++) {
+        const int sy = mirror_index(yy - r + border, border, h);
+        for (int xx = 0; xx < pw; xx++) pad[(size_t)yy * pw + xx] = g[sy * w + mirror_index(xx - r + border, border, w)];
+      }
+      orc_correlate_simd(pad.data(), pw, out8 + (size_t)o * n, w, w, h, len, len, 0, filt + (size_t)o * len * len);
+    }
+    return;
+  }
 #pragma omp parallel for collapse(2) schedule(static)
   for (int o = 0; o < 8; o++)
     for (int y = 0; y < h; y++) {
@@ -696,8 +728,22 @@ void orc_spmv(int w, int h, const float* M, const float* x, float* y) {
   const int n = w * h;
   int offs[121], diag[121];
   int nd = orc_stencil(5, w, offs, diag);
+  int r0 = 0, r1 = 0;
+  if (g_simd) {  // rows whose every column is in range, vectorised in the same order
+    int lin[121], c = 0, lo = 0, hi = 0;
+    for (int d = 0; d < nd; d++) {
+      c += offs[d];
+      lin[d] = c;
+      lo = std::min(lo, c);
+      hi = std::max(hi, c);
+    }
+    r0 = -lo;
+    r1 = std::max(r0, n - hi);
+    orc_spmv_interior_simd(M, n, x, y, r0, r1, lin, nd);
+  }
 #pragma omp parallel for schedule(static)
   for (int row = 0; row < n; row++) {
+    if (row >= r0 && row < r1) continue;
     float acc = 0;
     int col = row;
     for (int d = 0; d < nd; d++) {
@@ -734,9 +780,23 @@ void orc_spectral_pb(int w, int h, int nvec, const float* eigvals, const float* 
   orc_spb_filters(K);
   memset(spb, 0, sizeof(float) * 8 * (size_t)n);
   std::vector<float> m((size_t)bw * bh);
+  std::vector<float> so(g_simd ? (size_t)8 * n : 0), ko(49);
   for (int i = 1; i < nvec; i++) {
     orc_mirror_f(w, h, 3, eigvecs + (size_t)i * n, m.data());
     float scale = 1.0f / sqrtf(eigvals[i]);
+    if (g_simd) {  // per orientation, the same sums (u outer, v inner), register-tiled
+      for (int o = 0; o < 8; o++) {
+        for (int t = 0; t < 49; t++) ko[t] = K[o * 49 + t];  // [v + 3][u + 3]
+        orc_correlate_simd(m.data(), bw, so.data() + (size_t)o * n, w, w, h, 7, 7, 1, ko.data());
+      }
+#pragma omp parallel for schedule(static)
+      for (int p = 0; p < n; p++)
+        for (int o = 0; o < 8; o++) {
+          float& r = spb[(size_t)o * n + p];
+          r = fmaf(fabsf(so[(size_t)o * n + p]), scale, r);
+        }
+      continue;
+    }
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; y++)
       for (int x = 0; x < w; x++) {

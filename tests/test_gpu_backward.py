@@ -16,7 +16,12 @@ dmsc_spb_combine_backward behind torch.autograd).
   and their responses nearly 0, and on the edge frame the mirrored windows
   nearly cancel), with more than 30 % of the 4-pixel edge frame (the mirror
   adjoint) compared; and 1e-3 in relative L2 overall.
-* Refit: Adam on the cue weights and gamma lowers the loss.
+* Refit: Adam on the cue weights and gamma lowers the loss, on one image
+  and on the 16-image batch of 481 x 321 images (the differentiable
+  configuration).
+
+The forward and backward checks run at 160 x 120 and at the full 481 x 321
+size.
 """
 import numpy as np
 import pytest
@@ -28,11 +33,12 @@ from oracle import grad_ref as G
 
 pytestmark = pytest.mark.gpu
 
-W, H, CELLS, SEED = 160, 120, 12, 20261202
+SIZES = [(160, 120, 12, 20261202), (481, 321, 40, 20261202 + 200)]
 
 
-@pytest.fixture(scope="module")
-def data(ctx):
+@pytest.fixture(scope="module", params=SIZES, ids=["160x120", "481x321"])
+def data(ctx, request):
+    W, H, CELLS, SEED = request.param
     print(f"case: backward {W}x{H} cells={CELLS} seed={SEED}", flush=True)
     rgb, cells = dm.generate_image(W, H, CELLS, SEED, with_cells=True)
     mask = dm.boundary_mask(cells).astype(bool)
@@ -42,11 +48,11 @@ def data(ctx):
     L, a, b = ctx.rgb_to_lab(d_rgb)
     tex, _ = ctx.find_textons(grey)
     cues = ctx.local_cues(L, a, b, tex)
-    return out, cues, mask
+    return out, cues, mask, W, H
 
 
 def test_forward_bit_identical_to_pipeline(ctx, data):
-    out, cues, _ = data
+    out, cues, _, W, H = data
     n = W * H
     op = AG.SpbCombine(ctx, W, H, cues)
     prm = AG.reference_params(out["eigvals"], device=ctx.device)
@@ -61,7 +67,7 @@ def test_forward_bit_identical_to_pipeline(ctx, data):
 
 
 def test_backward_against_fp64(ctx, data):
-    out, cues, mask = data
+    out, cues, mask, W, H = data
     op = AG.SpbCombine(ctx, W, H, cues)
     prm = AG.reference_params(out["eigvals"], device=ctx.device).requires_grad_(True)
     vec = out["eigvecs"].clone().requires_grad_(True)
@@ -120,7 +126,7 @@ def test_backward_against_fp64(ctx, data):
 
 
 def test_refit_lowers_loss(ctx, data):
-    out, cues, mask = data
+    out, cues, mask, W, H = data
     op = AG.SpbCombine(ctx, W, H, cues)
     prm0 = AG.reference_params(out["eigvals"], device=ctx.device)
     wg = prm0[:13].clone().requires_grad_(True)
@@ -137,3 +143,36 @@ def test_refit_lowers_loss(ctx, data):
     print("refit losses", [round(v, 5) for v in losses[::5]], "->", round(losses[-1], 5), flush=True)
     assert np.all(np.isfinite(losses))
     assert losses[-1] < 0.9 * losses[0]
+
+
+def test_refit_batch_481x321(ctx):
+    """The differentiable configuration: 16 images of 481 x 321 (seeds
+    20261202 + 200 + i, the bench's), masks = cell boundaries dilated 1 px,
+    eigenvectors from the mixed eigensolver; Adam (lr 1e-2) on the 12 cue
+    weights and gamma over the mean loss of the 16 images lowers it."""
+    W, H, CELLS, N_IMG = 481, 321, 40, 16
+    print(f"case: refit over {N_IMG} images of {W}x{H}", flush=True)
+    ops, vecs, masks, etas = [], [], [], []
+    for i in range(N_IMG):
+        rgb, cells = dm.generate_image(W, H, CELLS, 20261202 + 200 + i, with_cells=True)
+        d_rgb = ctx.upload(rgb)
+        out = ctx.gpb(d_rgb, precision=dm.PREC_MIXED)
+        L, a, b = ctx.rgb_to_lab(d_rgb)
+        tex, _ = ctx.find_textons(ctx.rgb_to_grey(d_rgb))
+        ops.append(AG.SpbCombine(ctx, W, H, ctx.local_cues(L, a, b, tex)))
+        vecs.append(out["eigvecs"])
+        masks.append(torch.from_numpy(dm.boundary_mask(cells).astype(bool)).to(ctx.device))
+        etas.append(AG.reference_params(out["eigvals"], device=ctx.device)[13:])
+    wg = AG.reference_params(np.ones(9, np.float32), device=ctx.device)[:13].clone().requires_grad_(True)
+    opt = torch.optim.Adam([wg], lr=1e-2)
+    losses = []
+    for _ in range(40):
+        opt.zero_grad()
+        loss = sum(AG.balanced_bce(op(torch.cat([wg, e]), v), m)
+                   for op, v, m, e in zip(ops, vecs, masks, etas)) / N_IMG
+        loss.backward()
+        opt.step()
+        losses.append(loss.item())
+    print("batch refit losses", [round(v, 5) for v in losses[::8]], "->", round(losses[-1], 5), flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.85 * losses[0]

@@ -5,8 +5,8 @@ Integer and byte work (textons, quantised labels, histograms via the
 chi-squared planes, nonmax masks) and every FP32 stage whose operation order
 is fixed by contract (colour, filter bank, local cues, combine, matrix
 transform, SpMV, spectralPb, gPb) must match bit for bit.  The intervening
-contour uses the device expf: its matrix is compared with a relative
-tolerance of 1e-6 (1-ulp class differences of expf).
+contour's affinity matrix is bit-exact too: kernels and oracle share one
+project exp built from basic IEEE operations in place of the device expf.
 """
 import ctypes
 

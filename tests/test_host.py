@@ -191,9 +191,10 @@ STAGE_API_PROGRAM = r"""
 #include "texton.h"
 int main() {
   Stencil s(5, 481, 321, findPitchInFloats(481 * 321));
-  int offs[441], diag[441];
+  static int offs[STENCILAREAMAX], diag[CONSTSPACE];
   s.copyOffsets(offs);
   s.copyDiagonalOffsets(diag);
+  if (offs[s.getStencilArea()] != -2000000000 || offs[STENCILAREAMAX - 1] != -2000000000) return 3;
   long lin = 0;
   for (unsigned i = 0; i < s.getStencilArea(); i++) lin += offs[i];
   // last diagonal is (dy, dx) = (5, 0); entry (row, row + 5w) sits on it
@@ -232,6 +233,79 @@ def test_reference_stage_api_compiles_links_and_reports_errors(tmp_path):
     pitch = dm.pitch_in_floats(481 * 321)
     assert f"area 81 pitch {pitch} last {5 * 481} index {pitch * 80 + 1000}" in r.stdout
     assert "error:" in r.stdout
+
+
+READ_SMA_PROGRAM = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include "Stencil.h"
+int main(int argc, char** argv) {
+  Stencil s(5, 12, 10, 128);
+  try {
+    float* m = s.readStencilMatrix(argv[1]);
+    FILE* f = std::fopen(argv[2], "wb");
+    std::fwrite(m, sizeof(float), s.getStencilArea() * 128, f);
+    std::fclose(f);
+    std::free(m);
+  } catch (const std::runtime_error& e) {
+    std::printf("error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
+"""
+
+
+def write_sma(path, rows):
+    """Stencil.cpp:92-145 .sma layout: int n, int nnz, then per row: int
+    count, double values[count], int columns[count]."""
+    nnz = sum(len(r) for r in rows)
+    with open(path, "wb") as f:
+        f.write(np.array([len(rows), nnz], np.int32).tobytes())
+        for r in rows:
+            f.write(np.array([len(r)], np.int32).tobytes())
+            f.write(np.array([v for _, v in r], np.float64).tobytes())
+            f.write(np.array([c for c, _ in r], np.int32).tobytes())
+
+
+def test_read_stencil_matrix(tmp_path):
+    """Stencil::readStencilMatrix places each (row, column, value) of a .sma
+    file at its DIA slot (the diagonal of offset (dy, dx) = column - row split
+    across row wrap, Stencil.cpp:41-59) as a float, zero elsewhere, and
+    rejects a file whose row count is not width * height."""
+    w, h, pitch = 12, 10, 128
+    n = w * h
+    offs = [(dy, dx) for dy in range(-5, 6) for dx in range(-5, 6) if dy * dy + dx * dx <= 25]
+    rng = np.random.default_rng(17)
+    rows, want = [], np.zeros((81, pitch), np.float32)
+    for row in range(n):
+        entries = []
+        for d in rng.choice(81, size=6, replace=False):
+            dy, dx = offs[d]
+            col = row + dy * w + dx  # linear column: wraps across image rows like the reference
+            if 0 <= col < n:
+                v = float(rng.standard_normal())
+                entries.append((col, v))
+                want[d, row] = np.float32(v)
+        rows.append(entries)
+    sma = tmp_path / "m.sma"
+    write_sma(sma, rows)
+    src = tmp_path / "read_sma.cpp"
+    src.write_text(READ_SMA_PROGRAM)
+    exe = tmp_path / "read_sma"
+    libdir = os.path.join(ROOT, "damascene_amd")
+    subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I",
+                    "/opt/rocm/include", str(src), "-o", str(exe), "-L", libdir, "-ldmsc", f"-Wl,-rpath,{libdir}"],
+                   check=True, capture_output=True, text=True)
+    out = tmp_path / "m.f32"
+    r = subprocess.run([str(exe), str(sma), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, np.float32).reshape(81, pitch)
+    np.testing.assert_array_equal(got, want)
+    write_sma(sma, rows[:-1])
+    r = subprocess.run([str(exe), str(sma), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "rows for a 12x10 stencil" in r.stdout
 
 
 def test_result_file_formats(tmp_path):
@@ -296,3 +370,4 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
                       (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 0),
                       (b"chebyshev_shape", 24)):
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key
+

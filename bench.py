@@ -59,6 +59,12 @@ STEP_KERNEL = "k_lzmt<32, 8>"
 STEP_BYTES_PER_ROW = 81 * 4 + 8 + 4 + 4 + 8
 STEP_TIMED = 500
 TRUTH = os.path.join(ROOT, "tests", "golden", "truth_481x321.npz")
+# kernel groups for --pmc-summarize: (name regex, algorithmic bytes per image);
+# the histogram stage's bytes: u16 integral images written once and read
+# once, label frames, L/a/b/texton inputs and the 96 chi-squared planes
+PMC_GROUPS = {
+    "histogram": (r"k_fill_u8|k_cue_labels|k_integral_\w+|k_band_carry|k_gradients\w*", 890.3e6),
+}
 PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
                  ("lanczos_step_481x321_pmc.txt", "stencil_spmv_481x321_pmc.txt")]
 
@@ -178,7 +184,10 @@ def pmc_summarize(fetch_dir, write_dir, kernel):
             raise SystemExit(f"no {counter} values under {d}")
         return vals
 
-    pattern = re.escape(kernel) + r"\("  # the name as rocprofv3 prints it, up to the argument list
+    group = PMC_GROUPS.get(kernel)
+    # the name as rocprofv3 prints it, up to the argument list; a group
+    # sums every dispatch of its kernels (the workload's one gPb image)
+    pattern = (r"(?:^|::|\s)(?:" + group[0] + r")[<(]") if group else (re.escape(kernel) + r"\(")
     out = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         v = per_kernel(d, counter)
@@ -187,16 +196,20 @@ def pmc_summarize(fetch_dir, write_dir, kernel):
         if not copies or not hits:
             raise SystemExit(f"{counter}: copy kernel or {kernel} not found in {d}")
         cal = max(copies)
-        out[counter] = dict(copy_kib=cal, scale=262144.0 / cal, kib=sum(hits) / len(hits), launches=len(hits))
+        per = sum(hits) if group else sum(hits) / len(hits)
+        out[counter] = dict(copy_kib=cal, scale=262144.0 / cal, kib=per, launches=len(hits))
     f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
     traffic = (f["kib"] * f["scale"] + w["kib"] * w["scale"]) * 1024
-    algo = (STEP_BYTES_PER_ROW if kernel == STEP_KERNEL else SPMV_BYTES_PER_ROW) * W * H
+    if group:
+        algo = group[1]
+    else:
+        algo = (STEP_BYTES_PER_ROW if kernel == STEP_KERNEL else SPMV_BYTES_PER_ROW) * W * H
     lines = [
         f"calibration copy (256 MiB): FETCH_SIZE {f['copy_kib']:.0f} KiB (scale {f['scale']:.4f}), "
         f"WRITE_SIZE {w['copy_kib']:.0f} KiB (scale {w['scale']:.4f})",
         f"{kernel} ({f['launches']} launches): FETCH_SIZE {f['kib']:.1f} KiB, WRITE_SIZE {w['kib']:.1f} KiB "
-        f"per launch",
-        f"algorithmic bytes per launch {algo}",
+        f"{'summed over the image' if group else 'per launch'}",
+        f"algorithmic bytes {'per image' if group else 'per launch'} {algo:.0f}",
         f"traffic_bytes_per_launch {kernel} {traffic:.0f}",
         f"traffic / algorithmic = {traffic / algo:.3f}",
     ]
@@ -454,7 +467,9 @@ def fp64_residuals(ctx, out, w, h):
     the library's L_sym of the same image (FP32 entries, exact in FP64) and
     a plain DIA product over the 81 stencil offsets; and the Davis-Kahan bound
     sin(angle_j) <= |r_j| / gap_j against the computed neighbours (the gap
-    of the last vector to the unknown next eigenvalue is not bounded)."""
+    of the last vector to the unknown next eigenvalue is not bounded).
+    rounding_floor_j bounds what the FP32 storage of the normalised output
+    vector alone contributes to residual_j."""
     import torch
 
     n = w * h
@@ -468,7 +483,12 @@ def fp64_residuals(ctx, out, w, h):
     u0 = sq / sq.norm()
     V = out["eigvecs"].reshape(NVEC, n).double()[1:] * sq[None, :]
     V -= (V @ u0)[:, None] * u0[None, :]
-    V /= V.norm(dim=1, keepdim=True)
+    # what FP32 storage of the min-max mapped vectors alone leaves: entries in
+    # [0, 1] carry up to 2^-25, which D^1/2 scales and L (|L| <= 2) passes on,
+    # against the norm of the part that survives the projection
+    unorm = V.norm(dim=1)
+    floor = (2.0 * 2.0 ** -25 * sq.norm() / unorm).cpu().numpy()
+    V /= unorm[:, None]
     xp = torch.zeros(V.shape[0], n + 2 * pad, dtype=torch.float64, device=V.device)
     xp[:, pad:pad + n] = V
     LV = torch.zeros_like(V)
@@ -480,6 +500,7 @@ def fp64_residuals(ctx, out, w, h):
     got = np.asarray(out["eigvals"], np.float64)[1:]
     gaps = np.array([np.min(np.abs(lam[j] - np.delete(lam, j))) for j in range(len(lam))])
     return {"residual_max": float(res.max()), "residuals": [float(f"{r:.3e}") for r in res],
+            "rounding_floor": [float(f"{f:.3e}") for f in floor],
             "eigval_vs_rayleigh_max_rel": float(np.max(np.abs(got - lam) / lam)),
             "davis_kahan_sin_bound_max": float(np.max(res[:-1] / gaps[:-1]))}
 

@@ -60,6 +60,7 @@ SIGNATURES = {
     "dmsc_write_gpb_all": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, P]),
     "dmsc_write_pgm": (c_int, [ctypes.c_char_p, c_int, c_int, P]),
     "dmsc_lanczos_time_steps": (c_int, [P, c_int, c_int, P, c_int, c_int, c_int, P]),
+    "dmsc_chebyshev_time_products": (c_int, [P, c_int, c_int, P, c_int, c_int, P]),
     "dmsc_tune_set": (c_int, [ctypes.c_char_p, c_int]),
     "dmsc_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int)]),
     "dmsc_rgb_to_grey": (c_int, [P, c_int, c_int, P, P]),

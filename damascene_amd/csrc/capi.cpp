@@ -19,6 +19,7 @@ thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 int overlap_stages = 1;
+int integral_pass = 1;
 
 int fail(int code, const char* fmt, ...) {
   char buf[1024];
@@ -267,7 +268,7 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_shape")) {
-    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22 or 42");
+    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22, 42 (register tiles), 324, 344, 342, 328, 348 (LDS tiles) or 1324, 1328, 1344, 1348 (pipelined LDS tiles)");
     chebyshev_shape = value;
     return DMSC_OK;
   }
@@ -279,6 +280,11 @@ int dmsc_tune_set(const char* key, int value) {
   if (!std::strcmp(key, "overlap_stages")) {
     DMSC_ARG(value == 0 || value == 1, "overlap_stages must be 0 or 1");
     overlap_stages = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "integral_pass")) {
+    DMSC_ARG(value == 0 || value == 1, "integral_pass must be 0 (row + column passes) or 1 (banded single write)");
+    integral_pass = value;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -335,6 +341,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "overlap_stages")) {
     *value = overlap_stages;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "integral_pass")) {
+    *value = integral_pass;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
@@ -443,12 +453,10 @@ int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* r
   const int n = w * h;
   void* scr;
   const size_t st = (kmeans_state_bytes() + 255) & ~(size_t)255;
-  const size_t mixed_bytes = precision == DMSC_PREC_MIXED ? kmeans_mixed_work_bytes(n) : 0;
-  DMSC_TRY(ctx->scratch(2, st + sizeof(float) * (size_t)pitch_floats(n) + mixed_bytes, &scr));
+  DMSC_TRY(ctx->scratch(2, st + sizeof(float) * (size_t)pitch_floats(n), &scr));
   float* pdot = (float*)((char*)scr + st);
   if (precision == DMSC_PREC_MIXED) {
-    DMSC_TRY(launch_kmeans_mixed(ctx->stream, w, h, k, resp, labels, pdot, scr, max_iter,
-                                 (char*)scr + st + sizeof(float) * (size_t)pitch_floats(n)));
+    DMSC_TRY(launch_kmeans_mixed(ctx->stream, w, h, k, resp, labels, pdot, scr, max_iter));
   } else {
     DMSC_TRY(launch_kmeans(ctx->stream, w, h, k, nfilters, resp, labels, pdot, scr, max_iter));
   }
@@ -579,6 +587,14 @@ int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* M, int pitch, in
   DMSC_ARG(M && ms_per_step && pitch >= w * h, "bad arguments");
   DMSC_ARG(precision == DMSC_PREC_FP32 || precision == DMSC_PREC_MIXED, "bad precision");
   return time_lanczos_steps(ctx, w, h, M, pitch, precision, steps, ms_per_step);
+}
+
+int dmsc_chebyshev_time_products(dmsc_ctx* ctx, int w, int h, const float* M, int pitch, int products,
+                                 float* ms_per_product) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(M && ms_per_product && pitch >= w * h, "bad arguments");
+  return time_chebyshev_products(ctx, w, h, M, pitch, products, ms_per_product);
 }
 
 int dmsc_autotune_eigensolver(dmsc_ctx* ctx, int w, int h, float* M, int pitch, int precision, int steps,
@@ -768,11 +784,10 @@ int dmsc_gpb(dmsc_ctx* ctx, int w, int h, const uint32_t* rgb, const dmsc_gpb_pa
   {
     void* scr;
     const size_t st = (kmeans_state_bytes() + 255) & ~(size_t)255;
-    const size_t mixed_bytes = p.precision == DMSC_PREC_MIXED ? kmeans_mixed_work_bytes(n) : 0;
-    DMSC_TRY(ctx->scratch(2, st + sizeof(float) * P + mixed_bytes, &scr));
+    DMSC_TRY(ctx->scratch(2, st + sizeof(float) * P, &scr));
     float* pdot = (float*)((char*)scr + st);
-    if (p.precision == DMSC_PREC_MIXED && kmeans_mixed_work_bytes(n) > 0)
-      DMSC_TRY(launch_kmeans_mixed(s, w, h, ntex, resp, tex, pdot, scr, 15, (char*)scr + st + sizeof(float) * P));
+    if (p.precision == DMSC_PREC_MIXED)
+      DMSC_TRY(launch_kmeans_mixed(s, w, h, ntex, resp, tex, pdot, scr, 15));
     else
       DMSC_TRY(launch_kmeans(s, w, h, ntex, kNFilters, resp, tex, pdot, scr, 15));
     // the iteration count reaches the host without a stream synchronisation

@@ -57,8 +57,8 @@ constexpr double kFirstCut = 1e-2;  // damped interval of the first filter: [kFi
 constexpr double kUpper = 2.0;      // spectrum of L_sym lies in [0, 2]
 constexpr int kGramPx = 64;         // pixels staged per Gram round
 // pinned host block: Gram matrices at [8 KiB, 40 KiB), the rotation at [40 KiB, 48 KiB)
-constexpr size_t kPinGram = 8192, kPinRot = 40960;
-constexpr int kGrams = 4;  // Y'Y, Y'W, X0'Y, W'W
+constexpr size_t kPinGram = 8192, kPinRot = 40960, kPinTheta = 49152, kPinRes = 49408;
+constexpr int kGrams = 3;  // Y'Y, Y'W, X0'Y
 constexpr double kFloorMargin = 4.0;  // residual floor = kFloorMargin * 2 * 2^-24 (about 4.8e-7)
 
 // rows of the radius-5 disc at column offset dx: |dy| <= half_height(dx)
@@ -167,6 +167,212 @@ __global__ __launch_bounds__(64 * (kPlanes / VP)) void k_cheb_step(
   }
 }
 
+// The same filter step with the block's input staged in LDS.  A block owns
+// a 64 x (R * NW) pixel tile and one group of 8 vectors (two float4
+// planes); the four groups of a tile are consecutive blocks, so they run
+// on one XCD and read the tile's matrix rows through one L2.  The tile's
+// input values plus a 5-pixel frame, ((64 + 10) x (R * NW + 10) slots per
+// plane, slot (sy, sx) holding the input at linear index (y0 - 5 + sy) * w
+// + (x0 - 5 + sx) or 0 outside [0, n): the reference's linear indexing,
+// wrap across image rows included) are loaded once per block.  Each wave
+// owns R rows of the tile, each lane one column: per column offset dx it
+// loads the R * (2 half_height(dx) + 1) matrix values its rows need, then
+// walks the R + 2 half_height(dx) input rows from LDS, applying each to
+// every output row whose offset lies in the disc.  The sums run in the
+// order of k_cheb_step (dx outer, dy ascending inner), so the results are
+// bit-identical to it.  Lanes past the image's right or bottom edge
+// compute on clamped matrix addresses and store nothing.
+// One column offset dx (HM = half_height(dx)) of k_cheb_lds: the matrix
+// values of the lane's R rows, then the R + 2 HM input rows from LDS.
+template <int HM, int R, typename T>
+__device__ __forceinline__ void cheb_lds_dx(int dx, const float* __restrict__ A, size_t pitch, const int (&prow)[R],
+                                            const float4* s0, const float4* s1, int sw, T (&acc)[R][8]) {
+  float a[R][2 * HM + 1];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int dy = -HM; dy <= HM; dy++) a[r][dy + HM] = A[(size_t)disc_index(dy, dx) * pitch + (unsigned)prow[r]];
+#pragma unroll
+  for (int yy = -HM; yy < R + HM; yy++) {
+    const float4 v0 = s0[(yy + 5) * sw + dx];
+    const float4 v1 = s1[(yy + 5) * sw + dx];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int dy = yy - r;
+      if (dy < -HM || dy > HM) continue;
+      const T av = a[r][dy + HM];
+      acc[r][0] = mad(av, (T)v0.x, acc[r][0]);
+      acc[r][1] = mad(av, (T)v0.y, acc[r][1]);
+      acc[r][2] = mad(av, (T)v0.z, acc[r][2]);
+      acc[r][3] = mad(av, (T)v0.w, acc[r][3]);
+      acc[r][4] = mad(av, (T)v1.x, acc[r][4]);
+      acc[r][5] = mad(av, (T)v1.y, acc[r][5]);
+      acc[r][6] = mad(av, (T)v1.z, acc[r][6]);
+      acc[r][7] = mad(av, (T)v1.w, acc[r][7]);
+    }
+  }
+}
+
+// The same column-offset pass split in two, for software pipelining: the
+// matrix values of one offset (into a[r][dy + 5]) and the LDS walk that
+// applies them.
+// Buffer loads: the lane's row offset (prow * 4 bytes) is the one vector
+// offset of all its loads and the diagonal's offset (d * pitch * 4) a scalar,
+// so no per-load 64-bit address is formed; the descriptor's range is the
+// whole matrix (out-of-range reads would return 0).
+template <int HM, int R>
+__device__ __forceinline__ void cheb_load_a(int dx, __amdgpu_buffer_rsrc_t rs, unsigned pitch_bytes,
+                                            const unsigned (&voff)[R], float (&a)[R][11]) {
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int dy = -HM; dy <= HM; dy++)
+      a[r][dy + 5] = __builtin_bit_cast(
+          float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r], (unsigned)disc_index(dy, dx) * pitch_bytes, 0));
+}
+
+template <int HM, int R, typename T>
+__device__ __forceinline__ void cheb_apply(int dx, const float (&a)[R][11], const float4* s0, const float4* s1, int sw,
+                                           T (&acc)[R][8]) {
+#pragma unroll
+  for (int yy = -HM; yy < R + HM; yy++) {
+    const float4 v0 = s0[(yy + 5) * sw + dx];
+    const float4 v1 = s1[(yy + 5) * sw + dx];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int dy = yy - r;
+      if (dy < -HM || dy > HM) continue;
+      const T av = a[r][dy + 5];
+      acc[r][0] = mad(av, (T)v0.x, acc[r][0]);
+      acc[r][1] = mad(av, (T)v0.y, acc[r][1]);
+      acc[r][2] = mad(av, (T)v0.z, acc[r][2]);
+      acc[r][3] = mad(av, (T)v0.w, acc[r][3]);
+      acc[r][4] = mad(av, (T)v1.x, acc[r][4]);
+      acc[r][5] = mad(av, (T)v1.y, acc[r][5]);
+      acc[r][6] = mad(av, (T)v1.z, acc[r][6]);
+      acc[r][7] = mad(av, (T)v1.w, acc[r][7]);
+    }
+  }
+}
+
+// Empty asm statements that take the accumulators as operands: the sums
+// of a pass are complete at this point of the instruction stream, so the
+// compiler cannot defer the arithmetic behind later loads.
+template <int R, typename T>
+__device__ __forceinline__ void cheb_fence(T (&acc)[R][8]) {
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    asm volatile("" : "+v"(acc[r][0]), "+v"(acc[r][1]), "+v"(acc[r][2]), "+v"(acc[r][3]), "+v"(acc[r][4]),
+                 "+v"(acc[r][5]), "+v"(acc[r][6]), "+v"(acc[r][7])::"memory");
+}
+
+// One stage of the pipelined pass sequence: issue offset dx + 1's matrix
+// loads, then apply offset dx's values; the fences keep one stage's loads
+// and sums together (left alone, the compiler issues every load of the
+// eleven stages first and spills them).
+#define CHEB_STAGE(HMC, HMN, DX, CUR, NXT)                                   \
+  cheb_load_a<HMN, R>((DX) + 1, rs, pitch_bytes, voff, NXT);                 \
+  cheb_apply<HMC, R, T>((DX), CUR, s0, s1, SW, acc);                         \
+  cheb_fence<R, T>(acc);
+
+template <int R, int NW, bool F64, bool PF = false>
+__global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, const float* __restrict__ A,
+                                                      size_t pitch, const float4* __restrict__ x,
+                                                      const float4* __restrict__ z, float4* __restrict__ y,
+                                                      size_t vs, float alpha, float c, float beta) {
+  constexpr int TH = R * NW, SW = 64 + 10, SH = TH + 10, NT = 64 * NW;
+  constexpr int kSlots = 2 * SH * SW, kIt = (kSlots + NT - 1) / NT;
+  __shared__ float4 sx[2][SH][SW];
+  const int b = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int grp = b & 3, t = b >> 2;
+  const int x0 = (t % ntx) * 64, y0 = (t / ntx) * TH;
+  const int n = w * h;
+  const int pl0 = grp * 2;
+  {
+    float4 v[kIt];
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+      const int i = (int)threadIdx.x + j * NT;
+      const int pl = i / (SH * SW), rem = i - pl * (SH * SW);
+      const int sy = rem / SW, sxx = rem - sy * SW;
+      const int q = (y0 - 5 + sy) * w + (x0 - 5 + sxx);
+      const bool in = i < kSlots && (unsigned)q < (unsigned)n;
+      const float4 t4 = x[(size_t)(pl0 + (i < kSlots ? pl : 0)) * vs + (in ? (unsigned)q : 0u)];
+      v[j] = in ? t4 : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+      const int i = (int)threadIdx.x + j * NT;
+      if (i < kSlots) (&sx[0][0][0])[i] = v[j];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int r0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * R;  // wave-uniform
+  const int px = x0 + lane;
+  int prow[R];  // matrix row of each output (clamped to 0 past the image: computed, never stored)
+#pragma unroll
+  for (int r = 0; r < R; r++) prow[r] = (px < w && y0 + r0 + r < h) ? (y0 + r0 + r) * w + px : 0;
+  using T = typename std::conditional<F64, double, float>::type;
+  T acc[R][8];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[r][k] = T(0);
+  // input slot of (tile row r0, this lane's column) at dx = 0, dy = -5
+  const float4* s0 = &sx[0][r0][lane + 5];
+  const float4* s1 = &sx[1][r0][lane + 5];
+  if constexpr (PF) {
+    // column offsets -5..5 have half heights 0 3 4 4 4 5 4 4 4 3 0
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)A, (short)0, (int)(kStencilArea * pitch * sizeof(float)), 0x00020000);
+    const unsigned pitch_bytes = (unsigned)(pitch * sizeof(float));
+    unsigned voff[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) voff[r] = (unsigned)prow[r] * 4u;
+    float a0[R][11], a1[R][11];
+    cheb_load_a<0, R>(-5, rs, pitch_bytes, voff, a0);
+    CHEB_STAGE(0, 3, -5, a0, a1)
+    CHEB_STAGE(3, 4, -4, a1, a0)
+    CHEB_STAGE(4, 4, -3, a0, a1)
+    CHEB_STAGE(4, 4, -2, a1, a0)
+    CHEB_STAGE(4, 5, -1, a0, a1)
+    CHEB_STAGE(5, 4, 0, a1, a0)
+    CHEB_STAGE(4, 4, 1, a0, a1)
+    CHEB_STAGE(4, 4, 2, a1, a0)
+    CHEB_STAGE(4, 3, 3, a0, a1)
+    CHEB_STAGE(3, 0, 4, a1, a0)
+    cheb_apply<0, R, T>(5, a0, s0, s1, SW, acc);
+  } else {
+#pragma unroll 1
+    for (int dx = -5; dx <= 5; dx++) {
+      switch (half_height(dx)) {
+        case 0: cheb_lds_dx<0, R, T>(dx, A, pitch, prow, s0, s1, SW, acc); break;
+        case 3: cheb_lds_dx<3, R, T>(dx, A, pitch, prow, s0, s1, SW, acc); break;
+        case 4: cheb_lds_dx<4, R, T>(dx, A, pitch, prow, s0, s1, SW, acc); break;
+        default: cheb_lds_dx<5, R, T>(dx, A, pitch, prow, s0, s1, SW, acc); break;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (px >= w || y0 + r0 + r >= h) continue;
+    const int p = (y0 + r0 + r) * w + px;
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const size_t i = (size_t)(pl0 + v) * vs + p;
+      const float4 xo = sx[v][r0 + r + 5][lane + 5];
+      const float4 zo = z ? z[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 o;
+      o.x = (float)(alpha * mad((T)-c, (T)xo.x, acc[r][4 * v + 0]) - beta * zo.x);
+      o.y = (float)(alpha * mad((T)-c, (T)xo.y, acc[r][4 * v + 1]) - beta * zo.y);
+      o.z = (float)(alpha * mad((T)-c, (T)xo.z, acc[r][4 * v + 2]) - beta * zo.z);
+      o.w = (float)(alpha * mad((T)-c, (T)xo.w, acc[r][4 * v + 3]) - beta * zo.w);
+      y[i] = o;
+    }
+  }
+}
+
 // Starting block: uniform values in [-1, 1) from an integer hash of
 // (plane, pixel, component); rows n..vs-1 are 0.
 __global__ void k_cheb_init(int n, size_t vs, float4* __restrict__ x) {
@@ -187,7 +393,7 @@ __global__ void k_cheb_init(int n, size_t vs, float4* __restrict__ x) {
 }
 
 // Partial Gram matrices over one chunk of pixels, FP64: blockIdx.y = 0
-// G = Y'Y, 1 H = Y'W, 2 C = X0'Y, 3 K = W'W (entry [i][j] = U_i . V_j).  Thread t
+// G = Y'Y, 1 H = Y'W, 2 C = X0'Y (entry [i][j] = U_i . V_j).  Thread t
 // owns row i = t / 8 and columns 4 (t % 8) .. +3.
 __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, const float4* __restrict__ Y,
                                                    const float4* __restrict__ Wb, const float4* __restrict__ X0,
@@ -195,8 +401,8 @@ __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, 
   __shared__ float su[kGramPx][kB + 1];
   __shared__ float4 sv[kGramPx][kPlanes];
   const int which = blockIdx.y;
-  const float4* U = which == 2 ? X0 : which == 3 ? Wb : Y;
-  const float4* V = which == 1 || which == 3 ? Wb : Y;
+  const float4* U = which == 2 ? X0 : Y;
+  const float4* V = which == 1 ? Wb : Y;
   const int i = threadIdx.x >> 3, j4 = threadIdx.x & 7;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int p0 = blockIdx.x * chunk, p1 = min(n, p0 + chunk);
@@ -228,16 +434,112 @@ __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, 
   for (int k = 0; k < 4; k++) o[k] = acc[k];
 }
 
-// Sums the partials of each Gram entry in block order (deterministic).
-__global__ void k_cheb_gram_sum(int nblk, const double* __restrict__ part, double* __restrict__ gram) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= kGrams * kB * kB) return;
-  const int which = e / (kB * kB), idx = e % (kB * kB);
-  const double* p = part + (size_t)which * nblk * (kB * kB) + idx;
-  double s = 0.0;
-#pragma unroll 16
-  for (int b = 0; b < nblk; b++) s += p[(size_t)b * (kB * kB)];
-  gram[e] = s;
+// Residuals of the wanted Ritz pairs, formed directly: r_j = W s_j -
+// theta_j Y s_j per pixel in FP64 (S from the rotation buffer, row i =
+// input vector i), |r_j|^2 summed per block of 256 pixels into part[blk][j].
+// The Gram form s'K s - theta^2 cancels: W is stored in FP32, so K carries
+// a relative rounding of about 2^-23 and the difference keeps an error of
+// about theta * 2^-11.5, which at 61x37 (theta near 1e-2) exceeds the
+// residual itself; the direct form errs by about 2^-24 theta |y|.
+constexpr int kResidMax = kB / 2;
+__global__ __launch_bounds__(256) void k_cheb_resid(int n, size_t vs, const float4* __restrict__ Y,
+                                                    const float4* __restrict__ Wb, const double* __restrict__ S,
+                                                    const double* __restrict__ theta, int nvec,
+                                                    double* __restrict__ part) {
+  __shared__ double ss[kB * kResidMax];
+  __shared__ double th[kResidMax];
+  __shared__ double red[kResidMax][4];
+  for (int k = threadIdx.x; k < kB * kResidMax; k += 256) {
+    const int i = k / kResidMax, j = k - i * kResidMax;
+    ss[k] = j < nvec ? S[i * kB + j] : 0.0;
+  }
+  if (threadIdx.x < kResidMax) th[threadIdx.x] = threadIdx.x < nvec ? theta[threadIdx.x] : 0.0;
+  __syncthreads();
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  double r2[kResidMax];
+#pragma unroll
+  for (int j = 0; j < kResidMax; j++) r2[j] = 0.0;
+  if (p < n) {
+    double ys[kResidMax], ws[kResidMax];
+#pragma unroll
+    for (int j = 0; j < kResidMax; j++) ys[j] = ws[j] = 0.0;
+#pragma unroll 2
+    for (int pl = 0; pl < kPlanes; pl++) {
+      const float4 yv = Y[(size_t)pl * vs + p], wv = Wb[(size_t)pl * vs + p];
+      const float yk[4] = {yv.x, yv.y, yv.z, yv.w}, wk[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double* row = ss + (pl * 4 + k) * kResidMax;
+#pragma unroll
+        for (int j = 0; j < kResidMax; j++) {
+          ys[j] = fma((double)yk[k], row[j], ys[j]);
+          ws[j] = fma((double)wk[k], row[j], ws[j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kResidMax; j++) {
+      const double r = fma(-th[j], ys[j], ws[j]);
+      r2[j] = r * r;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kResidMax; j++) {
+    double v = r2[j];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kResidMax)
+    part[(size_t)blockIdx.x * kResidMax + threadIdx.x] =
+        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
+// Residual norms from k_cheb_resid's partials: block j sums column j, each
+// thread a strided subset, then the threads' sums in a fixed tree order.
+__global__ __launch_bounds__(256) void k_cheb_resid_sum(int nblk, const double* __restrict__ part,
+                                                        double* __restrict__ out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) v += part[(size_t)b * kResidMax + j];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = sqrt(((red[0] + red[1]) + red[2]) + red[3]);
+}
+
+// Sums the partials of each Gram entry: a block covers 64 entries with 16
+// threads each; thread j sums the partials of blocks j, j + 16, ... in
+// order, then the 16 sums are added in a fixed tree (deterministic).
+constexpr int kGramSumSplit = 16;
+__global__ __launch_bounds__(1024) void k_cheb_gram_sum(int nblk, const double* __restrict__ part,
+                                                        double* __restrict__ gram) {
+  __shared__ double red[kGramSumSplit][64];
+  const int el = threadIdx.x & 63, j = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + el;
+  double sum = 0.0;
+  if (e < kGrams * kB * kB) {
+    const int which = e / (kB * kB), idx = e % (kB * kB);
+    const double* p = part + (size_t)which * nblk * (kB * kB) + idx;
+#pragma unroll 4
+    for (int b = j; b < nblk; b += kGramSumSplit) sum += p[(size_t)b * (kB * kB)];
+  }
+  red[j][el] = sum;
+  __syncthreads();
+  if (j == 0 && e < kGrams * kB * kB) {
+    double t[kGramSumSplit];
+#pragma unroll
+    for (int k = 0; k < kGramSumSplit; k++) t[k] = red[k][el];
+#pragma unroll
+    for (int w = kGramSumSplit / 2; w >= 1; w >>= 1)
+#pragma unroll
+      for (int k = 0; k < w; k++) t[k] += t[k + w];
+    gram[e] = t[0];
+  }
 }
 
 // Rotation by S (kB x kB, FP64, row i = input vector i), FP64 sums: nout < 0
@@ -384,16 +686,56 @@ int launch_step(hipStream_t s, int w, int h, const float* A, size_t pitch, const
   return DMSC_OK;
 }
 
+template <int R, int NW, bool F64 = false, bool PF = false>
+int launch_lds(hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, const float4* z,
+               float4* y, size_t vs, double alpha, double c, double beta) {
+  // the pipelined form addresses the matrix through one buffer descriptor
+  // whose range (bytes) is a 31-bit count
+  if (PF && (size_t)kStencilArea * pitch * sizeof(float) >= ((size_t)1 << 31))
+    return launch_lds<R, NW, F64, false>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  const int ntx = div_up(w, 64), nty = div_up(h, R * NW);
+  hipLaunchKernelGGL((k_cheb_lds<R, NW, F64, PF>), dim3(ntx * nty * 4), dim3(64 * NW), 0, s, w, h, ntx, A, pitch, x,
+                     z, y, vs, (float)alpha, (float)c, (float)beta);
+  return DMSC_OK;
+}
+
 int step(int shape, hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, const float4* z, float4* y, size_t vs, double alpha, double c, double beta) {
+  if (shape == 324) return launch_lds<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 344) return launch_lds<4, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 342) return launch_lds<4, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 328) return launch_lds<2, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 348) return launch_lds<4, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1324) return launch_lds<2, 4, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1328) return launch_lds<2, 8, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1344) return launch_lds<4, 4, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1348) return launch_lds<4, 8, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 42) return launch_step<4, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 24) return launch_step<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 28) return launch_step<2, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   return launch_step<2, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
 }
 
+// W = A Y with FP64 sums on the filter's tile (bit-identical across tiles:
+// every tile sums in the same order).
+int product_f64(int shape, hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, float4* y,
+                size_t vs) {
+  if (shape == 324) return launch_lds<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 344) return launch_lds<4, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 342) return launch_lds<4, 2, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 348 || shape == 1348) return launch_lds<4, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1324) return launch_lds<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1344) return launch_lds<4, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  return launch_step<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+}
+
 }  // namespace
 
-bool chebyshev_shape_valid(int v) { return v == 22 || v == 42 || v == 24 || v == 28; }
+bool chebyshev_shape_valid(int v) {
+  return v == 22 || v == 42 || v == 24 || v == 28 || v == 324 || v == 344 || v == 342 || v == 328 || v == 348 ||
+         v == 1324 || v == 1328 || v == 1344 || v == 1348;
+}
 
 // The nvec smallest eigenpairs of L_sym (A, already converted): unit
 // eigenvectors to d_out[j * n + p], eigenvalues to vals; *products = the
@@ -407,9 +749,13 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   const int chunk = std::max(kGramPx, div_up(div_up(n, 512), kGramPx) * kGramPx);
   const int nblk = div_up(n, chunk);
   const size_t blk = (size_t)kPlanes * vs * sizeof(float4);
+  const int nrb = div_up(n, 256);  // k_cheb_resid blocks
   const size_t o_part = 4 * blk, o_gram = o_part + (size_t)kGrams * nblk * kB * kB * sizeof(double),
                o_rot = o_gram + (size_t)kGrams * kB * kB * sizeof(double),
-               bytes = o_rot + (size_t)kB * kB * sizeof(double);
+               o_theta = o_rot + (size_t)kB * kB * sizeof(double),
+               o_res = o_theta + (size_t)kResidMax * sizeof(double),
+               o_rpart = o_res + (size_t)kResidMax * sizeof(double),
+               bytes = o_rpart + (size_t)nrb * kResidMax * sizeof(double);
   void* base;
   DMSC_TRY(ctx->scratch(3, bytes, &base));
   char* Bp = (char*)base;
@@ -418,10 +764,15 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   double* part = (double*)(Bp + o_part);
   double* gram = (double*)(Bp + o_gram);
   double* rot = (double*)(Bp + o_rot);
+  double* d_theta = (double*)(Bp + o_theta);
+  double* d_res = (double*)(Bp + o_res);
+  double* rpart = (double*)(Bp + o_rpart);
   int* pin;
   DMSC_TRY(ctx->pinned(&pin));
   double* h_gram = (double*)((char*)pin + kPinGram);
   double* h_rot = (double*)((char*)pin + kPinRot);
+  double* h_theta = (double*)((char*)pin + kPinTheta);
+  double* h_res = (double*)((char*)pin + kPinRes);
 
   const StencilOffsets so = make_stencil_offsets(w);
   for (int d = 0; d < kStencilArea; d++)  // the filter kernel's closed-form diagonal order
@@ -456,29 +807,25 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     }
     float4* Y = ring[cur];
     float4* Wb = ring[(cur + 1) % 3];
-    DMSC_TRY((launch_step<2, 4, true>(s, w, h, A, pitchz, Y, nullptr, Wb, vs, 1.0, 0.0, 0.0)));
+    DMSC_TRY(product_f64(shape, s, w, h, A, pitchz, Y, Wb, vs));
     hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, kGrams), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
-    hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(kGrams * kB * kB, 256)), dim3(256), 0, s, nblk, part, gram);
+    hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(kGrams * kB * kB, 64)), dim3(1024), 0, s, nblk, part, gram);
     DMSC_HIP(hipMemcpyAsync(h_gram, gram, kGrams * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));
     DMSC_HIP(hipStreamSynchronize(s));
     if (!rayleigh_ritz(h_gram, h_gram + kB * kB, theta, S))
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: filtered block lost rank at outer iteration %d", outer);
-    // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j:
-    // s'K s - theta^2 (s'G s = 1, s'H s = theta), exact to FP64 rounding
-    // of terms of size theta^2
+    // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j,
+    // formed per pixel on the device (k_cheb_resid)
+    std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
+    for (int j = 0; j < nvec; j++) h_theta[j] = theta[j];
+    DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
+    DMSC_HIP(hipMemcpyAsync(d_theta, h_theta, sizeof(double) * nvec, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cheb_resid, dim3(nrb), dim3(256), 0, s, n, vs, Y, Wb, rot, d_theta, nvec, rpart);
+    hipLaunchKernelGGL(k_cheb_resid_sum, dim3(nvec), dim3(256), 0, s, nrb, rpart, d_res);
+    DMSC_HIP(hipMemcpyAsync(h_res, d_res, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
+    DMSC_HIP(hipStreamSynchronize(s));
     double resid = 0.0;
-    {
-      const double* K = h_gram + 3 * kB * kB;
-      for (int j = 0; j < nvec; j++) {
-        double q = 0.0;
-        for (int a = 0; a < kB; a++) {
-          double r = 0.0;
-          for (int b = 0; b < kB; b++) r += K[a * kB + b] * S[b * kB + j];
-          q += S[a * kB + j] * r;
-        }
-        resid = std::max(resid, std::sqrt(std::max(0.0, q - theta[j] * theta[j])));
-      }
-    }
+    for (int j = 0; j < nvec; j++) resid = std::max(resid, h_res[j]);
     double move = 0.0;
     if (outer > 0) {
       const double* C = h_gram + 2 * kB * kB;
@@ -491,8 +838,6 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     if (lanczos_trace)
       std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e residual %.3e\n",
                    outer, deg, cut, theta[0], theta[nvec - 1], move, resid);
-    std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
-    DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
     // done when the wanted vectors stopped moving, or (a degenerate
     // eigenvalue leaves its vectors free to turn) when every residual meets
     // the mixed Lanczos bound, tol * 10^-lanczos_mixed_stop; or when the
@@ -534,6 +879,51 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     deg = std::min(kMaxDeg, std::max(kMinDeg, deg));
   }
   return fail(DMSC_ERR_NOCONV, "block eigensolver did not converge in %d outer iterations", kMaxOuter);
+}
+
+// Times `products` filter steps (after 5 untimed ones) of the current
+// filter tile on the converted matrix A with HIP events on the context's
+// stream: the kernel the block eigensolver spends its time in, on a block
+// of kB vectors.  Returns milliseconds per step.
+int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product) {
+  if (products < 1 || products > 100000) return fail(DMSC_ERR_ARG, "products out of range");
+  hipStream_t s = ctx->stream;
+  const int n = w * h;
+  const size_t vs = (size_t)pitch_floats(n);
+  const size_t blk = (size_t)kPlanes * vs * sizeof(float4);
+  void* base;
+  DMSC_TRY(ctx->scratch(3, 3 * blk, &base));
+  float4* ring[3] = {(float4*)base, (float4*)((char*)base + blk), (float4*)((char*)base + 2 * blk)};
+  for (int i = 0; i < 3; i++)
+    hipLaunchKernelGGL(k_cheb_init, dim3(div_up((long)kPlanes * vs, 256)), dim3(256), 0, s, n, vs, ring[i]);
+  // the solver's first filter (damped interval [kFirstCut, 2]), so the
+  // block stays O(1) over any number of steps
+  const double e = (kUpper - kFirstCut) / 2, c = (kUpper + kFirstCut) / 2;
+  double sigma = e / (0.0 - c);
+  const double tau = 2 / sigma;
+  hipEvent_t ev[2];
+  DMSC_HIP(hipEventCreate(&ev[0]));
+  DMSC_HIP(hipEventCreate(&ev[1]));
+  int rc = step(chebyshev_shape, s, w, h, A, (size_t)pitch, ring[0], nullptr, ring[1], vs, sigma / e, c, 0.0);
+  for (int i = 1; i < products + 6 && rc == DMSC_OK; i++) {
+    if (i == 6) rc = hipEventRecord(ev[0], s) == hipSuccess ? DMSC_OK : fail(DMSC_ERR_HIP, "event record failed");
+    const double sn = 1 / (tau - sigma);
+    if (rc == DMSC_OK)
+      rc = step(chebyshev_shape, s, w, h, A, (size_t)pitch, ring[i % 3], ring[(i + 2) % 3], ring[(i + 1) % 3], vs,
+                2 * sn / e, c, sigma * sn);
+    sigma = sn;
+  }
+  float ms = 0.f;
+  if (rc == DMSC_OK && hipEventRecord(ev[1], s) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess &&
+      hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+    *ms_per_product = ms / (float)products;
+  } else if (rc == DMSC_OK) {
+    rc = fail(DMSC_ERR_HIP, "filter timing failed");
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  DMSC_HIP(hipGetLastError());
+  return rc;
 }
 
 }  // namespace dmsc

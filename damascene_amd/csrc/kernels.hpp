@@ -22,8 +22,7 @@ size_t kmeans_state_bytes();
 int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp, int* labels, float* pdot,
                   void* state, int max_iter);
 int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, int* labels, float* pdot,
-                        void* state, int max_iter, void* work);
-size_t kmeans_mixed_work_bytes(int n);
+                        void* state, int max_iter);
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations, int* rechecks = nullptr);
 const int* kmeans_iterations_ptr(const void* state);  // device address of the iteration count
 
@@ -40,6 +39,7 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
                            const float* B, const int* tex, int ntex, float* bg, float* cga, float* cgb, float* tg,
                            int pitch, float* grads_out, hipEvent_t hist_done = nullptr);
 extern int overlap_stages;  // dmsc_gpb: L/a/b local cues beside the textons (tuning key "overlap_stages")
+extern int integral_pass;   // local cues: 1 single-write banded integral images, 0 row + column passes
 
 // ------------------------------------------------------------- combine / nonmax / gPb
 int launch_combine(hipStream_t s, int n, int pitch, const float* bg, const float* cga, const float* cgb,
@@ -123,6 +123,7 @@ extern int batch_lanes;
 extern int eigensolver_method;
 extern int chebyshev_shape;
 bool chebyshev_shape_valid(int v);
+int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product);
 int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec, float tol, float* d_out,
                     double* vals, int* products);
 

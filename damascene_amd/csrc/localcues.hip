@@ -39,6 +39,7 @@ struct FrameGeom {
   int nw[4], nh[4];
   long long lab_off[kFrames];  // byte offset of frame f = ch*4 + o (label buffer)
   long long int_off[kFrames];  // uint16 element offset of frame f (integral buffer)
+  long long band_off[kFrames];  // uint16 element offset of frame f's band sums (k_integral_band)
   int tab_off[4];              // offset of orientation o's tables: bX[bw] bY[bw] gX[bh] gY[bh]
   int rect[4][4][8];           // [orientation][radius index][corner A..H]
   int tbp;                     // bins per integral pixel of the texture frames (32 or 64)
@@ -139,6 +140,133 @@ __global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, int f0, uint
   }
 }
 
+// Single-write integral images.  A frame is cut into bands of kBand rows;
+// I(x, y) for y in band b is the carry C_b(x) = sum over earlier bands of
+// their row-prefixed column counts P_b'(x) plus the band's own running sum of
+// row prefixes.  Pass 1 (FINAL = false) leaves P_b per band, k_band_carry
+// turns them into the exclusive prefix over bands in place, pass 2 (FINAL)
+// starts each band's running sums at its carry and writes every integral
+// value once.  One block of 1024 threads per (band, frame); thread t owns
+// columns x = t + 1024 c (c < NCH) and keeps their running sums for all
+// bins in registers (two u16 bins per word, modulo 2^16 like the stored
+// integrals).  The row prefix of bin k at column x is the count of k among
+// the lanes below x in its wave (ballot + mbcnt) plus the exclusive total
+// of the earlier waves (a 16 NCH-entry scan per bin through LDS).  Each
+// thread stores its pixel's whole record (BP u16, 16 bytes at a time), so
+// a wave writes consecutive records.
+constexpr int kBand = 16;
+constexpr int kIntThreads = 1024, kIntWaves = kIntThreads / 64;
+
+template <int BP, int NCH, bool FINAL>
+__global__ __launch_bounds__(kIntThreads) void k_integral_band(FrameGeom g, int f0,
+                                                               const uint8_t* __restrict__ frames,
+                                                               uint16_t* __restrict__ bands,
+                                                               uint16_t* __restrict__ I) {
+  constexpr int NQ = BP / 2;  // words per record (two bins each)
+  constexpr int NS = NCH * kIntWaves;
+  __shared__ uint32_t tot[2][NS][BP];  // per-wave bin totals, by row parity
+  __shared__ uint32_t pre[2][NS][BP];  // their exclusive prefix over the row
+  const int f = f0 + blockIdx.y, o = f & 3, b = blockIdx.x;
+  const int nw = g.nw[o], nh = g.nh[o];
+  const int y0 = b * kBand;
+  if (y0 >= nh) return;  // the whole block: past this frame's last band
+  const int y1 = min(nh, y0 + kBand);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint16_t* bsum = bands + g.band_off[f] + (size_t)b * nw * BP;
+  uint32_t S[NCH][NQ];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int x = c * kIntThreads + (int)threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NQ; q += 4) {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (FINAL && x < nw) v = *reinterpret_cast<const uint4*>(bsum + (size_t)x * BP + 2 * q);
+      S[c][q] = v.x;
+      S[c][q + 1] = v.y;
+      S[c][q + 2] = v.z;
+      S[c][q + 3] = v.w;
+    }
+  }
+  for (int y = y0; y < y1; y++) {
+    const int par = (y - y0) & 1;
+    const uint8_t* src = frames + g.lab_off[f] + (size_t)y * nw;
+    uint32_t Rl[NCH][NQ];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int x = c * kIntThreads + (int)threadIdx.x;
+      const uint32_t lab = x < nw ? (uint32_t)src[x] : 255u;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const unsigned long long m0 = __ballot(lab == (uint32_t)(2 * q));
+        const unsigned long long m1 = __ballot(lab == (uint32_t)(2 * q + 1));
+        const uint32_t r0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u)) +
+                            (lab == (uint32_t)(2 * q));
+        const uint32_t r1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u)) +
+                            (lab == (uint32_t)(2 * q + 1));
+        Rl[c][q] = r0 | (r1 << 16);
+        if (lane == 0) {
+          tot[par][c * kIntWaves + wv][2 * q] = (uint32_t)__popcll(m0);
+          tot[par][c * kIntWaves + wv][2 * q + 1] = (uint32_t)__popcll(m1);
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < BP) {
+      uint32_t run = 0;
+      for (int k = 0; k < NS; k++) {
+        pre[par][k][threadIdx.x] = run;
+        run += tot[par][k][threadIdx.x];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int x = c * kIntThreads + (int)threadIdx.x;
+      const uint32_t* pw = pre[par][c * kIntWaves + wv];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        // row prefix of bins 2q, 2q+1 (each below 2^16: no carry between halves)
+        const uint32_t r = Rl[c][q] + (pw[2 * q] | (pw[2 * q + 1] << 16));
+        const uint32_t lo = (S[c][q] + r) & 0xffffu;
+        const uint32_t hi = ((S[c][q] >> 16) + (r >> 16)) << 16;
+        S[c][q] = lo | hi;
+      }
+      if (FINAL && x < nw) {
+        uint4* dst = reinterpret_cast<uint4*>(I + g.int_off[f] + ((size_t)y * nw + x) * BP);
+#pragma unroll
+        for (int q = 0; q < NQ; q += 4) dst[q / 4] = make_uint4(S[c][q], S[c][q + 1], S[c][q + 2], S[c][q + 3]);
+      }
+    }
+  }
+  if (!FINAL) {
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int x = c * kIntThreads + (int)threadIdx.x;
+      if (x < nw) {
+        uint4* dst = reinterpret_cast<uint4*>(bsum + (size_t)x * BP);
+#pragma unroll
+        for (int q = 0; q < NQ; q += 4) dst[q / 4] = make_uint4(S[c][q], S[c][q + 1], S[c][q + 2], S[c][q + 3]);
+      }
+    }
+  }
+}
+
+// Band sums -> exclusive prefix over the bands of each frame (in place).
+__global__ __launch_bounds__(256) void k_band_carry(FrameGeom g, int f0, uint16_t* __restrict__ bands) {
+  const int f = f0 + blockIdx.y, o = f & 3;
+  const int nw = g.nw[o], nb = (g.nh[o] + kBand - 1) / kBand, bp = bin_pitch(g, f);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nw * bp) return;
+  uint16_t* p = bands + g.band_off[f] + e;
+  const size_t stride = (size_t)nw * bp;
+  uint16_t acc = 0;
+  for (int b = 0; b < nb; b++) {
+    const uint16_t t = p[(size_t)b * stride];
+    p[(size_t)b * stride] = acc;
+    acc = (uint16_t)(acc + t);
+  }
+}
+
 struct BlurK {
   float k[16];
 };
@@ -231,14 +359,20 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
 //   sense A (top/bottom, slot o+4): a = A-C+E-D, b = H-F+D-E
 //   sense B (left/right, slot o):   a = A-F+G-B, b = H-C+B-G
 // (rotate.cu:400-445; gradient.cu:240-241).
+// Blocks: blockIdx.x runs over (pixel block, scale) with the scale fastest,
+// numbered XCD-aware, so each XCD handles a contiguous band of image rows at
+// all three scales and its L2 keeps the band's rotated integral rows (plus
+// the corner reach above and below) for every gather; blockIdx.y is the
+// (channel, orientation).
 template <int NB, int KR>
 __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad0, const int* __restrict__ tabs,
                                                    const uint16_t* __restrict__ I, BlurK bk,
                                                    float* __restrict__ grads, size_t plane) {
   const int n = g.w * g.h;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int scale = blockIdx.y;
-  const int ch = ch0 + (blockIdx.z >> 2), o = blockIdx.z & 3, f = ch * 4 + o;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+  const int p = (lb / 3) * blockDim.x + threadIdx.x;
+  const int scale = lb % 3;
+  const int ch = ch0 + (blockIdx.y >> 2), o = blockIdx.y & 3, f = ch * 4 + o;
   if (p >= n) return;
   const int y = p / g.w, x = p - y * g.w;
   const int* t = tabs + g.tab_off[o];
@@ -282,12 +416,13 @@ __global__ __launch_bounds__(256) void k_gradients64(FrameGeom g, int rad0, cons
                                                      const uint16_t* __restrict__ I, float* __restrict__ grads,
                                                      size_t plane) {
   const int n = g.w * g.h;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = dev::xcd_block(blockIdx.x, gridDim.x);  // (pixel block, scale), as k_gradients
+  const int gt = (lb / 3) * blockDim.x + threadIdx.x;
   const int half = gt & 1;
   const bool live = (gt >> 1) < n;
   const int p = live ? (gt >> 1) : n - 1;  // both lanes of a pair agree; all lanes reach the shuffles
-  const int scale = blockIdx.y;
-  const int ch = 3, o = blockIdx.z, f = ch * 4 + o;
+  const int scale = lb % 3;
+  const int ch = 3, o = blockIdx.y, f = ch * 4 + o;
   const int y = p / g.w, x = p - y * g.w;
   const int* t = tabs + g.tab_off[o];
   const int ix = x + kBorder, iy = y + kBorder;
@@ -398,7 +533,8 @@ size_t parabola_lds() {
 struct CuePlan {
   int w = -1, h = -1;
   FrameGeom g;
-  size_t lab_bytes = 0, int_elems = 0;
+  size_t lab_bytes = 0, int_elems = 0, band_elems = 0;
+  bool band_ok = false;  // frames narrow enough for the single-write integral pass
   int tap_stride = 0;
   int* d_tabs = nullptr;
   float* d_coef[4] = {nullptr, nullptr, nullptr, nullptr};  // [8][2r+1][2r+1] per radius
@@ -461,16 +597,26 @@ int build_plan(hipStream_t s, int w, int h, int tbp, CuePlan& P) {
           return fail(DMSC_ERR_INTERNAL, "half-disc corners leave the frame (o=%d)", o);
       }
   }
-  size_t lo = 0, io = 0;
+  size_t lo = 0, io = 0, bo = 0;
   for (int f = 0; f < kFrames; f++) {
     const int o = f & 3;
     g.lab_off[f] = (long long)lo;
     g.int_off[f] = (long long)io;
+    g.band_off[f] = (long long)bo;
     lo += ((size_t)g.nw[o] * g.nh[o] + 255) & ~(size_t)255;
     io += (size_t)g.nw[o] * g.nh[o] * (f >= 12 ? tbp : 32);
+    bo += (size_t)div_up(g.nh[o], kBand) * g.nw[o] * (f >= 12 ? tbp : 32);
   }
   P.lab_bytes = lo;
   P.int_elems = io;
+  P.band_elems = bo;
+  // the single-write integral pass covers frames up to 2 * kIntThreads
+  // columns; wider ones use the row + column passes
+  {
+    int maxw = 0;
+    for (int o = 0; o < 4; o++) maxw = std::max(maxw, g.nw[o]);
+    P.band_ok = maxw <= 2 * kIntThreads;
+  }
   DMSC_HIP(hipMalloc(&P.d_tabs, sizeof(int) * tabs.size()));
   DMSC_HIP(hipMemcpy(P.d_tabs, tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice));
   for (int ri = 0; ri < 4; ri++) {
@@ -503,6 +649,26 @@ int build_plan(hipStream_t s, int w, int h, int tbp, CuePlan& P) {
   P.h = h;
   (void)s;
   return DMSC_OK;
+}
+
+// Integral images of frames f0 .. f0 + nf - 1 (BP bins) in three launches:
+// band sums, carries, final pass (frames up to 2 * kIntThreads wide).
+template <int BP, int NCH>
+int integral_bands_n(hipStream_t s, const FrameGeom& g, int f0, int nf, int maxw, int maxh, const uint8_t* frames,
+                     uint16_t* bands, uint16_t* I) {
+  const dim3 grid(div_up(maxh, kBand), nf);
+  hipLaunchKernelGGL((k_integral_band<BP, NCH, false>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands, I);
+  hipLaunchKernelGGL(k_band_carry, dim3(div_up(maxw * BP, 256), nf), dim3(256), 0, s, g, f0, bands);
+  hipLaunchKernelGGL((k_integral_band<BP, NCH, true>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands, I);
+  DMSC_HIP(hipGetLastError());
+  return DMSC_OK;
+}
+
+template <int BP>
+int integral_bands(hipStream_t s, const FrameGeom& g, int f0, int nf, int maxw, int maxh, const uint8_t* frames,
+                   uint16_t* bands, uint16_t* I) {
+  if (maxw <= kIntThreads) return integral_bands_n<BP, 1>(s, g, f0, nf, maxw, maxh, frames, bands, I);
+  return integral_bands_n<BP, 2>(s, g, f0, nf, maxw, maxh, frames, bands, I);
 }
 
 struct PlanCache {
@@ -571,7 +737,8 @@ int local_cues_prepare(Ctx* ctx, int w, int h, int ntex) {
   DMSC_TRY(get_plan(ctx, w, h, ntex, &P));
   void* scratch;
   const size_t gbytes = (size_t)4 * 24 * w * h * sizeof(float);
-  return ctx->scratch(1, P->lab_bytes + P->int_elems * sizeof(uint16_t) + gbytes + 1024, &scratch);
+  return ctx->scratch(1, P->lab_bytes + P->int_elems * sizeof(uint16_t) + gbytes + P->band_elems * sizeof(uint16_t) + 1024,
+                      &scratch);
 }
 
 int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, const float* L, const float* A,
@@ -587,10 +754,11 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
   void* scratch;
   const size_t gbytes = (size_t)4 * 24 * n * sizeof(float);
   const size_t ibytes = P->int_elems * sizeof(uint16_t);
-  DMSC_TRY(ctx->scratch(1, P->lab_bytes + ibytes + gbytes + 1024, &scratch));
+  DMSC_TRY(ctx->scratch(1, P->lab_bytes + ibytes + gbytes + P->band_elems * sizeof(uint16_t) + 1024, &scratch));
   uint8_t* frames = (uint8_t*)scratch;
   uint16_t* I = (uint16_t*)((char*)scratch + P->lab_bytes);
   float* grads = grads_out ? grads_out : (float*)((char*)scratch + P->lab_bytes + ibytes);
+  uint16_t* bands = (uint16_t*)((char*)scratch + P->lab_bytes + ibytes + gbytes);
   const bool colour = part & 1, texture = part & 2;
   // frames of channel ch are f = 4 ch + o; the texture frames 12..15 come last
   const size_t split = (size_t)g.lab_off[12];
@@ -602,15 +770,24 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
   int maxh = 0, maxw = 0;
   for (int o = 0; o < 4; o++) { maxh = std::max(maxh, g.nh[o]); maxw = std::max(maxw, g.nw[o]); }
   if (colour) {
-    hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 12), dim3(256), (size_t)maxw, s, g, 0, frames, I);
-    hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 32 / 2, 256), 12), dim3(256), 0, s, g, 0, I);
+    if (P->band_ok && integral_pass == 1) {
+      DMSC_TRY(integral_bands<32>(s, g, 0, 12, maxw, maxh, frames, bands, I));
+    } else {
+      hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 12), dim3(256), (size_t)maxw, s, g, 0, frames, I);
+      hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 32 / 2, 256), 12), dim3(256), 0, s, g, 0, I);
+    }
   }
   if (texture) {
-    if (tbp == 32)
-      hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
-    else
-      hipLaunchKernelGGL(k_integral_rows<64>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
-    hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * tbp / 2, 256), 4), dim3(256), 0, s, g, 12, I);
+    if (P->band_ok && integral_pass == 1) {
+      if (tbp == 32) DMSC_TRY(integral_bands<32>(s, g, 12, 4, maxw, maxh, frames, bands, I));
+      else DMSC_TRY(integral_bands<64>(s, g, 12, 4, maxw, maxh, frames, bands, I));
+    } else {
+      if (tbp == 32)
+        hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+      else
+        hipLaunchKernelGGL(k_integral_rows<64>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);
+      hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * tbp / 2, 256), 4), dim3(256), 0, s, g, 12, I);
+    }
   }
   const size_t plane = (size_t)n;
   BlurK none;
@@ -618,17 +795,17 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
   // Bg: channel 0, radii {3,5,10}; Cga/Cgb: channels 1-2, radii {5,10,20};
   // Tg: channel 3, radii {5,10,20}.
   if (colour) {
-    hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 0, 0,
+    hipLaunchKernelGGL((k_gradients<25, 6>), dim3(div_up(n, 256) * 3, 4), dim3(256), 0, s, g, 0, 0,
                        P->d_tabs, I, P->blur_bg, grads, plane);
-    hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256), 3, 8), dim3(256), 0, s, g, 1, 1,
+    hipLaunchKernelGGL((k_gradients<25, 3>), dim3(div_up(n, 256) * 3, 8), dim3(256), 0, s, g, 1, 1,
                        P->d_tabs, I, P->blur_cg, grads, plane);
   }
   if (texture) {
     if (ntex == 64)
-      hipLaunchKernelGGL(k_gradients64, dim3(div_up(2 * n, 256), 3, 4), dim3(256), 0, s, g, 1, P->d_tabs, I, grads,
-                         plane);
+      hipLaunchKernelGGL(k_gradients64, dim3(div_up(2 * n, 256) * 3, 4), dim3(256), 0, s, g, 1, P->d_tabs, I,
+                         grads, plane);
     else
-      hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256), 3, 4), dim3(256), 0, s, g, 3, 1,
+      hipLaunchKernelGGL((k_gradients<32, 0>), dim3(div_up(n, 256) * 3, 4), dim3(256), 0, s, g, 3, 1,
                          P->d_tabs, I, none, grads, plane);
   }
   if (hist_done) DMSC_HIP(hipEventRecord(hist_done, s));

@@ -264,37 +264,14 @@ __global__ __launch_bounds__(256) void k_km_assign(int n, const float* __restric
 }
 
 // ---------------------------------------------------------------- mixed path
-// Split BF16 copy of filters 0..31 of every pixel, [pixel][filter] (the MFMA
-// B operands): xb[p] the BF16 of the response, xb[n + p] the BF16 of the
-// remainder; filter 32 stays FP32 and filter 33's response is zero.
-__global__ void k_km_bf16(int n, const float* __restrict__ resp, uint16_t* __restrict__ xb) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  uint32_t hi[16], lo[16];
-#pragma unroll
-  for (int f = 0; f < 32; f += 2) {
-    uint16_t h0, l0, h1, l1;
-    split_bf16(resp[(size_t)f * n + p], h0, l0);
-    split_bf16(resp[(size_t)(f + 1) * n + p], h1, l1);
-    hi[f / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-    lo[f / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-  }
-  uint4* dh = reinterpret_cast<uint4*>(xb + (size_t)p * 32);
-  uint4* dl = reinterpret_cast<uint4*>(xb + ((size_t)n + p) * 32);
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    dh[q] = make_uint4(hi[4 * q], hi[4 * q + 1], hi[4 * q + 2], hi[4 * q + 3]);
-    dl[q] = make_uint4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
-  }
-}
-
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Distances from v_mfma_f32_16x16x32_bf16 products (clusters as rows, in
 // K/16 tiles of 16, pixels as columns, filters 0..31 as K) plus the FP32
 // filter-32 term.  Both operands are split, x ~ x_hi + x_lo and c ~ c_hi +
-// c_lo (BF16 each), and x.c is taken as x_lo c_hi + x_hi c_lo + x_hi c_hi
+// c_lo (BF16 each, split in registers from the FP32 responses), and x.c is
+// taken as x_lo c_hi + x_hi c_lo + x_hi c_hi
 // (three MFMAs into one FP32 accumulator).  Per (pixel, cluster) the error
 // against the exact FP32 distance is below e = 2^-14 |x| |c| + 2^-20 (|x|^2
 // + |c|^2): the split leaves at most 3.01 * 2^-18 |x_f c_f| per product
@@ -311,7 +288,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // 4*(l/16)+i of column l%16.
 template <int K>
 __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __restrict__ resp,
-                                                        const uint16_t* __restrict__ xb,
                                                         const float* __restrict__ pdot, int* __restrict__ labels,
                                                         KmState* st, int it) {
   constexpr int NT = K / 16;  // cluster tiles
@@ -337,15 +313,15 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
   int changed = 0, rech = 0;
   // each wave labels 64 pixels, 16 per MFMA column tile; every tile's
   // operands are loaded before the first tile is computed
-  uint4 bv[4], blv[4];
+  float xv[4][8];
   float pdv[4], x32v[4];
   int oldl[4];  // the labels being replaced (for the change count)
 #pragma unroll
   for (int tile = 0; tile < 4; tile++) {
     const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
     const int pp = p < n ? p : n - 1;
-    bv[tile] = *reinterpret_cast<const uint4*>(xb + (size_t)pp * 32 + 8 * grp);
-    blv[tile] = *reinterpret_cast<const uint4*>(xb + ((size_t)n + pp) * 32 + 8 * grp);
+#pragma unroll
+    for (int j = 0; j < 8; j++) xv[tile][j] = resp[(size_t)(8 * grp + j) * n + pp];
     pdv[tile] = pdot[pp];
     x32v[tile] = resp[(size_t)32 * n + pp];
     oldl[tile] = labels[pp];
@@ -354,9 +330,12 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
   for (int tile = 0; tile < 4; tile++) {
     const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
     const bool live = p < n;
+    uint16_t hb[8], lb[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) split_bf16(xv[tile][j], hb[j], lb[j]);
     bf16x8 b, bl;
-    __builtin_memcpy(&b, &bv[tile], 16);
-    __builtin_memcpy(&bl, &blv[tile], 16);
+    __builtin_memcpy(&b, hb, 16);
+    __builtin_memcpy(&bl, lb, 16);
     f32x4 s[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {  // clusters 16t..16t+15; small terms first
@@ -403,16 +382,16 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
       certain = best + e < lo_other && best + e < 10000000.0f;
     }
     if (grp == 0 && live) {
-      int bl = bc;
+      int fl = bc;
       if (!certain) {  // near tie (or no finite distance): exact FP32 distances
         float x[kNFilters];
 #pragma unroll
         for (int f = 0; f < kNFilters; f++) x[f] = resp[(size_t)f * n + p];
-        bl = km_exact_label<K>(x, pd, sc);
+        fl = km_exact_label<K>(x, pd, sc);
         rech++;
       }
-      changed += bl != oldl[tile];
-      labels[p] = bl;
+      changed += fl != oldl[tile];
+      labels[p] = fl;
     }
   }
   if (changed) atomicAdd(&schanges, changed);
@@ -440,20 +419,16 @@ int launch_texton_conv(hipStream_t s, int w, int h, const float* grey, const flo
 
 size_t kmeans_state_bytes() { return sizeof(KmState); }
 
-// Split BF16 responses for the mixed path: 2 x 32 filters per pixel.
-size_t kmeans_mixed_work_bytes(int n) { return (size_t)n * 64 * sizeof(uint16_t) + 256; }
-
 namespace {
 template <int K>
 void km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels, const float* pdot, KmState* st,
-                   int max_iter, const uint16_t* xb) {
-  const int blocks_acc = std::min(div_up(n, 256), 256);
+                   int max_iter, bool mixed) {
+  const int blocks_acc = std::min(div_up(n, 256), 1024);
   for (int it = 0; it < max_iter; it++) {
     hipLaunchKernelGGL(k_km_accum<K>, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
     hipLaunchKernelGGL(k_km_centroids<K>, dim3(1), dim3(256), 0, s, nf, st, it);
-    if (xb)
-      hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb, pdot, labels,
-                         st, it);
+    if (mixed)
+      hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
     else
       hipLaunchKernelGGL(k_km_assign<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
   }
@@ -461,31 +436,29 @@ void km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels,
 
 template <int K>
 int run_kmeans(hipStream_t s, int w, int h, int nf, const float* resp, int* labels, float* pdot, KmState* st,
-               int max_iter, uint16_t* xb) {
+               int max_iter, bool mixed) {
   const int n = w * h;
   hipLaunchKernelGGL(k_km_init<K>, dim3(div_up(n, 256)), dim3(256), 0, s, w, h, nf, resp, labels, pdot, st);
-  if (xb) hipLaunchKernelGGL(k_km_bf16, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, xb);
-  km_iterations<K>(s, n, nf, resp, labels, pdot, st, max_iter, xb);
+  km_iterations<K>(s, n, nf, resp, labels, pdot, st, max_iter, mixed);
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
 }
 }  // namespace
 
 int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, int* labels, float* pdot,
-                        void* state, int max_iter, void* work) {
+                        void* state, int max_iter) {
   KmState* st = (KmState*)state;
-  uint16_t* xb = (uint16_t*)work;
-  if ((uintptr_t)xb % 16 || (uintptr_t)state % 16) return fail(DMSC_ERR_INTERNAL, "k-means buffers misaligned");
-  if (k == 64) return run_kmeans<64>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, xb);
-  if (k == 32) return run_kmeans<32>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, xb);
+  if ((uintptr_t)state % 16) return fail(DMSC_ERR_INTERNAL, "k-means state misaligned");
+  if (k == 64) return run_kmeans<64>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, true);
+  if (k == 32) return run_kmeans<32>(s, w, h, kNFilters, resp, labels, pdot, st, max_iter, true);
   return fail(DMSC_ERR_ARG, "k-means supports 32 or 64 clusters");
 }
 
 int launch_kmeans(hipStream_t s, int w, int h, int k, int nf, const float* resp, int* labels,
                   float* pdot, void* state, int max_iter) {
   KmState* st = (KmState*)state;
-  if (k == 64) return run_kmeans<64>(s, w, h, nf, resp, labels, pdot, st, max_iter, nullptr);
-  if (k == 32) return run_kmeans<32>(s, w, h, nf, resp, labels, pdot, st, max_iter, nullptr);
+  if (k == 64) return run_kmeans<64>(s, w, h, nf, resp, labels, pdot, st, max_iter, false);
+  if (k == 32) return run_kmeans<32>(s, w, h, nf, resp, labels, pdot, st, max_iter, false);
   return fail(DMSC_ERR_ARG, "k-means supports 32 or 64 clusters");
 }
 

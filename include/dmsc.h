@@ -268,6 +268,13 @@ int dmsc_write_pgm(const char* path, int w, int h, const float* h_data);
  * kernels are the ones dmsc_generalized_eigensolve uses. */
 int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int matrix_pitch, int precision,
                             int steps, float* ms_per_step);
+/* No reference counterpart: times `products` filter steps of the block
+ * eigensolver (eigensolver = 1) on a converted matrix (L_sym, as
+ * dmsc_convert_matrix leaves it) with the current chebyshev_shape tile, a
+ * block of 32 vectors, after 5 untimed steps, with HIP events on the
+ * context's stream; returns the average milliseconds per step. */
+int dmsc_chebyshev_time_products(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch,
+                                 int products, float* ms_per_product);
 /* Autotuner (no reference counterpart; the reference's launch shapes are
  * fixed).  dmsc_autotune_eigensolver converts d_matrix in place once, then
  * times every step-kernel shape of the given precision -- mixed: tiled

@@ -107,6 +107,7 @@ def test_backward_against_fp64(ctx, data):
     tie = top2[1] - top2[0] < 1e-5 * np.abs(all64).max()  # orientation: moves every vector's gradient
     tiny = rstar < 1e-5 * np.abs(R64).max()  # response sign of vector i: moves vector i's gradient
     clean, kept, namb = 0.0, [], 0
+    num, den = 0.0, 0.0
     for i in range(1, vec.shape[0]):
         amb = tie | tiny[i - 1]
         namb += int(amb.sum())
@@ -115,13 +116,20 @@ def test_backward_against_fp64(ctx, data):
             for dx in range(-3, 4):
                 near |= np.roll(np.roll(amb, dy, 0), dx, 1)
         clean = max(clean, diff[i][~near].max() / scale)
+        num += float(np.sum((gv[i] - gv64[i])[~near] ** 2))
+        den += float(np.sum(gv64[i][~near] ** 2))
         kept.append((edge & ~near).sum() / edge.sum())
     edge_kept = float(np.mean(kept))
+    l2_clean = (num / den) ** 0.5
     print(f"ambiguous (vector, pixel) pairs {namb} (orientation near-ties {int(tie.sum())}); eigvec grads away "
-          f"from their windows: max diff / max {clean:.2e} ({edge_kept:.0%} of the edge frame compared)", flush=True)
+          f"from their windows: max diff / max {clean:.2e}, rel L2 {l2_clean:.2e} ({edge_kept:.0%} of the edge "
+          f"frame compared)", flush=True)
     assert clean < 1e-4
+    assert l2_clean < 2e-5
     assert edge_kept > 0.3  # the mirror adjoint is exercised
-    assert l2 < 1e-3
+    # over every entry, the windows of the ambiguous pixels included (their
+    # share grows with the image; 481x321 measured 1.04e-3)
+    assert l2 < 3e-3
     assert gp[13] == 0 and np.all(gv[0] == 0)
 
 

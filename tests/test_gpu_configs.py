@@ -25,6 +25,7 @@ import torch
 
 import bench as B
 import damascene_amd as dm
+from damascene_amd._lib import check, lib
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -203,3 +204,27 @@ def test_sequential_stages_bit_identical(ctx):
     st = b["stage_ms"]
     print("sequential stage ms", [round(v, 3) for v in st], flush=True)
     assert 0 < st[6] < st[0] and 0 < st[7] < st[1]
+
+
+@pytest.mark.parametrize("w,h,texton_choice", [(481, 321, dm.TEXTON32), (1024, 768, dm.TEXTON32),
+                                               (160, 120, dm.TEXTON64), (61, 37, dm.TEXTON32)])
+def test_integral_passes_bit_identical(ctx, w, h, texton_choice):
+    """The single-write banded integral images (tune key integral_pass 1,
+    the default) against the row + column passes (0): the chi-squared
+    planes of all four channels, which read every corner of every
+    integral image, bit for bit."""
+    print(f"case: integral passes {w}x{h} texton_choice {texton_choice}", flush=True)
+    d_rgb = ctx.upload(dm.generate_image(w, h, max(4, w * h // 4000), SEED + 7))
+    L, a, b = ctx.rgb_to_lab(d_rgb)
+    tex, _ = ctx.find_textons(ctx.rgb_to_grey(d_rgb), texton_choice=texton_choice)
+    raws = {}
+    try:
+        for mode in (0, 1):
+            check(lib().dmsc_tune_set(b"integral_pass", mode))
+            _, raws[mode] = ctx.local_cues(L, a, b, tex, raw=True, texton_choice=texton_choice)
+            torch.cuda.synchronize()
+    finally:
+        check(lib().dmsc_tune_set(b"integral_pass", 1))
+    g0, g1 = bits(host(raws[0])), bits(host(raws[1]))
+    assert np.count_nonzero(g0 != g1) == 0
+    assert np.isfinite(host(raws[1])).all() and float(np.abs(host(raws[1])).max()) > 0

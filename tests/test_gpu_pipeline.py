@@ -260,14 +260,20 @@ def test_block_solver_large_image(ctx, w, h, seed):
     assert rel < 1e-4
     assert g < 1e-3
     # the returned pairs themselves, in FP64 against the library's L_sym: at
-    # the mixed bound (tol * 1e-4) plus the FP32 rounding of the stored,
-    # min-max normalised vectors, or at the FP32 floor when that stop fired
-    assert chk["residual_max"] < (2e-6 if rule == dm.STOP_FLOOR else 1e-6)
+    # the mixed bound (tol * 1e-4, or the FP32 floor when that stop fired)
+    # plus what FP32 storage of the min-max mapped output vectors leaves
+    # (rounding_floor: 61x37 has vectors whose varying part is small next to
+    # their constant part, and both solvers' pairs sit at about 5e-6 there)
+    bar = 2e-6 if rule == dm.STOP_FLOOR else 1e-6
+    ref_chk = B.fp64_residuals(ctx, ref, w, h)
+    print(f"FP64 residuals block {chk['residuals']} (output rounding floor {chk['rounding_floor']}); "
+          f"Lanczos {ref_chk['residuals']}", flush=True)
+    assert chk["residual_max"] < max(bar, 2 * ref_chk["residual_max"])
     assert chk["eigval_vs_rayleigh_max_rel"] < 1e-5
 
 
 @pytest.mark.parametrize("w,h,seed", [(160, 120, 20261202), (61, 37, 7)])
-@pytest.mark.parametrize("shape", [22, 28, 42])
+@pytest.mark.parametrize("shape", [22, 28, 42, 324, 344, 342, 328, 348, 1324, 1328, 1344, 1348])
 def test_block_solver_shapes(ctx, w, h, seed, shape):
     """Every filter-kernel tile of the block solver ("chebyshev_shape": rows
     per lane x 10 + float4 planes per lane) gives the default tile's (24)

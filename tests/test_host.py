@@ -361,13 +361,14 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
                                       None) == 1
     assert L.dmsc_autotune_eigensolver(None, 64, 64, None, 4096, 0, 100, None, None, 0, None) == 1
     assert L.dmsc_kmeans_rechecks(None, None) == 1
+    assert L.dmsc_chebyshev_time_products(None, 64, 64, None, 4096, 10, None) == 1
     for key, bad in ((b"lanczos_tile", 6405), (b"lanczos_max_iter", 10), (b"lanczos_mixed_stop", 9),
                      (b"lanczos_check_max", 30), (b"batch_lanes", 0), (b"eigensolver", 2),
-                     (b"chebyshev_shape", 44)):
+                     (b"chebyshev_shape", 44), (b"chebyshev_shape", 346), (b"integral_pass", 2)):
         assert L.dmsc_tune_set(key, bad) == 1, key
     v = ctypes.c_int()
     for key, want in ((b"lanczos_tile", 3208), (b"lanczos_max_iter", 6000), (b"lanczos_mixed_stop", 4),
                       (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 0),
-                      (b"chebyshev_shape", 24)):
+                      (b"chebyshev_shape", 24), (b"integral_pass", 1)):
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key
 

@@ -287,6 +287,11 @@ int dmsc_tune_set(const char* key, int value) {
     integral_pass = value;
     return DMSC_OK;
   }
+  if (!std::strcmp(key, "chebyshev_reduction")) {
+    DMSC_ARG(value >= 2 && value <= 10000, "chebyshev_reduction must be in [2, 10000]");
+    chebyshev_reduction = value;
+    return DMSC_OK;
+  }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);
 }
 int dmsc_tune_get(const char* key, int* value) {
@@ -345,6 +350,10 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "integral_pass")) {
     *value = integral_pass;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_reduction")) {
+    *value = chebyshev_reduction;
     return DMSC_OK;
   }
   return fail(DMSC_ERR_ARG, "unknown tuning key %s", key);

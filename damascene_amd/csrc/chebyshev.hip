@@ -45,6 +45,7 @@ namespace dmsc {
 
 int eigensolver_method = 0;  // mixed path: 0 Lanczos, 1 Chebyshev block (tune key "eigensolver")
 int chebyshev_shape = 24;    // filter tile: rows per lane * 10 + float4 planes per lane (24, 28, 22, 42)
+int chebyshev_reduction = 10;  // per outer iteration, the filter degree aims to cut the slowest wanted error by this
 
 namespace {
 
@@ -875,7 +876,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     // whose components grow by exp(acosh(t)) per degree against the damped
     // interval (t = its distance from the interval's centre in half-widths)
     const double tj = ((kUpper + cut) / 2 - theta[nvec - 1]) / ((kUpper - cut) / 2);
-    deg = tj > 1.0 ? (int)std::ceil(std::log(10.0) / std::acosh(tj)) : kMaxDeg;
+    deg = tj > 1.0 ? (int)std::ceil(std::log((double)chebyshev_reduction) / std::acosh(tj)) : kMaxDeg;
     deg = std::min(kMaxDeg, std::max(kMinDeg, deg));
   }
   return fail(DMSC_ERR_NOCONV, "block eigensolver did not converge in %d outer iterations", kMaxOuter);

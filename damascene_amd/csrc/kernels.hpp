@@ -122,6 +122,7 @@ extern int batch_lanes;
 // planes per lane: 24, 28, 22 or 42).
 extern int eigensolver_method;
 extern int chebyshev_shape;
+extern int chebyshev_reduction;  // tune key "chebyshev_reduction"
 bool chebyshev_shape_valid(int v);
 int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product);
 int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec, float tol, float* d_out,

@@ -6,11 +6,13 @@
 //   1. labels:    quantised L/a/b (25 bins) or textons (32/64), mirrored (border
 //                 30) and scattered through the Bresenham rotation tables into
 //                 uint8 rotated frames (255 = outside the rotated image);
-//   2. integrals: per-bin 2-D inclusive counts, row pass then column pass,
-//                 stored as uint16 modulo 2^16: every half-disc count is a
-//                 signed sum of four corners whose true value lies in
-//                 [0, 2r^2], so modular arithmetic recovers it exactly at half
-//                 the bytes of int32;
+//   2. integrals: per-bin 2-D inclusive counts, stored as uint16 modulo
+//                 2^16: every half-disc count is a signed sum of four corners
+//                 whose true value lies in [0, 2r^2], so modular arithmetic
+//                 recovers it exactly at half the bytes of int32.  A frame's
+//                 integral is BP/8 planes of 8 bins (16 bytes) per rotated
+//                 pixel: a wave's 16-byte gathers of one plane at consecutive
+//                 pixels read one contiguous run;
 //   3. gradients: per pixel, scale and split: 4+4 corner gathers per bin,
 //                 optional bin blur, normalisation to sum 8 and chi-squared
 //                 with the reference's 32-slot tree sums (64 textons: two
@@ -48,6 +50,14 @@ struct FrameGeom {
 // Bins stored per integral-image pixel of frame f: 32 for the L/a/b frames
 // (25 used), 32 or 64 for the texton frames 12..15.
 __device__ __forceinline__ int bin_pitch(const FrameGeom& g, int f) { return f >= 12 ? g.tbp : 32; }
+
+// Element index (uint16) of bin `bin` at rotated pixel `pix` of frame f:
+// plane bin / 8 holds 8 bins per pixel.
+__device__ __forceinline__ size_t int_index(const FrameGeom& g, int f, long long pix, int bin) {
+  const int o = f & 3;
+  const size_t npx = (size_t)g.nw[o] * g.nh[o];
+  return (size_t)g.int_off[f] + (size_t)(bin >> 3) * npx * 8 + (size_t)pix * 8 + (bin & 7);
+}
 
 struct CueOut {
   float* p[4];  // Bg, Cga, Cgb, Tg; each [24][pitch]
@@ -104,10 +114,10 @@ __global__ __launch_bounds__(256) void k_integral_rows(FrameGeom g, int f0, cons
   __syncthreads();
   uint32_t run = 0;
   for (int s = 0; s < seg; s++) run += segtot[s][bin];
-  uint16_t* dst = I + g.int_off[f] + (size_t)y * nw * BP;
+  uint16_t* dst = I + int_index(g, f, (long long)y * nw, bin);
   for (int x = x0; x < x1; x++) {
     run += (row[x] == bin);
-    dst[(size_t)x * BP + bin] = (uint16_t)run;
+    dst[(size_t)x * 8] = (uint16_t)run;
   }
 }
 
@@ -117,8 +127,9 @@ __global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, int f0, uint
   const int nw = g.nw[o], nh = g.nh[o], pairs = bin_pitch(g, f) / 2;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (x, bin pair)
   if (t >= nw * pairs) return;
-  uint32_t* col = reinterpret_cast<uint32_t*>(I + g.int_off[f]) + t;
-  const size_t stride = (size_t)nw * pairs;
+  const int x = t / pairs, pr = t - x * pairs;
+  uint32_t* col = reinterpret_cast<uint32_t*>(I + int_index(g, f, x, 2 * pr));
+  const size_t stride = (size_t)nw * 4;  // u32 words per plane row
   uint32_t lo = 0, hi = 0;
   int y = 0;
   for (; y + 8 <= nh; y += 8) {
@@ -151,21 +162,21 @@ __global__ __launch_bounds__(256) void k_integral_cols(FrameGeom g, int f0, uint
 // bins in registers (two u16 bins per word, modulo 2^16 like the stored
 // integrals).  The row prefix of bin k at column x is the count of k among
 // the lanes below x in its wave (ballot + mbcnt) plus the exclusive total
-// of the earlier waves (a 16 NCH-entry scan per bin through LDS).  Each
-// thread stores its pixel's whole record (BP u16, 16 bytes at a time), so
-// a wave writes consecutive records.
+// of the earlier waves (their totals through LDS, one barrier per row).
+// Each thread stores its pixel's BP bins as BP/8 16-byte plane entries, so
+// a wave's store to one plane is one contiguous kilobyte.
 constexpr int kBand = 16;
 constexpr int kIntThreads = 1024, kIntWaves = kIntThreads / 64;
 
-template <int BP, int NCH, bool FINAL>
+template <int BP, int NBU, int NCH, bool FINAL>
 __global__ __launch_bounds__(kIntThreads) void k_integral_band(FrameGeom g, int f0,
                                                                const uint8_t* __restrict__ frames,
                                                                uint16_t* __restrict__ bands,
                                                                uint16_t* __restrict__ I) {
   constexpr int NQ = BP / 2;  // words per record (two bins each)
+  constexpr int NQU = (NBU + 1) / 2;  // words holding used bins (labels are < NBU; higher bins stay 0)
   constexpr int NS = NCH * kIntWaves;
-  __shared__ uint32_t tot[2][NS][BP];  // per-wave bin totals, by row parity
-  __shared__ uint32_t pre[2][NS][BP];  // their exclusive prefix over the row
+  __shared__ uint32_t tot[2][NS][NBU];  // per-wave bin totals, by row parity
   const int f = f0 + blockIdx.y, o = f & 3, b = blockIdx.x;
   const int nw = g.nw[o], nh = g.nh[o];
   const int y0 = b * kBand;
@@ -187,54 +198,65 @@ __global__ __launch_bounds__(kIntThreads) void k_integral_band(FrameGeom g, int 
       S[c][q + 3] = v.w;
     }
   }
+  const uint8_t* lab_base = frames + g.lab_off[f];
+  uint32_t labn[NCH];  // the next row's labels, loaded one row ahead
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int x = c * kIntThreads + (int)threadIdx.x;
+    labn[c] = x < nw ? (uint32_t)lab_base[(size_t)y0 * nw + x] : 255u;
+  }
   for (int y = y0; y < y1; y++) {
     const int par = (y - y0) & 1;
-    const uint8_t* src = frames + g.lab_off[f] + (size_t)y * nw;
-    uint32_t Rl[NCH][NQ];
+    uint32_t lab[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
+      lab[c] = labn[c];
       const int x = c * kIntThreads + (int)threadIdx.x;
-      const uint32_t lab = x < nw ? (uint32_t)src[x] : 255u;
+      labn[c] = (y + 1 < y1 && x < nw) ? (uint32_t)lab_base[(size_t)(y + 1) * nw + x] : 255u;
+    }
+    uint32_t Rl[NCH][NQU];
 #pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        const unsigned long long m0 = __ballot(lab == (uint32_t)(2 * q));
-        const unsigned long long m1 = __ballot(lab == (uint32_t)(2 * q + 1));
+    for (int c = 0; c < NCH; c++) {
+#pragma unroll
+      for (int q = 0; q < NQU; q++) {
+        const unsigned long long m0 = __ballot(lab[c] == (uint32_t)(2 * q));
         const uint32_t r0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u)) +
-                            (lab == (uint32_t)(2 * q));
-        const uint32_t r1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u)) +
-                            (lab == (uint32_t)(2 * q + 1));
-        Rl[c][q] = r0 | (r1 << 16);
-        if (lane == 0) {
-          tot[par][c * kIntWaves + wv][2 * q] = (uint32_t)__popcll(m0);
-          tot[par][c * kIntWaves + wv][2 * q + 1] = (uint32_t)__popcll(m1);
+                            (lab[c] == (uint32_t)(2 * q));
+        uint32_t r1 = 0;
+        if (2 * q + 1 < NBU) {
+          const unsigned long long m1 = __ballot(lab[c] == (uint32_t)(2 * q + 1));
+          r1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u)) +
+               (lab[c] == (uint32_t)(2 * q + 1));
+          if (lane == 0) tot[par][c * kIntWaves + wv][2 * q + 1] = (uint32_t)__popcll(m1);
         }
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < BP) {
-      uint32_t run = 0;
-      for (int k = 0; k < NS; k++) {
-        pre[par][k][threadIdx.x] = run;
-        run += tot[par][k][threadIdx.x];
+        if (lane == 0) tot[par][c * kIntWaves + wv][2 * q] = (uint32_t)__popcll(m0);
+        Rl[c][q] = r0 | (r1 << 16);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       const int x = c * kIntThreads + (int)threadIdx.x;
-      const uint32_t* pw = pre[par][c * kIntWaves + wv];
+      // bin `lane`'s count over the row's earlier waves, then broadcast per bin
+      uint32_t pv = 0;
+      if (lane < NBU)
+        for (int k = 0; k < c * kIntWaves + wv; k++) pv += tot[par][k][lane];
 #pragma unroll
-      for (int q = 0; q < NQ; q++) {
+      for (int q = 0; q < NQU; q++) {
         // row prefix of bins 2q, 2q+1 (each below 2^16: no carry between halves)
-        const uint32_t r = Rl[c][q] + (pw[2 * q] | (pw[2 * q + 1] << 16));
+        const uint32_t p0 = (uint32_t)__builtin_amdgcn_readlane((int)pv, 2 * q);
+        const uint32_t p1 = 2 * q + 1 < NBU ? (uint32_t)__builtin_amdgcn_readlane((int)pv, 2 * q + 1) : 0u;
+        const uint32_t r = Rl[c][q] + (p0 | (p1 << 16));
         const uint32_t lo = (S[c][q] + r) & 0xffffu;
         const uint32_t hi = ((S[c][q] >> 16) + (r >> 16)) << 16;
         S[c][q] = lo | hi;
       }
       if (FINAL && x < nw) {
-        uint4* dst = reinterpret_cast<uint4*>(I + g.int_off[f] + ((size_t)y * nw + x) * BP);
+        // plane q / 4 of this pixel: 16 bytes; consecutive lanes, consecutive pixels
+        const size_t npx = (size_t)nw * nh;
+        uint4* dst = reinterpret_cast<uint4*>(I + g.int_off[f]) + (size_t)y * nw + x;
 #pragma unroll
-        for (int q = 0; q < NQ; q += 4) dst[q / 4] = make_uint4(S[c][q], S[c][q + 1], S[c][q + 2], S[c][q + 3]);
+        for (int q = 0; q < NQ; q += 4) dst[(q / 4) * npx] = make_uint4(S[c][q], S[c][q + 1], S[c][q + 2], S[c][q + 3]);
       }
     }
   }
@@ -381,16 +403,17 @@ __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad
   const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
   const long long pix = (long long)ry * g.nw[o] + rx;
   const int* rc = g.rect[o][rad0 + scale];
-  const uint16_t* base = I + g.int_off[f];
+  const uint4* base = reinterpret_cast<const uint4*>(I + g.int_off[f]);
+  const size_t npx = (size_t)g.nw[o] * g.nh[o];
   constexpr int NQ = (NB + 7) / 8;
   uint32_t pk[8][16];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    const uint4* src = reinterpret_cast<const uint4*>(base + (size_t)(rc[c] + pix) * 32);
+    const uint4* src = base + (size_t)(rc[c] + pix);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if (q < NQ) {
-        const uint4 v = src[q];
+        const uint4 v = src[q * npx];
         pk[c][4 * q] = v.x;
         pk[c][4 * q + 1] = v.y;
         pk[c][4 * q + 2] = v.z;
@@ -410,8 +433,8 @@ __global__ __launch_bounds__(256) void k_gradients(FrameGeom g, int ch0, int rad
 
 // The texture channel with 64 textons (rotate.cu:780-782,
 // computeGradient<64, 64, false, *>): lanes 2p and 2p+1 share pixel p,
-// holding bins 0..31 and 32..63 of every corner (adjacent 64-byte halves
-// of the pixel's 128-byte integral record).  Output as k_gradients.
+// holding bins 0..31 and 32..63 of every corner (integral planes 0..3 and
+// 4..7).  Output as k_gradients.
 __global__ __launch_bounds__(256) void k_gradients64(FrameGeom g, int rad0, const int* __restrict__ tabs,
                                                      const uint16_t* __restrict__ I, float* __restrict__ grads,
                                                      size_t plane) {
@@ -430,14 +453,15 @@ __global__ __launch_bounds__(256) void k_gradients64(FrameGeom g, int rad0, cons
   const int ry = t[2 * g.bw + g.bh + iy] + t[g.bw + ix];
   const long long pix = (long long)ry * g.nw[o] + rx;
   const int* rc = g.rect[o][rad0 + scale];
-  const uint16_t* base = I + g.int_off[f] + 32 * half;
+  const size_t npx = (size_t)g.nw[o] * g.nh[o];
+  const uint4* base = reinterpret_cast<const uint4*>(I + g.int_off[f]) + (size_t)(4 * half) * npx;  // planes 4h..4h+3
   uint32_t pk[8][16];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    const uint4* src = reinterpret_cast<const uint4*>(base + (size_t)(rc[c] + pix) * 64);
+    const uint4* src = base + (size_t)(rc[c] + pix);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const uint4 v = src[q];
+      const uint4 v = src[q * npx];
       pk[c][4 * q] = v.x;
       pk[c][4 * q + 1] = v.y;
       pk[c][4 * q + 2] = v.z;
@@ -653,22 +677,25 @@ int build_plan(hipStream_t s, int w, int h, int tbp, CuePlan& P) {
 
 // Integral images of frames f0 .. f0 + nf - 1 (BP bins) in three launches:
 // band sums, carries, final pass (frames up to 2 * kIntThreads wide).
-template <int BP, int NCH>
+template <int BP, int NBU, int NCH>
 int integral_bands_n(hipStream_t s, const FrameGeom& g, int f0, int nf, int maxw, int maxh, const uint8_t* frames,
                      uint16_t* bands, uint16_t* I) {
   const dim3 grid(div_up(maxh, kBand), nf);
-  hipLaunchKernelGGL((k_integral_band<BP, NCH, false>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands, I);
+  hipLaunchKernelGGL((k_integral_band<BP, NBU, NCH, false>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands,
+                     I);
   hipLaunchKernelGGL(k_band_carry, dim3(div_up(maxw * BP, 256), nf), dim3(256), 0, s, g, f0, bands);
-  hipLaunchKernelGGL((k_integral_band<BP, NCH, true>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands, I);
+  hipLaunchKernelGGL((k_integral_band<BP, NBU, NCH, true>), grid, dim3(kIntThreads), 0, s, g, f0, frames, bands, I);
   DMSC_HIP(hipGetLastError());
   return DMSC_OK;
 }
 
-template <int BP>
+// BP bins per record, NBU of them used (the L/a/b frames hold 25 quantised
+// bins in 32-bin records; the unused bins stay 0)
+template <int BP, int NBU>
 int integral_bands(hipStream_t s, const FrameGeom& g, int f0, int nf, int maxw, int maxh, const uint8_t* frames,
                    uint16_t* bands, uint16_t* I) {
-  if (maxw <= kIntThreads) return integral_bands_n<BP, 1>(s, g, f0, nf, maxw, maxh, frames, bands, I);
-  return integral_bands_n<BP, 2>(s, g, f0, nf, maxw, maxh, frames, bands, I);
+  if (maxw <= kIntThreads) return integral_bands_n<BP, NBU, 1>(s, g, f0, nf, maxw, maxh, frames, bands, I);
+  return integral_bands_n<BP, NBU, 2>(s, g, f0, nf, maxw, maxh, frames, bands, I);
 }
 
 struct PlanCache {
@@ -771,7 +798,7 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
   for (int o = 0; o < 4; o++) { maxh = std::max(maxh, g.nh[o]); maxw = std::max(maxw, g.nw[o]); }
   if (colour) {
     if (P->band_ok && integral_pass == 1) {
-      DMSC_TRY(integral_bands<32>(s, g, 0, 12, maxw, maxh, frames, bands, I));
+      DMSC_TRY((integral_bands<32, 25>(s, g, 0, 12, maxw, maxh, frames, bands, I)));
     } else {
       hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 12), dim3(256), (size_t)maxw, s, g, 0, frames, I);
       hipLaunchKernelGGL(k_integral_cols, dim3(div_up(maxw * 32 / 2, 256), 12), dim3(256), 0, s, g, 0, I);
@@ -779,8 +806,8 @@ int launch_local_cues_part(Ctx* ctx, hipStream_t s, int part, int w, int h, cons
   }
   if (texture) {
     if (P->band_ok && integral_pass == 1) {
-      if (tbp == 32) DMSC_TRY(integral_bands<32>(s, g, 12, 4, maxw, maxh, frames, bands, I));
-      else DMSC_TRY(integral_bands<64>(s, g, 12, 4, maxw, maxh, frames, bands, I));
+      if (tbp == 32) DMSC_TRY((integral_bands<32, 32>(s, g, 12, 4, maxw, maxh, frames, bands, I)));
+      else DMSC_TRY((integral_bands<64, 64>(s, g, 12, 4, maxw, maxh, frames, bands, I)));
     } else {
       if (tbp == 32)
         hipLaunchKernelGGL(k_integral_rows<32>, dim3(maxh, 4), dim3(256), (size_t)maxw, s, g, 12, frames, I);

@@ -87,6 +87,8 @@ SIGNATURES = {
     "dmsc_gpb": (c_int, [P, c_int, c_int, P, P, P, c_int]),
     "dmsc_gpb_batch": (c_int, [P, c_int, P, P, c_int]),
     "dmsc_kmeans_rechecks": (c_int, [P, ctypes.POINTER(c_int)]),
+    "dmsc_kmeans_noalloc": (c_int, [P, c_int, c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P]),
+    "dmsc_kmeans_noalloc_max_pixels": (ctypes.c_size_t, []),
     "dmsc_eigensolve_report": (c_int, [P, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double)]),
     "dmsc_autotune_eigensolver": (c_int, [P, c_int, c_int, P, c_int, c_int, c_int, P, P, c_int, P]),
     "dmsc_autotune_spmv": (c_int, [P, c_int, c_int, P, c_int, P, P, c_int, P, P]),

@@ -128,6 +128,8 @@ int dmsc_ctx_create(dmsc_ctx** out, int device, void* stream) {
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(DMSC_ERR_NODEV, "no HIP device available");
   DMSC_ARG(device >= 0 && device < count, "device %d out of range (%d devices)", device, count);
   DMSC_HIP(hipSetDevice(device));
+  // the fixed scratch of dmsc_kmeans_noalloc: this device's address, once
+  DMSC_TRY(noalloc_scratch_init(device, nullptr));
   dmsc_ctx* c = new dmsc_ctx();
   c->device = device;
   // NULL selects the device's default stream (as in HIP/CUDA); callers that

@@ -25,6 +25,9 @@ int launch_kmeans_mixed(hipStream_t s, int w, int h, int k, const float* resp, i
                         void* state, int max_iter);
 int kmeans_read_state(hipStream_t s, void* state, int* changes, int* iterations, int* rechecks = nullptr);
 const int* kmeans_iterations_ptr(const void* state);  // device address of the iteration count
+// Device address of the current device's copy of the fixed k-means scratch
+// of dmsc_kmeans_noalloc (looked up once per device; `device` is current).
+int noalloc_scratch_init(int device, void** out);
 
 // ------------------------------------------------------------- local cues
 size_t local_cues_scratch_bytes(int w, int h);

@@ -117,6 +117,23 @@ int dmsc_kmeans(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* d
  * BF16 MFMA distances could not certify the winner and were recomputed with
  * the exact FP32 distances. */
 int dmsc_kmeans_rechecks(dmsc_ctx* ctx, int* rechecks);
+/* No reference counterpart: dmsc_kmeans for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and labels as
+ * dmsc_kmeans; its scratch (the k-means state and |x|^2 per pixel) lives in
+ * a fixed device array of DMSC_KMEANS_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from library load to process exit, so the call allocates nothing.  Inputs
+ * of more than dmsc_kmeans_noalloc_max_pixels() pixels (2,088,320: a
+ * 1920x1080 image fits) return DMSC_ERR_ARG.  With changes and iterations
+ * both NULL the call only enqueues work on the context's stream (no
+ * read-back, no synchronisation).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other contexts,
+ * other threads) race. */
+#define DMSC_KMEANS_NOALLOC_SCRATCH_BYTES (8u << 20)
+size_t dmsc_kmeans_noalloc_max_pixels(void);
+int dmsc_kmeans_noalloc(dmsc_ctx* ctx, int w, int h, int k, int nfilters, const float* d_resp,
+                        int* d_labels, int max_iter, int conv_thresh, int precision, int* changes,
+                        int* iterations);
 /* texton.h:16 findTextons (texton_choice 1 = 32 textons, 2 = 64). */
 int dmsc_find_textons(dmsc_ctx* ctx, int w, int h, const float* d_grey, int* d_textons,
                       int texton_choice, int precision, int* iterations);

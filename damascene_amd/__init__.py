@@ -35,6 +35,34 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def tune_get(key):
+    """Current value of a process-wide tuning key (dmsc_tune_get)."""
+    v = ctypes.c_int()
+    check(lib().dmsc_tune_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+class tuned:
+    """Context manager: set tuning keys (dmsc_tune_set) for a block of code
+    and restore their previous values afterwards, e.g.
+    ``with dm.tuned(eigensolver=0): ctx.gpb(...)``."""
+
+    def __init__(self, **keys):
+        self.keys = keys
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.keys.items():
+            self.old[k] = tune_get(k)
+            check(lib().dmsc_tune_set(k.encode(), int(v)))
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            check(lib().dmsc_tune_set(k.encode(), v))
+        return False
+
+
 def pitch_in_floats(n):
     return lib().dmsc_pitch_in_floats(int(n))
 

@@ -265,7 +265,7 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "eigensolver")) {
-    DMSC_ARG(value == 0 || value == 1, "eigensolver must be 0 (Lanczos) or 1 (Chebyshev block)");
+    DMSC_ARG(value == 0 || value == 1, "eigensolver must be 0 (Lanczos) or 1 (Chebyshev block, the default)");
     eigensolver_method = value;
     return DMSC_OK;
   }

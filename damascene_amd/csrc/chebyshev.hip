@@ -43,8 +43,11 @@
 
 namespace dmsc {
 
-int eigensolver_method = 0;  // mixed path: 0 Lanczos, 1 Chebyshev block (tune key "eigensolver")
-int chebyshev_shape = 24;    // filter tile: rows per lane * 10 + float4 planes per lane (24, 28, 22, 42)
+int eigensolver_method = 1;  // mixed path: 0 Lanczos, 1 Chebyshev block, the default (tune key "eigensolver")
+// filter tile: 24, 28, 22, 42 register tiles (rows per lane * 10 + float4
+// planes per lane); 3RN LDS tiles (R rows per lane, N waves per block);
+// 13RN the same with pipelined matrix loads (1328, the default)
+int chebyshev_shape = 1328;
 int chebyshev_reduction = 10;  // per outer iteration, the filter degree aims to cut the slowest wanted error by this
 
 namespace {
@@ -732,6 +735,8 @@ int product_f64(int shape, hipStream_t s, int w, int h, const float* A, size_t p
 }
 
 }  // namespace
+
+int chebyshev_max_vectors() { return kB / 2; }
 
 bool chebyshev_shape_valid(int v) {
   return v == 22 || v == 42 || v == 24 || v == 28 || v == 324 || v == 344 || v == 342 || v == 328 || v == 348 ||

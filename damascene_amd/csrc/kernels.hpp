@@ -127,6 +127,7 @@ extern int eigensolver_method;
 extern int chebyshev_shape;
 extern int chebyshev_reduction;  // tune key "chebyshev_reduction"
 bool chebyshev_shape_valid(int v);
+int chebyshev_max_vectors();  // most eigenvectors the block solver returns
 int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product);
 int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec, float tol, float* d_out,
                     double* vals, int* products);

@@ -1314,7 +1314,9 @@ int solve(Ctx* ctx, int w, int h, float* A, int pitch, int nvec, float tol, bool
     DMSC_HIP(hipGetLastError());
     return DMSC_OK;
   };
-  if (mixed && eigensolver_method == 1 && time_steps == 0 && d_eigvecs) {
+  // the block solver (the mixed default) handles up to half its block; more
+  // wanted vectors run on the mixed Lanczos solver
+  if (mixed && eigensolver_method == 1 && nvec <= chebyshev_max_vectors() && time_steps == 0 && d_eigvecs) {
     double vals[kMaxEig];
     DMSC_TRY(chebyshev_solve(ctx, w, h, A, pitch, nvec, tol, d_eigvecs, vals, iterations));
     DMSC_TRY(finish());

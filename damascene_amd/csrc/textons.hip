@@ -146,10 +146,14 @@ __global__ __launch_bounds__(256) void k_km_accum(int n, int nf, const float* __
   const int buf = it & 1;
   for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
     const int c = labels[p];
+    // all 34 responses are loaded before the first atomic, so their loads
+    // are in flight together
+    uint32_t v[kNFilters];
+#pragma unroll
+    for (int f = 0; f < kNFilters; f++) v[f] = (uint32_t)(int)(100000.0f * resp[(size_t)f * n + p]);
     if (c < 0 || c >= K) continue;  // unassigned pixel (all distances >= 1e7)
 #pragma unroll
-    for (int f = 0; f < kNFilters; f++)
-      atomicAdd(&smass[f * K + c], (uint32_t)(int)(100000.0f * resp[(size_t)f * n + p]));
+    for (int f = 0; f < kNFilters; f++) atomicAdd(&smass[f * K + c], v[f]);
     atomicAdd(&scount[c], 1u);
   }
   __syncthreads();

@@ -28,6 +28,14 @@ PLAIN_SHAPES = [(0, 3208, 256), (0, 3208, 64)]
 TOL = {dm.PREC_MIXED: (1e-5, 1e-3), dm.PREC_FP32: (1e-3, 1e-3)}
 
 
+@pytest.fixture(scope="module", autouse=True)
+def lanczos_solver():
+    """These are the Lanczos step kernels: the mixed path runs its Lanczos
+    solver here (the block solver is its default)."""
+    with dm.tuned(eigensolver=0):
+        yield
+
+
 def set_shape(fused, tile, k1):
     check(lib().dmsc_tune_set(b"lanczos_mixed_fused", fused))
     check(lib().dmsc_tune_set(b"lanczos_tile", tile))

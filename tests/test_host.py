@@ -368,8 +368,8 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
         assert L.dmsc_tune_set(key, bad) == 1, key
     v = ctypes.c_int()
     for key, want in ((b"lanczos_tile", 3208), (b"lanczos_max_iter", 6000), (b"lanczos_mixed_stop", 4),
-                      (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 0),
-                      (b"chebyshev_shape", 24), (b"integral_pass", 1)):
+                      (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 1),
+                      (b"chebyshev_shape", 1328), (b"integral_pass", 1), (b"chebyshev_reduction", 10)):
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key
     assert L.dmsc_kmeans_noalloc(None, 64, 64, 32, 34, None, None, 15, 0, 0, None, None) == 1
 

@@ -260,6 +260,16 @@ class Context:
                                        ctypes.byref(best), us))
         return best.value, {s: us[i] for i, s in enumerate((0, 1, 2, 4))}
 
+    def autotune_chebyshev(self, w, h, m, products=100):
+        """dmsc_autotune_chebyshev on a converted matrix m: returns (filter tile
+        selected for w x h, {tile: us per step})."""
+        best, nt = ctypes.c_int(), ctypes.c_int()
+        shapes = (ctypes.c_int * 16)()
+        us = (ctypes.c_float * 16)()
+        check(lib().dmsc_autotune_chebyshev(self.handle, w, h, _p(m), m.shape[1], products, ctypes.byref(best),
+                                            shapes, us, 16, ctypes.byref(nt)))
+        return best.value, {shapes[i]: us[i] for i in range(min(nt.value, 16))}
+
     def spmv(self, w, h, m, x):
         y = self.empty(h, w)
         check(lib().dmsc_stencil_spmv(self.handle, w, h, _p(m), m.shape[1], _p(x), _p(y)))

@@ -61,6 +61,7 @@ SIGNATURES = {
     "dmsc_write_pgm": (c_int, [ctypes.c_char_p, c_int, c_int, P]),
     "dmsc_lanczos_time_steps": (c_int, [P, c_int, c_int, P, c_int, c_int, c_int, P]),
     "dmsc_chebyshev_time_products": (c_int, [P, c_int, c_int, P, c_int, c_int, P]),
+    "dmsc_autotune_chebyshev": (c_int, [P, c_int, c_int, P, c_int, c_int, P, P, P, c_int, P]),
     "dmsc_tune_set": (c_int, [ctypes.c_char_p, c_int]),
     "dmsc_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int)]),
     "dmsc_rgb_to_grey": (c_int, [P, c_int, c_int, P, P]),
@@ -94,6 +95,7 @@ SIGNATURES = {
     "dmsc_autotune_spmv": (c_int, [P, c_int, c_int, P, c_int, P, P, c_int, P, P]),
     "dmsc_autotune_clear": (c_int, []),
     "dmsc_combine_weights": (c_int, [c_int, P, P]),
+    "dmsc_symmetric_eigen": (c_int, [c_int, P, P, P]),
     "dmsc_spb_combine_forward": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, c_int, P, P]),
     "dmsc_spb_combine_backward": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, P, P, P]),
 }

@@ -28,7 +28,7 @@ HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-builtin", "-I", CSRC,
               "-I", os.path.join(ROOT, "include"), "-Wall"]
 
-HOST_ONLY = ["tables.cpp", "generator.cpp"]
+HOST_ONLY = ["tables.cpp", "generator.cpp", "symeig.cpp"]
 
 
 def _sources():

@@ -270,7 +270,7 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_shape")) {
-    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22, 42 (register tiles), 324, 344, 342, 328, 348 (LDS tiles) or 1324, 1328, 1344, 1348 (pipelined LDS tiles)");
+    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22, 42 (register tiles), 324, 344, 342, 328, 348 (LDS tiles) or 1324, 1328, 1336, 1337, 1344, 1348 (pipelined LDS tiles)");
     chebyshev_shape = value;
     return DMSC_OK;
   }
@@ -696,8 +696,19 @@ int dmsc_spb_combine_backward(dmsc_ctx* ctx, int w, int h, int cue_pitch, const 
                               grad_params, grad_eigvecs);
 }
 
+int dmsc_autotune_chebyshev(dmsc_ctx* ctx, int w, int h, const float* M, int pitch, int products, int* best_shape,
+                            int* shapes_tried, float* us_per_product, int max_tried, int* n_tried) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(M && pitch >= w * h && products >= 1 && products <= 10000, "bad arguments");
+  DMSC_ARG(max_tried >= 0 && ((shapes_tried && us_per_product) || max_tried == 0), "bad arguments");
+  return autotune_chebyshev(ctx, w, h, M, pitch, products, best_shape, shapes_tried, us_per_product, max_tried,
+                            n_tried);
+}
+
 int dmsc_autotune_clear(void) {
   autotune_clear();
+  chebyshev_autotune_clear();
   return DMSC_OK;
 }
 

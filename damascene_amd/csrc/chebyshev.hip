@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -601,44 +603,6 @@ bool cholesky(std::vector<double>& L, int m) {
   return true;
 }
 
-// Cyclic Jacobi on symmetric M (m x m): eigenvalues in ev, eigenvectors in
-// the columns of Z.
-void jacobi(std::vector<double> M, int m, std::vector<double>& ev, std::vector<double>& Z) {
-  Z.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; i++) Z[i * m + i] = 1.0;
-  for (int sweep = 0; sweep < 60; sweep++) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < m; i++)
-      for (int j = 0; j < m; j++) (i == j ? tot : off) += M[i * m + j] * M[i * m + j];
-    if (off <= 1e-30 * (tot + off)) break;
-    for (int p = 0; p < m - 1; p++)
-      for (int q = p + 1; q < m; q++) {
-        const double apq = M[p * m + q];
-        if (apq == 0.0) continue;
-        const double th = (M[q * m + q] - M[p * m + p]) / (2.0 * apq);
-        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
-        for (int k = 0; k < m; k++) {  // M <- J' M J
-          const double mkp = M[k * m + p], mkq = M[k * m + q];
-          M[k * m + p] = cs * mkp - sn * mkq;
-          M[k * m + q] = sn * mkp + cs * mkq;
-        }
-        for (int k = 0; k < m; k++) {
-          const double mpk = M[p * m + k], mqk = M[q * m + k];
-          M[p * m + k] = cs * mpk - sn * mqk;
-          M[q * m + k] = sn * mpk + cs * mqk;
-        }
-        for (int k = 0; k < m; k++) {
-          const double zkp = Z[k * m + p], zkq = Z[k * m + q];
-          Z[k * m + p] = cs * zkp - sn * zkq;
-          Z[k * m + q] = sn * zkp + cs * zkq;
-        }
-      }
-  }
-  ev.resize(m);
-  for (int i = 0; i < m; i++) ev[i] = M[i * m + i];
-}
-
 // H s = theta G s: S (row-major kB x kB, column j = s_j, ascending theta)
 // with S' G S = I.  False if G is not positive definite.
 bool rayleigh_ritz(const double* G, const double* H, std::vector<double>& theta, std::vector<double>& S) {
@@ -663,8 +627,8 @@ bool rayleigh_ritz(const double* G, const double* H, std::vector<double>& theta,
     }
   for (int i = 0; i < m; i++)
     for (int j = 0; j < i; j++) M[i * m + j] = M[j * m + i] = 0.5 * (M[i * m + j] + M[j * m + i]);
-  std::vector<double> ev, Z;
-  jacobi(M, m, ev, Z);
+  std::vector<double> ev(m), Z((size_t)m * m);
+  if (!sym_eigen(M.data(), m, ev.data(), Z.data())) return false;
   std::vector<int> ord(m);
   for (int i = 0; i < m; i++) ord[i] = i;
   std::sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
@@ -713,6 +677,8 @@ int step(int shape, hipStream_t s, int w, int h, const float* A, size_t pitch, c
   if (shape == 1328) return launch_lds<2, 8, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 1344) return launch_lds<4, 4, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 1348) return launch_lds<4, 8, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1336) return launch_lds<3, 6, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 1337) return launch_lds<3, 7, false, true>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 42) return launch_step<4, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 24) return launch_step<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 28) return launch_step<2, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
@@ -731,16 +697,43 @@ int product_f64(int shape, hipStream_t s, int w, int h, const float* A, size_t p
   if (shape == 1324) return launch_lds<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1344) return launch_lds<4, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1336) return launch_lds<3, 6, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1337) return launch_lds<3, 7, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   return launch_step<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
 }
 
 }  // namespace
 
+namespace {
+// filter tiles chosen by dmsc_autotune_chebyshev, per image size
+std::map<std::pair<int, int>, int>& tuned_filter() {
+  static std::map<std::pair<int, int>, int> t;
+  return t;
+}
+std::mutex& tuned_filter_lock() {
+  static std::mutex m;
+  return m;
+}
+}  // namespace
+
+// The filter tile used at w x h: the autotuner's pick for that size, else the
+// tuning key's.  Every tile gives bit-identical results.
+int chebyshev_shape_for(int w, int h) {
+  std::lock_guard<std::mutex> lk(tuned_filter_lock());
+  auto it = tuned_filter().find(std::make_pair(w, h));
+  return it != tuned_filter().end() ? it->second : chebyshev_shape;
+}
+
+void chebyshev_autotune_clear() {
+  std::lock_guard<std::mutex> lk(tuned_filter_lock());
+  tuned_filter().clear();
+}
+
 int chebyshev_max_vectors() { return kB / 2; }
 
 bool chebyshev_shape_valid(int v) {
   return v == 22 || v == 42 || v == 24 || v == 28 || v == 324 || v == 344 || v == 342 || v == 328 || v == 348 ||
-         v == 1324 || v == 1328 || v == 1344 || v == 1348;
+         v == 1324 || v == 1328 || v == 1344 || v == 1348 || v == 1336 || v == 1337;
 }
 
 // The nvec smallest eigenpairs of L_sym (A, already converted): unit
@@ -786,7 +779,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
         disc_index(so.dy[d], so.dx[d]) != d)
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: stencil diagonal %d at (%d, %d) is not in raster disc order", d,
                   so.dy[d], so.dx[d]);
-  const int shape = chebyshev_shape;
+  const int shape = chebyshev_shape_for(w, h);
   const size_t pitchz = (size_t)pitch;
 
   hipLaunchKernelGGL(k_cheb_init, dim3(div_up((long)kPlanes * vs, 256)), dim3(256), 0, s, n, vs, X0);
@@ -817,21 +810,17 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, kGrams), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
     hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(kGrams * kB * kB, 64)), dim3(1024), 0, s, nblk, part, gram);
     DMSC_HIP(hipMemcpyAsync(h_gram, gram, kGrams * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));
+    // one host round trip per outer iteration: the Gram matrices of this
+    // block and the residuals of the previous Ritz pairs (enqueued last
+    // iteration, read here)
     DMSC_HIP(hipStreamSynchronize(s));
     if (!rayleigh_ritz(h_gram, h_gram + kB * kB, theta, S))
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: filtered block lost rank at outer iteration %d", outer);
-    // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j,
-    // formed per pixel on the device (k_cheb_resid)
-    std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
-    for (int j = 0; j < nvec; j++) h_theta[j] = theta[j];
-    DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
-    DMSC_HIP(hipMemcpyAsync(d_theta, h_theta, sizeof(double) * nvec, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_cheb_resid, dim3(nrb), dim3(256), 0, s, n, vs, Y, Wb, rot, d_theta, nvec, rpart);
-    hipLaunchKernelGGL(k_cheb_resid_sum, dim3(nvec), dim3(256), 0, s, nrb, rpart, d_res);
-    DMSC_HIP(hipMemcpyAsync(h_res, d_res, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
-    DMSC_HIP(hipStreamSynchronize(s));
-    double resid = 0.0;
-    for (int j = 0; j < nvec; j++) resid = std::max(resid, h_res[j]);
+    double resid_prev = HUGE_VAL;  // the previous outer iteration's Ritz pairs
+    if (outer > 0) {
+      resid_prev = 0.0;
+      for (int j = 0; j < nvec; j++) resid_prev = std::max(resid_prev, h_res[j]);
+    }
     double move = 0.0;
     if (outer > 0) {
       const double* C = h_gram + 2 * kB * kB;
@@ -842,31 +831,49 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       }
     }
     if (lanczos_trace)
-      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e residual %.3e\n",
-                   outer, deg, cut, theta[0], theta[nvec - 1], move, resid);
+      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e previous residual %.3e\n",
+                   outer, deg, cut, theta[0], theta[nvec - 1], move, resid_prev);
     // done when the wanted vectors stopped moving, or (a degenerate
-    // eigenvalue leaves its vectors free to turn) when every residual meets
-    // the mixed Lanczos bound, tol * 10^-lanczos_mixed_stop; or when the
-    // residuals stagnated (three outer iterations without improving on the
-    // best by 10 %) at the floor that FP32 storage of the block sets allows:
-    // a stored entry carries a relative rounding of u = 2^-24, so a Ritz
-    // vector's residual cannot fall below about u |A - theta I| <= 2u
-    // (L_sym's spectrum lies in [0, 2]); kFloorMargin covers the rounding
-    // the filter steps accumulate (61x37 image: the residuals flatten at
-    // 1.5e-7).  That stop is reported to the caller as DMSC_STOP_FLOOR.
-    flat = resid < 0.9 * best ? 0 : flat + 1;
-    best = std::min(best, resid);
+    // eigenvalue leaves its vectors free to turn) when every residual of the
+    // previous Ritz pairs met the mixed Lanczos bound, tol *
+    // 10^-lanczos_mixed_stop (this iteration's pairs are at least as good);
+    // or when those residuals stagnated (three outer iterations without
+    // improving on the best by 10 %) at the floor that FP32 storage of the
+    // block sets allows: a stored entry carries a relative rounding of u =
+    // 2^-24, so a Ritz vector's residual cannot fall below about
+    // u |A - theta I| <= 2u (L_sym's spectrum lies in [0, 2]); kFloorMargin
+    // covers the rounding the filter steps accumulate (61x37 image: the
+    // residuals flatten at 1.5e-7).  That stop is reported to the caller as
+    // DMSC_STOP_FLOOR.
+    if (outer > 0) {
+      flat = resid_prev < 0.9 * best ? 0 : flat + 1;
+      best = std::min(best, resid_prev);
+    }
     const double target = (double)tol * std::pow(10.0, -lanczos_mixed_stop);
     const double floor32 = kFloorMargin * 2.0 * std::ldexp(1.0, -24);
     int rule = 0;
     if (outer > 0) {
-      if (resid <= target) rule = DMSC_STOP_RESIDUAL;
-      else if (move <= bound) rule = DMSC_STOP_STEADY;
-      else if (flat >= 3 && resid <= floor32) rule = DMSC_STOP_FLOOR;
+      if (move <= bound) rule = DMSC_STOP_STEADY;
+      else if (resid_prev <= target) rule = DMSC_STOP_RESIDUAL;
+      else if (flat >= 3 && resid_prev <= floor32) rule = DMSC_STOP_FLOOR;
     }
+    // the rotation S and the wanted Ritz values go to the device either way
+    std::memcpy(h_rot, S.data(), sizeof(double) * kB * kB);
+    for (int j = 0; j < nvec; j++) h_theta[j] = theta[j];
+    DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
+    DMSC_HIP(hipMemcpyAsync(d_theta, h_theta, sizeof(double) * nvec, hipMemcpyHostToDevice, s));
+    // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j,
+    // formed per pixel on the device (k_cheb_resid); read back at the next
+    // round trip (or below, when the solve stops here)
+    hipLaunchKernelGGL(k_cheb_resid, dim3(nrb), dim3(256), 0, s, n, vs, Y, Wb, rot, d_theta, nvec, rpart);
+    hipLaunchKernelGGL(k_cheb_resid_sum, dim3(nvec), dim3(256), 0, s, nrb, rpart, d_res);
+    DMSC_HIP(hipMemcpyAsync(h_res, d_res, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
     if (rule) {
       hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, nvec, nullptr, d_out);
       DMSC_HIP(hipGetLastError());
+      DMSC_HIP(hipStreamSynchronize(s));
+      double resid = 0.0;
+      for (int j = 0; j < nvec; j++) resid = std::max(resid, h_res[j]);
       for (int j = 0; j < nvec; j++) vals[j] = theta[j];
       if (products) *products = total + deg + 1;
       ctx->eig_stop = rule;
@@ -891,7 +898,10 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
 // filter tile on the converted matrix A with HIP events on the context's
 // stream: the kernel the block eigensolver spends its time in, on a block
 // of kB vectors.  Returns milliseconds per step.
-int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product) {
+int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product,
+                            int shape) {
+  if (shape == 0) shape = chebyshev_shape_for(w, h);
+  if (!chebyshev_shape_valid(shape)) return fail(DMSC_ERR_ARG, "unknown filter tile %d", shape);
   if (products < 1 || products > 100000) return fail(DMSC_ERR_ARG, "products out of range");
   hipStream_t s = ctx->stream;
   const int n = w * h;
@@ -910,12 +920,12 @@ int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, i
   hipEvent_t ev[2];
   DMSC_HIP(hipEventCreate(&ev[0]));
   DMSC_HIP(hipEventCreate(&ev[1]));
-  int rc = step(chebyshev_shape, s, w, h, A, (size_t)pitch, ring[0], nullptr, ring[1], vs, sigma / e, c, 0.0);
+  int rc = step(shape, s, w, h, A, (size_t)pitch, ring[0], nullptr, ring[1], vs, sigma / e, c, 0.0);
   for (int i = 1; i < products + 6 && rc == DMSC_OK; i++) {
     if (i == 6) rc = hipEventRecord(ev[0], s) == hipSuccess ? DMSC_OK : fail(DMSC_ERR_HIP, "event record failed");
     const double sn = 1 / (tau - sigma);
     if (rc == DMSC_OK)
-      rc = step(chebyshev_shape, s, w, h, A, (size_t)pitch, ring[i % 3], ring[(i + 2) % 3], ring[(i + 1) % 3], vs,
+      rc = step(shape, s, w, h, A, (size_t)pitch, ring[i % 3], ring[(i + 2) % 3], ring[(i + 1) % 3], vs,
                 2 * sn / e, c, sigma * sn);
     sigma = sn;
   }
@@ -930,6 +940,35 @@ int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, i
   (void)hipEventDestroy(ev[1]);
   DMSC_HIP(hipGetLastError());
   return rc;
+}
+
+// Times every filter tile at w x h (`products` steps each, on the converted
+// matrix) and makes the fastest that size's tile for later solves.
+int autotune_chebyshev(Ctx* ctx, int w, int h, const float* A, int pitch, int products, int* best, int* shapes,
+                       float* us, int max_tried, int* n_tried) {
+  static const int kShapes[] = {1328, 1324, 1336, 1337, 1344, 1348, 328, 324, 344, 342, 348, 24, 28, 22, 42};
+  int nt = 0, top = 0;
+  float top_us = 0.f;
+  for (int sh : kShapes) {
+    float ms = 0.f;
+    DMSC_TRY(time_chebyshev_products(ctx, w, h, A, pitch, products, &ms, sh));
+    if (shapes && nt < max_tried) {
+      shapes[nt] = sh;
+      us[nt] = ms * 1e3f;
+    }
+    nt++;
+    if (top == 0 || ms * 1e3f < top_us) {
+      top = sh;
+      top_us = ms * 1e3f;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(tuned_filter_lock());
+    tuned_filter()[std::make_pair(w, h)] = top;
+  }
+  if (best) *best = top;
+  if (n_tried) *n_tried = nt;
+  return DMSC_OK;
 }
 
 }  // namespace dmsc

@@ -128,7 +128,13 @@ extern int chebyshev_shape;
 extern int chebyshev_reduction;  // tune key "chebyshev_reduction"
 bool chebyshev_shape_valid(int v);
 int chebyshev_max_vectors();  // most eigenvectors the block solver returns
-int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product);
+// shape 0: the tile chebyshev_shape_for(w, h) picks
+int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product,
+                            int shape = 0);
+int chebyshev_shape_for(int w, int h);
+int autotune_chebyshev(Ctx* ctx, int w, int h, const float* A, int pitch, int products, int* best, int* shapes,
+                       float* us, int max_tried, int* n_tried);
+void chebyshev_autotune_clear();
 int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec, float tol, float* d_out,
                     double* vals, int* products);
 

@@ -39,6 +39,10 @@ constexpr int kMaxIter = 6000;    // Lanczos cap (lanczos.cu:4)
 constexpr int kCheckIter = 500;   // Lanczos check period (lanczos.cu:5)
 constexpr int kMinSide = 32;      // mirror border 30 needs >= 30 px per side
 
+// Eigenvalues (unsorted) and eigenvectors (column j of row-major Z) of a
+// symmetric n x n FP64 matrix: Householder tridiagonalisation + implicit
+// symmetric QR (symeig.cpp).  False if the iteration did not converge.
+bool sym_eigen(const double* A, int n, double* ev, double* Z);
 inline int pitch_floats(long n) { return (int)(((n + 63) / 64) * 64); }
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 

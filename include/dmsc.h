@@ -85,6 +85,11 @@ int dmsc_gamma_lut(float gamma, float* h_lut /*256*/);
 /* combine.cu:16-21: the 12 mPb coefficients and 12 combinedG weights for
  * texton_choice 1 (32 textons) or 2 (64). */
 int dmsc_combine_weights(int texton_choice, float* h_coef /*12*/, float* h_wts /*12*/);
+/* No reference counterpart: eigenvalues (unsorted) and eigenvectors (column
+ * j of the row-major n x n output) of a symmetric FP64 matrix, by
+ * Householder tridiagonalisation and implicit symmetric QR: the block
+ * eigensolver's Rayleigh-Ritz step, exported for testing. */
+int dmsc_symmetric_eigen(int n, const double* h_a, double* h_eigvals, double* h_eigvecs);
 /* convert.cu:88-95 loadPPM_rgbU (cutLoadPPM4ub packing): binary PPM (P6,
  * maxval 255) -> host uint32 per pixel, R in bits 0-7, G 8-15, B 16-23.
  * h_rgb may be NULL to query the size. */
@@ -292,6 +297,15 @@ int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int ma
  * context's stream; returns the average milliseconds per step. */
 int dmsc_chebyshev_time_products(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch,
                                  int products, float* ms_per_product);
+/* No reference counterpart: times every filter tile of the block eigensolver
+ * (the 13 chebyshev_shape values) at w x h on a converted matrix, `products`
+ * steps each, and makes the fastest the tile of later block solves at that
+ * size (tiles give bit-identical results).  shapes_tried/us_per_product
+ * receive up to max_tried (tile, microseconds per step) pairs;
+ * dmsc_autotune_clear forgets the choices. */
+int dmsc_autotune_chebyshev(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch, int products,
+                            int* best_shape, int* shapes_tried, float* us_per_product, int max_tried,
+                            int* n_tried);
 /* Autotuner (no reference counterpart; the reference's launch shapes are
  * fixed).  dmsc_autotune_eigensolver converts d_matrix in place once, then
  * times every step-kernel shape of the given precision -- mixed: tiled

@@ -122,3 +122,23 @@ def test_autotune_spmv(ctx, image, defaults):
         y = ctx.spmv(W, H, m, xd).cpu().numpy()
         assert np.array_equal(y.view(np.uint32), ref.view(np.uint32)), f"SpMV shape {shape}"
     check(lib().dmsc_tune_set(b"spmv_rows_per_lane", best))
+
+
+def test_autotune_chebyshev_filter_tile(ctx, image):
+    """dmsc_autotune_chebyshev times all 15 filter tiles of the block
+    solver at one size and makes the fastest that size's tile; every tile
+    gives bit-identical results, so the mixed pipeline's output after tuning
+    equals the untuned one bit for bit, and dmsc_autotune_clear forgets it."""
+    with dm.tuned(eigensolver=1):
+        ref = ctx.gpb(image, precision=dm.PREC_MIXED)
+        m = ctx.intervene(ref["nmax"])
+        ctx.convert_matrix(W, H, m)
+        best, us = ctx.autotune_chebyshev(W, H, m, products=20)
+        print(f"filter tiles at {W}x{H}: best {best}; " + ", ".join(f"{k}: {v:.1f} us" for k, v in us.items()),
+              flush=True)
+        assert len(us) == 15 and best in us and us[best] == min(us.values())
+        out = ctx.gpb(image, precision=dm.PREC_MIXED)
+        check(lib().dmsc_autotune_clear())
+    for f in ("eigvecs", "gpb"):
+        assert np.array_equal(out[f].cpu().numpy().view(np.uint32), ref[f].cpu().numpy().view(np.uint32)), f
+    assert out["lanczos_iterations"] == ref["lanczos_iterations"]

@@ -362,6 +362,7 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     assert L.dmsc_autotune_eigensolver(None, 64, 64, None, 4096, 0, 100, None, None, 0, None) == 1
     assert L.dmsc_kmeans_rechecks(None, None) == 1
     assert L.dmsc_chebyshev_time_products(None, 64, 64, None, 4096, 10, None) == 1
+    assert L.dmsc_autotune_chebyshev(None, 64, 64, None, 4096, 10, None, None, None, 0, None) == 1
     for key, bad in ((b"lanczos_tile", 6405), (b"lanczos_max_iter", 10), (b"lanczos_mixed_stop", 9),
                      (b"lanczos_check_max", 30), (b"batch_lanes", 0), (b"eigensolver", 2),
                      (b"chebyshev_shape", 44), (b"chebyshev_shape", 346), (b"integral_pass", 2)):
@@ -373,3 +374,37 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key
     assert L.dmsc_kmeans_noalloc(None, 64, 64, 32, 34, None, None, 15, 0, 0, None, None) == 1
 
+
+
+@pytest.mark.parametrize("n,graded,seed", [(1, False, 0), (2, False, 1), (5, True, 2), (32, False, 3), (32, True, 4),
+                                            (32, True, 5), (64, False, 6)])
+def test_symmetric_eigen_against_numpy(n, graded, seed):
+    """The block eigensolver's Rayleigh-Ritz eigensolve (Householder +
+    implicit symmetric QR, host FP64) against numpy's LAPACK eigh: sorted
+    eigenvalues, eigenvector residuals |A z - lam z| and orthonormality, on
+    random symmetric matrices and on graded ones like the solver's (diagonal
+    from 1e-8 to 1 with small couplings, and a repeated eigenvalue)."""
+    rng = np.random.default_rng(seed)
+    if graded:
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = np.logspace(-8, 0, n)
+        if n > 3:
+            lam[2] = lam[1]  # a double eigenvalue
+        A = (Q * lam) @ Q.T
+    else:
+        A = rng.standard_normal((n, n))
+    A = 0.5 * (A + A.T)
+    ev = np.zeros(n)
+    Z = np.zeros((n, n))
+    assert lib().dmsc_symmetric_eigen(n, A.ctypes.data, ev.ctypes.data, Z.ctypes.data) == 0
+    ref = np.linalg.eigvalsh(A)
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(np.sort(ev) - ref)) <= 1e-13 * scale
+    assert np.max(np.linalg.norm(A @ Z - Z * ev[None, :], axis=0)) <= 1e-13 * scale
+    assert np.max(np.abs(Z.T @ Z - np.eye(n))) <= 1e-13
+
+
+def test_symmetric_eigen_rejects_bad_arguments():
+    x = np.zeros(4)
+    assert lib().dmsc_symmetric_eigen(0, x.ctypes.data, x.ctypes.data, x.ctypes.data) == 1
+    assert lib().dmsc_symmetric_eigen(2, None, x.ctypes.data, x.ctypes.data) == 1

@@ -13,10 +13,13 @@ tests/golden/truth_481x321.npz (eigenvalue relative error, subspace angle of
 eigenvectors 1..8, gPb max-abs error, nonmax mismatches against the oracle).
 
 roofline: the dominant kernel of the timed region, the block solver's filter
-step k_cheb_lds<2, 8, false, true> (one launch per filter step: the 81-
-diagonal stencil product of a block of 32 vectors, y = a (A x - c x) - b z;
-708 algorithmic bytes per row: the matrix row 324 B, x, z and y 128 B each),
-its average duration measured with HIP events on the library's stream over
+step at the tile the library picks at 481x321 (reported as `tile`: 2337,
+k_cheb_ldp<3, 7>, 64 x 21 pixels per block, whose 512 blocks fill one round
+of the chip's block slots).  One launch per filter step
+forms the 81-diagonal stencil product of a block of 32 vectors, y = a (A x -
+c x) - b z, at 708 algorithmic bytes per row (the matrix row 324 B; x, z and
+y 128 B each).  Its average duration is measured with HIP events on the
+library's stream over
 FILTER_TIMED launches; `traffic` is the HBM bytes per launch from the
 rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed under profiles/.  Beside it:
 `valu` (the same launch's FP32 multiply-adds against the FP32 peak),
@@ -64,7 +67,12 @@ STEP_BYTES_PER_ROW = 81 * 4 + 8 + 4 + 4 + 8
 STEP_TIMED = 500
 # block eigensolver filter step (the default tile), per row: the matrix row
 # (324 B) and, for 32 vectors, x, z read and y written (3 x 128 B)
-FILTER_KERNEL = "k_cheb_lds<2, 8, false, true>"
+# the filter kernel of each pipelined LDS tile (chebyshev_shape value); the
+# library picks 2337 at 481x321 (one round of the chip's block slots)
+FILTER_KERNELS = {2328: "k_cheb_ldp<2, 8>", 2337: "k_cheb_ldp<3, 7>", 2324: "k_cheb_ldp<2, 4>",
+                  1328: "k_cheb_lds<2, 8, false, true>", 1337: "k_cheb_lds<3, 7, false, true>",
+                  1324: "k_cheb_lds<2, 4, false, true>", 1336: "k_cheb_lds<3, 6, false, true>",
+                  1344: "k_cheb_lds<4, 4, false, true>", 1348: "k_cheb_lds<4, 8, false, true>"}
 FILTER_BYTES_PER_ROW = 81 * 4 + 3 * 32 * 4
 FILTER_FLOPS_PER_ROW = 2 * 81 * 32
 FILTER_TIMED = 300
@@ -104,6 +112,10 @@ def parse():
     p.add_argument("--pmc-mfma", metavar="DIR",
                    help="summarise a rocprofv3 --pmc SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_VALU_MFMA_BUSY_CYCLES "
                         "GRBM_GUI_ACTIVE pass of --pmc-workload into MFMA FLOPs and utilisation of --kernel")
+    p.add_argument("--pmc-valu", metavar="DIR",
+                   help="summarise a rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_WAVES "
+                        "GRBM_GUI_ACTIVE pass of --pmc-workload into VALU issue utilisation of --kernel (a kernel "
+                        "name or the 'histogram' group)")
     p.add_argument("--pmc-sq", metavar="DIR",
                    help="summarise a rocprofv3 --pmc pass of SQ wave-state counters (SQ_WAVES SQ_WAVE_CYCLES "
                         "SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VMEM "
@@ -117,7 +129,9 @@ def pmc_mfma(d, kernel):
     """MFMA work and utilisation of `kernel` from one rocprofv3 pass with
     SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE:
     BF16 MFMA FLOPs = 512 per MOP; utilisation = busy cycles summed over the
-    SIMDs / (GPU-active cycles x 1024 SIMDs), rocprofv3's MfmaUtil."""
+    SIMDs / (GPU-active cycles per XCD x 1024 SIMDs), GRBM_GUI_ACTIVE summing
+    the 8 XCDs (and including the profiler's dispatch window, so the figure
+    is a lower bound)."""
     import csv
     import glob
 
@@ -140,8 +154,63 @@ def pmc_mfma(d, kernel):
     ns = sum(e["ns"] for e in rows.values()) / n
     print(f"{kernel} ({n} dispatches): BF16 MFMA {flops / 1e6:.1f} MFLOP per dispatch, "
           f"MFMA busy {busy:.0f} SIMD-cycles, GPU active {active:.0f} cycles, "
-          f"MfmaUtil {100 * busy / (active * 1024):.3f} %, {ns / 1e3:.1f} us under the profiler "
+          f"MFMA utilisation >= {100 * busy / (active / 8 * 1024):.3f} %, {ns / 1e3:.1f} us under the profiler "
           f"({flops / ns / 1e3:.2f} TFLOP/s)")
+
+
+def pmc_valu(d, kernel):
+    """VALU issue of `kernel` (a name, or a PMC_GROUPS group summed over its
+    kernels) from one rocprofv3 pass with SQ_INSTS_VALU, SQ_INSTS_LDS,
+    SQ_INSTS_VMEM_RD, SQ_WAVES and GRBM_GUI_ACTIVE: VALU instructions per
+    wave, and the VALU issue utilisation = SQ_INSTS_VALU x 4 cycles (a wave64
+    FP32 instruction on a 16-lane SIMD) / (cycles x 1024 SIMDs).  Two cycle
+    counts are used, each an upper bound on the kernel's own cycles, so both
+    figures are lower bounds on how busy the vector ALUs were:
+    GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs and includes the
+    profiler's dispatch window) and the dispatch duration at the 2.4 GHz peak
+    engine clock."""
+    import csv
+    import glob
+
+    group = PMC_GROUPS.get(kernel)
+    pattern = (r"(?:^|::|\s)(?:" + group[0] + r")[<(]") if group else (re.escape(kernel) + r"\(")
+    rows = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if not re.search(pattern, r["Kernel_Name"]):
+                continue
+            e = rows.setdefault((r["Kernel_Name"], r["Dispatch_Id"]),
+                                {"ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"])})
+            c, v = r["Counter_Name"], float(r["Counter_Value"])
+            e[c] = max(e.get(c, 0.0), v) if c == "GRBM_GUI_ACTIVE" else e.get(c, 0.0) + v
+    if not rows:
+        raise SystemExit(f"no counter rows for {kernel} under {d}")
+    by_kernel = {}
+    for (k, _), e in rows.items():
+        name = re.sub(r"\(.*", "", k.replace("dmsc::(anonymous namespace)::", "").replace("void ", ""))
+        acc = by_kernel.setdefault(name, {"n": 0})
+        acc["n"] += 1
+        for c, v in e.items():
+            acc[c] = acc.get(c, 0.0) + v
+    def util(valu, grbm, ns):
+        return (100 * valu * 4 / max(grbm / 8 * 1024, 1), 100 * valu * 4 / max(ns * 2.4 * 1024, 1))
+
+    tot = {"SQ_INSTS_VALU": 0.0, "GRBM_GUI_ACTIVE": 0.0, "ns": 0.0}
+    for name, a in sorted(by_kernel.items(), key=lambda kv: -kv[1].get("ns", 0.0)):
+        n = a["n"]
+        waves = a.get("SQ_WAVES", 0.0) / n
+        valu, act, ns = a.get("SQ_INSTS_VALU", 0.0) / n, a.get("GRBM_GUI_ACTIVE", 0.0) / n, a["ns"] / n
+        u1, u2 = util(valu, act, ns)
+        print(f"{name} ({n} dispatches, {ns / 1e3:.1f} us each under the profiler): {waves:.0f} waves, VALU "
+              f"{valu / max(waves, 1):.0f} / LDS {a.get('SQ_INSTS_LDS', 0.0) / n / max(waves, 1):.0f} / VMEM read "
+              f"{a.get('SQ_INSTS_VMEM_RD', 0.0) / n / max(waves, 1):.0f} instructions per wave; VALU issue >= "
+              f"{u1:.1f} % (GRBM_GUI_ACTIVE / 8 = {act / 8:.0f} cycles), >= {u2:.1f} % (duration at 2.4 GHz)")
+        for k in tot:
+            tot[k] += a.get(k, 0.0)
+    if group:
+        u1, u2 = util(tot["SQ_INSTS_VALU"], tot["GRBM_GUI_ACTIVE"], tot["ns"])
+        print(f"{kernel} (all kernels, summed, {tot['ns'] / 1e3:.1f} us): VALU issue >= {u1:.1f} % (GRBM_GUI_ACTIVE "
+              f"/ 8), >= {u2:.1f} % (duration at 2.4 GHz)")
 
 
 def pmc_sq(d, kernel):
@@ -214,7 +283,7 @@ def pmc_summarize(fetch_dir, write_dir, kernel):
     traffic = (f["kib"] * f["scale"] + w["kib"] * w["scale"]) * 1024
     if group:
         algo = group[1]
-    elif kernel == FILTER_KERNEL:
+    elif kernel in FILTER_KERNELS.values():
         algo = FILTER_BYTES_PER_ROW * W * H
     else:
         algo = (STEP_BYTES_PER_ROW if kernel == STEP_KERNEL else SPMV_BYTES_PER_ROW) * W * H
@@ -307,15 +376,18 @@ def filter_roofline(ctx, nmax):
     ms = ctypes.c_float()
     check(lib().dmsc_chebyshev_time_products(ctx.handle, W, H, m.data_ptr(), m.shape[1], FILTER_TIMED,
                                              ctypes.byref(ms)))
+    tile = ctx.chebyshev_tile(W, H)
+    kernel = FILTER_KERNELS.get(tile, f"filter tile {tile}")
     avg_s = ms.value / 1e3
     algo = FILTER_BYTES_PER_ROW * W * H
     achieved = algo / avg_s / 1e9
     tflops = FILTER_FLOPS_PER_ROW * W * H / avg_s / 1e12
     return {
-        "kernel": FILTER_KERNEL + " (block eigensolver filter step: 81-diagonal stencil product of 32 vectors, "
-                                  "LDS-staged input, pipelined matrix loads)",
+        "kernel": kernel + " (block eigensolver filter step: 81-diagonal stencil product of 32 vectors, "
+                           "LDS-staged input, pipelined matrix loads)",
+        "tile": tile,
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(FILTER_KERNEL),
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(kernel),
         "avg_us": round(avg_s * 1e6, 3), "algorithmic_bytes_per_launch": algo, "launches_timed": FILTER_TIMED,
         "valu": {"achieved_tflops": round(tflops, 2), "peak_tflops": FP32_PEAK_TFLOPS,
                  "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "flops_per_launch": FILTER_FLOPS_PER_ROW * W * H},
@@ -580,7 +652,8 @@ def large(ctx, steps):
     ctx.convert_matrix(w, h, mf)
     ms = ctypes.c_float()
     check(lib().dmsc_chebyshev_time_products(ctx.handle, w, h, mf.data_ptr(), mf.shape[1], 100, ctypes.byref(ms)))
-    res["filter"] = {"kernel": FILTER_KERNEL, "avg_us": round(ms.value * 1e3, 3),
+    tile = ctx.chebyshev_tile(w, h)
+    res["filter"] = {"kernel": FILTER_KERNELS.get(tile, f"filter tile {tile}"), "avg_us": round(ms.value * 1e3, 3),
                      "achieved_gbs": round(FILTER_BYTES_PER_ROW * n / (ms.value * 1e-3) / 1e9, 1)}
     res["filter"]["frac"] = round(res["filter"]["achieved_gbs"] / HBM_PEAK_GBS, 4)
     m = ctx.intervene(outs["mixed"]["nmax"])
@@ -765,6 +838,9 @@ def main():
         return
     if a.pmc_mfma:
         pmc_mfma(a.pmc_mfma, a.kernel)
+        return
+    if a.pmc_valu:
+        pmc_valu(a.pmc_valu, a.kernel)
         return
     if a.pmc_sq:
         pmc_sq(a.pmc_sq, a.kernel)

@@ -260,15 +260,21 @@ class Context:
                                        ctypes.byref(best), us))
         return best.value, {s: us[i] for i, s in enumerate((0, 1, 2, 4))}
 
+    def chebyshev_tile(self, w, h):
+        """dmsc_chebyshev_tile: the block solver's filter tile at w x h."""
+        v = ctypes.c_int()
+        check(lib().dmsc_chebyshev_tile(self.handle, w, h, ctypes.byref(v)))
+        return v.value
+
     def autotune_chebyshev(self, w, h, m, products=100):
         """dmsc_autotune_chebyshev on a converted matrix m: returns (filter tile
         selected for w x h, {tile: us per step})."""
         best, nt = ctypes.c_int(), ctypes.c_int()
-        shapes = (ctypes.c_int * 16)()
-        us = (ctypes.c_float * 16)()
+        shapes = (ctypes.c_int * 32)()
+        us = (ctypes.c_float * 32)()
         check(lib().dmsc_autotune_chebyshev(self.handle, w, h, _p(m), m.shape[1], products, ctypes.byref(best),
-                                            shapes, us, 16, ctypes.byref(nt)))
-        return best.value, {shapes[i]: us[i] for i in range(min(nt.value, 16))}
+                                            shapes, us, 32, ctypes.byref(nt)))
+        return best.value, {shapes[i]: us[i] for i in range(min(nt.value, 32))}
 
     def spmv(self, w, h, m, x):
         y = self.empty(h, w)

@@ -62,6 +62,7 @@ SIGNATURES = {
     "dmsc_lanczos_time_steps": (c_int, [P, c_int, c_int, P, c_int, c_int, c_int, P]),
     "dmsc_chebyshev_time_products": (c_int, [P, c_int, c_int, P, c_int, c_int, P]),
     "dmsc_autotune_chebyshev": (c_int, [P, c_int, c_int, P, c_int, c_int, P, P, P, c_int, P]),
+    "dmsc_chebyshev_tile": (c_int, [P, c_int, c_int, P]),
     "dmsc_tune_set": (c_int, [ctypes.c_char_p, c_int]),
     "dmsc_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int)]),
     "dmsc_rgb_to_grey": (c_int, [P, c_int, c_int, P, P]),

@@ -270,7 +270,7 @@ int dmsc_tune_set(const char* key, int value) {
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_shape")) {
-    DMSC_ARG(chebyshev_shape_valid(value), "chebyshev_shape must be 24, 28, 22, 42 (register tiles), 324, 344, 342, 328, 348 (LDS tiles) or 1324, 1328, 1336, 1337, 1344, 1348 (pipelined LDS tiles)");
+    DMSC_ARG(value == 0 || chebyshev_shape_valid(value), "chebyshev_shape must be 0 (chosen per size), 24, 28, 22, 42 (register tiles), 324, 344, 342, 328, 348 (LDS tiles), 1324, 1328, 1336, 1337, 1344, 1348 (pipelined LDS tiles) or 2324, 2328, 2337 (pipelined, LDS rows read ahead)");
     chebyshev_shape = value;
     return DMSC_OK;
   }
@@ -287,6 +287,16 @@ int dmsc_tune_set(const char* key, int value) {
   if (!std::strcmp(key, "integral_pass")) {
     DMSC_ARG(value == 0 || value == 1, "integral_pass must be 0 (row + column passes) or 1 (banded single write)");
     integral_pass = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_first_degree")) {
+    DMSC_ARG(value >= 2 && value <= 300, "chebyshev_first_degree must be in [2, 300]");
+    chebyshev_first_degree = value;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_start")) {
+    DMSC_ARG(value == 0 || value == 1, "chebyshev_start must be 0 (hashed uniform values) or 1 (low-frequency cosines)");
+    chebyshev_start = value;
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_reduction")) {
@@ -352,6 +362,14 @@ int dmsc_tune_get(const char* key, int* value) {
   }
   if (!std::strcmp(key, "integral_pass")) {
     *value = integral_pass;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_first_degree")) {
+    *value = chebyshev_first_degree;
+    return DMSC_OK;
+  }
+  if (!std::strcmp(key, "chebyshev_start")) {
+    *value = chebyshev_start;
     return DMSC_OK;
   }
   if (!std::strcmp(key, "chebyshev_reduction")) {
@@ -704,6 +722,14 @@ int dmsc_autotune_chebyshev(dmsc_ctx* ctx, int w, int h, const float* M, int pit
   DMSC_ARG(max_tried >= 0 && ((shapes_tried && us_per_product) || max_tried == 0), "bad arguments");
   return autotune_chebyshev(ctx, w, h, M, pitch, products, best_shape, shapes_tried, us_per_product, max_tried,
                             n_tried);
+}
+
+int dmsc_chebyshev_tile(dmsc_ctx* ctx, int w, int h, int* shape) {
+  CHECK_CTX(ctx);
+  CHECK_SIZE(w, h);
+  DMSC_ARG(shape != nullptr, "null pointer");
+  *shape = chebyshev_shape_for(ctx->device, w, h);
+  return DMSC_OK;
 }
 
 int dmsc_autotune_clear(void) {

@@ -48,15 +48,18 @@ namespace dmsc {
 int eigensolver_method = 1;  // mixed path: 0 Lanczos, 1 Chebyshev block, the default (tune key "eigensolver")
 // filter tile: 24, 28, 22, 42 register tiles (rows per lane * 10 + float4
 // planes per lane); 3RN LDS tiles (R rows per lane, N waves per block);
-// 13RN the same with pipelined matrix loads (1328, the default)
-int chebyshev_shape = 1328;
-int chebyshev_reduction = 10;  // per outer iteration, the filter degree aims to cut the slowest wanted error by this
+// 13RN the same with pipelined matrix loads; 23RN those with the LDS rows
+// read ahead of their use; 0 (the default) picks 2328 or 2337 per image
+// size (chebyshev_shape_for)
+int chebyshev_shape = 0;
+int chebyshev_reduction = 10;
+int chebyshev_first_degree = 50;  // filter degree of the first outer iteration (tune key "chebyshev_first_degree")
+int chebyshev_start = 1;  // starting block: 0 hashed uniform values, 1 low-frequency cosines (the default)  // per outer iteration, the filter degree aims to cut the slowest wanted error by this
 
 namespace {
 
 constexpr int kB = 32;             // block vectors
 constexpr int kPlanes = kB / 4;    // float4 planes
-constexpr int kDeg = 50;           // filter degree of the first outer iteration
 constexpr int kMinDeg = 30, kMaxDeg = 300;
 constexpr int kMaxOuter = 60;
 constexpr double kFirstCut = 1e-2;  // damped interval of the first filter: [kFirstCut, 2]
@@ -237,13 +240,30 @@ __device__ __forceinline__ void cheb_load_a(int dx, __amdgpu_buffer_rsrc_t rs, u
           float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff[r], (unsigned)disc_index(dy, dx) * pitch_bytes, 0));
 }
 
-template <int HM, int R, typename T>
+// PD > 0: each input row is read from LDS PD rows ahead of the row it
+// feeds, so PD rows of reads are in flight while the FMAs run (the sums and
+// their order are unchanged).
+template <int HM, int R, typename T, int PD = 0>
 __device__ __forceinline__ void cheb_apply(int dx, const float (&a)[R][11], const float4* s0, const float4* s1, int sw,
                                            T (&acc)[R][8]) {
+  constexpr int NY = R + 2 * HM;
+  float4 b0[NY], b1[NY];
+#pragma unroll
+  for (int k = 0; k < PD && k < NY; k++) {
+    b0[k] = s0[(k - HM + 5) * sw + dx];
+    b1[k] = s1[(k - HM + 5) * sw + dx];
+  }
 #pragma unroll
   for (int yy = -HM; yy < R + HM; yy++) {
-    const float4 v0 = s0[(yy + 5) * sw + dx];
-    const float4 v1 = s1[(yy + 5) * sw + dx];
+    const int k = yy + HM;
+    if (PD == 0) {
+      b0[k] = s0[(yy + 5) * sw + dx];
+      b1[k] = s1[(yy + 5) * sw + dx];
+    } else if (k + PD < NY) {
+      b0[k + PD] = s0[(yy + PD + 5) * sw + dx];
+      b1[k + PD] = s1[(yy + PD + 5) * sw + dx];
+    }
+    const float4 v0 = b0[k], v1 = b1[k];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int dy = yy - r;
@@ -278,14 +298,13 @@ __device__ __forceinline__ void cheb_fence(T (&acc)[R][8]) {
 // eleven stages first and spills them).
 #define CHEB_STAGE(HMC, HMN, DX, CUR, NXT)                                   \
   cheb_load_a<HMN, R>((DX) + 1, rs, pitch_bytes, voff, NXT);                 \
-  cheb_apply<HMC, R, T>((DX), CUR, s0, s1, SW, acc);                         \
+  cheb_apply<HMC, R, T, PD>((DX), CUR, s0, s1, SW, acc);                     \
   cheb_fence<R, T>(acc);
 
-template <int R, int NW, bool F64, bool PF = false>
-__global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, const float* __restrict__ A,
-                                                      size_t pitch, const float4* __restrict__ x,
-                                                      const float4* __restrict__ z, float4* __restrict__ y,
-                                                      size_t vs, float alpha, float c, float beta) {
+template <int R, int NW, bool F64, bool PF, int PD>
+__device__ __forceinline__ void cheb_lds_body(int w, int h, int ntx, const float* __restrict__ A, size_t pitch,
+                                              const float4* __restrict__ x, const float4* __restrict__ z,
+                                              float4* __restrict__ y, size_t vs, float alpha, float c, float beta) {
   constexpr int TH = R * NW, SW = 64 + 10, SH = TH + 10, NT = 64 * NW;
   constexpr int kSlots = 2 * SH * SW, kIt = (kSlots + NT - 1) / NT;
   __shared__ float4 sx[2][SH][SW];
@@ -348,7 +367,7 @@ __global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, con
     CHEB_STAGE(4, 4, 2, a1, a0)
     CHEB_STAGE(4, 3, 3, a0, a1)
     CHEB_STAGE(3, 0, 4, a1, a0)
-    cheb_apply<0, R, T>(5, a0, s0, s1, SW, acc);
+    cheb_apply<0, R, T, PD>(5, a0, s0, s1, SW, acc);
   } else {
 #pragma unroll 1
     for (int dx = -5; dx <= 5; dx++) {
@@ -379,6 +398,35 @@ __global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, con
   }
 }
 
+template <int R, int NW, bool F64, bool PF = false>
+__global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, const float* __restrict__ A,
+                                                      size_t pitch, const float4* __restrict__ x,
+                                                      const float4* __restrict__ z, float4* __restrict__ y,
+                                                      size_t vs, float alpha, float c, float beta) {
+  cheb_lds_body<R, NW, F64, PF, 0>(w, h, ntx, A, pitch, x, z, y, vs, alpha, c, beta);
+}
+
+// k_cheb_lds (FP32 sums, pipelined matrix loads) with the LDS rows read
+// three rows ahead of their FMAs: bit-identical results.
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_cheb_ldp(int w, int h, int ntx, const float* __restrict__ A,
+                                                      size_t pitch, const float4* __restrict__ x,
+                                                      const float4* __restrict__ z, float4* __restrict__ y,
+                                                      size_t vs, float alpha, float c, float beta) {
+  cheb_lds_body<R, NW, false, true, 3>(w, h, ntx, A, pitch, x, z, y, vs, alpha, c, beta);
+}
+
+// Uniform value in [-1, 1) from an integer hash of element index i.
+__device__ __forceinline__ float cheb_hash(size_t i) {
+  unsigned u = (unsigned)i * 0x9E3779B1u + 0x7F4A7C15u;
+  u ^= u >> 16;
+  u *= 0x85EBCA6Bu;
+  u ^= u >> 13;
+  u *= 0xC2B2AE35u;
+  u ^= u >> 16;
+  return (float)(u >> 8) * (1.0f / 8388608.0f) - 1.0f;
+}
+
 // Starting block: uniform values in [-1, 1) from an integer hash of
 // (plane, pixel, component); rows n..vs-1 are 0.
 __global__ void k_cheb_init(int n, size_t vs, float4* __restrict__ x) {
@@ -386,16 +434,46 @@ __global__ void k_cheb_init(int n, size_t vs, float4* __restrict__ x) {
   if (i >= (size_t)kPlanes * vs) return;
   const int p = (int)(i % vs);
   float e[4];
+  for (int k = 0; k < 4; k++) e[k] = p < n ? cheb_hash(i * 4 + k) : 0.0f;
+  x[i] = make_float4(e[0], e[1], e[2], e[3]);
+}
+
+// The 32 lowest cosine modes of a w x h grid (the eigenvectors of the grid's
+// uniform Laplacian): vector j is cos(pi kx_j (x + 1/2) / w) cos(pi ky_j (y +
+// 1/2) / h).
+struct CosModes {
+  int kx[kB], ky[kB];
+};
+
+// Starting block of low-frequency cosines plus 2^-6 of the hashed values
+// of k_cheb_init, so that every eigenvector has a share of the start.
+__global__ void k_cheb_init_cos(int w, int n, size_t vs, CosModes m, float4* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)kPlanes * vs) return;
+  const int pl = (int)(i / vs), p = (int)(i % vs);
+  const int h = n / w, y = p / w, xx = p - y * w;
+  float e[4];
   for (int k = 0; k < 4; k++) {
-    unsigned u = (unsigned)(i * 4 + k) * 0x9E3779B1u + 0x7F4A7C15u;
-    u ^= u >> 16;
-    u *= 0x85EBCA6Bu;
-    u ^= u >> 13;
-    u *= 0xC2B2AE35u;
-    u ^= u >> 16;
-    e[k] = p < n ? (float)(u >> 8) * (1.0f / 8388608.0f) - 1.0f : 0.0f;
+    const int j = pl * 4 + k;
+    e[k] = p < n ? cospif(m.kx[j] * (xx + 0.5f) / (float)w) * cospif(m.ky[j] * (y + 0.5f) / (float)h) +
+                       0.015625f * cheb_hash(i * 4 + k)
+                 : 0.0f;
   }
   x[i] = make_float4(e[0], e[1], e[2], e[3]);
+}
+
+CosModes cos_modes(int w, int h) {
+  std::vector<std::pair<double, int>> f;
+  for (int kx = 0; kx < std::min(w, kB); kx++)
+    for (int ky = 0; ky < std::min(h, kB); ky++)
+      f.push_back({(double)kx * kx / ((double)w * w) + (double)ky * ky / ((double)h * h), kx * kB + ky});
+  std::sort(f.begin(), f.end());
+  CosModes m{};
+  for (int j = 0; j < kB; j++) {  // a grid of fewer than 32 modes repeats them (the hashed part differs)
+    m.kx[j] = f[j % f.size()].second / kB;
+    m.ky[j] = f[j % f.size()].second % kB;
+  }
+  return m;
 }
 
 // Partial Gram matrices over one chunk of pixels, FP64: blockIdx.y = 0
@@ -667,7 +745,21 @@ int launch_lds(hipStream_t s, int w, int h, const float* A, size_t pitch, const 
   return DMSC_OK;
 }
 
+template <int R, int NW>
+int launch_ldp(hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, const float4* z, float4* y,
+               size_t vs, double alpha, double c, double beta) {
+  if ((size_t)kStencilArea * pitch * sizeof(float) >= ((size_t)1 << 31))  // one buffer descriptor's range
+    return launch_lds<R, NW, false, false>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  const int ntx = div_up(w, 64), nty = div_up(h, R * NW);
+  hipLaunchKernelGGL((k_cheb_ldp<R, NW>), dim3(ntx * nty * 4), dim3(64 * NW), 0, s, w, h, ntx, A, pitch, x, z, y, vs,
+                     (float)alpha, (float)c, (float)beta);
+  return DMSC_OK;
+}
+
 int step(int shape, hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, const float4* z, float4* y, size_t vs, double alpha, double c, double beta) {
+  if (shape == 2324) return launch_ldp<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 2328) return launch_ldp<2, 8>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
+  if (shape == 2337) return launch_ldp<3, 7>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 324) return launch_lds<2, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 344) return launch_lds<4, 4>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
   if (shape == 342) return launch_lds<4, 2>(s, w, h, A, pitch, x, z, y, vs, alpha, c, beta);
@@ -698,7 +790,9 @@ int product_f64(int shape, hipStream_t s, int w, int h, const float* A, size_t p
   if (shape == 1328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1344) return launch_lds<4, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1336) return launch_lds<3, 6, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
-  if (shape == 1337) return launch_lds<3, 7, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1337 || shape == 2337) return launch_lds<3, 7, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 2324) return launch_lds<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 2328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   return launch_step<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
 }
 
@@ -716,12 +810,40 @@ std::mutex& tuned_filter_lock() {
 }
 }  // namespace
 
+namespace {
+// Blocks of a filter kernel the device holds at once: CUs x resident blocks
+// per CU (LDS bounds both default tiles to two per CU on MI355X).
+template <typename K>
+int block_slots(int device, K kernel, int threads) {
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
+}  // namespace
+
 // The filter tile used at w x h: the autotuner's pick for that size, else the
-// tuning key's.  Every tile gives bit-identical results.
-int chebyshev_shape_for(int w, int h) {
+// tuning key's; with the key at 0, tile 2337 (64 x 21 pixels, 7 waves of 3
+// rows, LDS rows read ahead) when its grid fits in one round of the device's
+// block slots and the 64 x 16 tile's (8 waves of 2 rows) does not, else
+// 2328.  At 481x321 the 64 x 21 tile runs 512 blocks in one round of 512
+// slots where the 64 x 16 one runs 672 in 1.31 rounds, the second mostly
+// idle (25.1 against 26.9 us per step); where both need several rounds the
+// shorter blocks balance better.  Every tile gives bit-identical results.
+int chebyshev_shape_for(int device, int w, int h) {
   std::lock_guard<std::mutex> lk(tuned_filter_lock());
   auto it = tuned_filter().find(std::make_pair(w, h));
-  return it != tuned_filter().end() ? it->second : chebyshev_shape;
+  if (it != tuned_filter().end()) return it->second;
+  if (chebyshev_shape) return chebyshev_shape;
+  static std::map<int, std::pair<int, int>> slots;  // device -> (tile 2328, tile 2337)
+  auto sl = slots.find(device);
+  if (sl == slots.end())
+    sl = slots.emplace(device, std::make_pair(block_slots(device, k_cheb_ldp<2, 8>, 64 * 8),
+                                              block_slots(device, k_cheb_ldp<3, 7>, 64 * 7))).first;
+  const int ntx = div_up(w, 64);
+  const long b1328 = 4L * ntx * div_up(h, 16), b1337 = 4L * ntx * div_up(h, 21);
+  return (b1337 <= sl->second.second && b1328 > sl->second.first) ? 2337 : 2328;
 }
 
 void chebyshev_autotune_clear() {
@@ -733,7 +855,8 @@ int chebyshev_max_vectors() { return kB / 2; }
 
 bool chebyshev_shape_valid(int v) {
   return v == 22 || v == 42 || v == 24 || v == 28 || v == 324 || v == 344 || v == 342 || v == 328 || v == 348 ||
-         v == 1324 || v == 1328 || v == 1344 || v == 1348 || v == 1336 || v == 1337;
+         v == 1324 || v == 1328 || v == 1344 || v == 1348 || v == 1336 || v == 1337 || v == 2324 || v == 2328 ||
+         v == 2337;
 }
 
 // The nvec smallest eigenpairs of L_sym (A, already converted): unit
@@ -779,14 +902,18 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
         disc_index(so.dy[d], so.dx[d]) != d)
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: stencil diagonal %d at (%d, %d) is not in raster disc order", d,
                   so.dy[d], so.dx[d]);
-  const int shape = chebyshev_shape_for(w, h);
+  const int shape = chebyshev_shape_for(ctx->device, w, h);
   const size_t pitchz = (size_t)pitch;
 
-  hipLaunchKernelGGL(k_cheb_init, dim3(div_up((long)kPlanes * vs, 256)), dim3(256), 0, s, n, vs, X0);
+  if (chebyshev_start == 1)
+    hipLaunchKernelGGL(k_cheb_init_cos, dim3(div_up((long)kPlanes * vs, 256)), dim3(256), 0, s, w, n, vs,
+                       cos_modes(w, h), X0);
+  else
+    hipLaunchKernelGGL(k_cheb_init, dim3(div_up((long)kPlanes * vs, 256)), dim3(256), 0, s, n, vs, X0);
   std::vector<double> theta, S;
   const double bound = 0.03 * (double)tol;  // sin of the largest per-vector move between iterations
   double cut = kFirstCut;
-  int deg = kDeg, total = 0, flat = 0;
+  int deg = chebyshev_first_degree, total = 0, flat = 0;
   double best = HUGE_VAL;
   for (int outer = 0; outer < kMaxOuter; outer++) {
     // scaled Chebyshev filter of degree deg damping [cut, kUpper], normalised at 0
@@ -900,7 +1027,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
 // of kB vectors.  Returns milliseconds per step.
 int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product,
                             int shape) {
-  if (shape == 0) shape = chebyshev_shape_for(w, h);
+  if (shape == 0) shape = chebyshev_shape_for(ctx->device, w, h);
   if (!chebyshev_shape_valid(shape)) return fail(DMSC_ERR_ARG, "unknown filter tile %d", shape);
   if (products < 1 || products > 100000) return fail(DMSC_ERR_ARG, "products out of range");
   hipStream_t s = ctx->stream;
@@ -946,7 +1073,8 @@ int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, i
 // matrix) and makes the fastest that size's tile for later solves.
 int autotune_chebyshev(Ctx* ctx, int w, int h, const float* A, int pitch, int products, int* best, int* shapes,
                        float* us, int max_tried, int* n_tried) {
-  static const int kShapes[] = {1328, 1324, 1336, 1337, 1344, 1348, 328, 324, 344, 342, 348, 24, 28, 22, 42};
+  static const int kShapes[] = {1328, 1324, 1336, 1337, 2328, 2324, 2337, 1344, 1348, 328, 324, 344, 342, 348,
+                                24, 28, 22, 42};
   int nt = 0, top = 0;
   float top_us = 0.f;
   for (int sh : kShapes) {

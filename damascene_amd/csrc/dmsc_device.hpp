@@ -74,6 +74,21 @@ __device__ __forceinline__ float tree32(T* t) {
   return t[0];
 }
 
+// tree32 for t[NZ..31] = +0 and t[0..NZ) >= +0: the same sums, without
+// the additions of a known +0 (exact for a non-negative left operand).
+template <int NZ, typename T>
+__device__ __forceinline__ float tree32z(T* t) {
+  int nz = NZ;
+#pragma unroll
+  for (int s = 16; s >= 1; s >>= 1) {
+#pragma unroll
+    for (int i = 0; i < s; i++)
+      if (i + s < nz) t[i] = t[i] + t[i + s];
+    nz = nz < s ? nz : s;
+  }
+  return t[0];
+}
+
 // Logical block of a 1-D grid, XCD-aware.  The device deals workgroups to
 // its 8 XCDs in turn (workgroup i to XCD i % 8), and each XCD has its own
 // L2; this bijection of [0, G) gives XCD x one contiguous range of logical

@@ -121,17 +121,19 @@ extern int lanczos_mixed_stop;  // mixed path: stop when every residual is <= to
 extern int lanczos_max_iter;  // iteration cap, at most kMaxIter (multiple of 25)
 extern int batch_lanes;
 // Mixed path eigensolver: 0 Lanczos, 1 Chebyshev-filtered block subspace
-// iteration (chebyshev.hip); its filter tile (rows per lane * 10 + float4
-// planes per lane: 24, 28, 22 or 42).
+// iteration (chebyshev.hip); its filter tile (chebyshev_shape_valid; 0 =
+// chosen per size by chebyshev_shape_for).
 extern int eigensolver_method;
 extern int chebyshev_shape;
 extern int chebyshev_reduction;  // tune key "chebyshev_reduction"
+extern int chebyshev_start;      // tune key "chebyshev_start"
+extern int chebyshev_first_degree;  // tune key "chebyshev_first_degree"
 bool chebyshev_shape_valid(int v);
 int chebyshev_max_vectors();  // most eigenvectors the block solver returns
 // shape 0: the tile chebyshev_shape_for(w, h) picks
 int time_chebyshev_products(Ctx* ctx, int w, int h, const float* A, int pitch, int products, float* ms_per_product,
                             int shape = 0);
-int chebyshev_shape_for(int w, int h);
+int chebyshev_shape_for(int device, int w, int h);  // the filter tile a solve at w x h uses
 int autotune_chebyshev(Ctx* ctx, int w, int h, const float* A, int pitch, int products, int* best, int* shapes,
                        float* us, int max_tried, int* n_tried);
 void chebyshev_autotune_clear();

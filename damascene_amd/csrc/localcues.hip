@@ -311,17 +311,23 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
                                             int j1, int j2, int j3, const BlurK& bk) {
   static_assert(!PAIR || (NB == 32 && KR == 0), "paired halves hold 32 unblurred bins");
   float a[32], b[32];
+  // two bins per 32-bit word: the corner sums in packed 16-bit arithmetic,
+  // each half modulo 2^16 on its own
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int j = 0; j < 32; j++) {
-    const int e = j >> 1, sh = (j & 1) * 16;
-    if (j < NB) {
-      const uint32_t s0 = (pk[i0][e] >> sh) - (pk[i1][e] >> sh) + (pk[i2][e] >> sh) - (pk[i3][e] >> sh);
-      const uint32_t s1 = (pk[j0][e] >> sh) - (pk[j1][e] >> sh) + (pk[j2][e] >> sh) - (pk[j3][e] >> sh);
-      a[j] = (float)(s0 & 0xffffu);
-      b[j] = (float)(s1 & 0xffffu);
+  for (int e = 0; e < 16; e++) {
+    if (2 * e < NB) {
+      const u16x2 s0 = __builtin_bit_cast(u16x2, pk[i0][e]) - __builtin_bit_cast(u16x2, pk[i1][e]) +
+                       __builtin_bit_cast(u16x2, pk[i2][e]) - __builtin_bit_cast(u16x2, pk[i3][e]);
+      const u16x2 s1 = __builtin_bit_cast(u16x2, pk[j0][e]) - __builtin_bit_cast(u16x2, pk[j1][e]) +
+                       __builtin_bit_cast(u16x2, pk[j2][e]) - __builtin_bit_cast(u16x2, pk[j3][e]);
+      a[2 * e] = (float)s0.x;
+      b[2 * e] = (float)s1.x;
+      a[2 * e + 1] = 2 * e + 1 < NB ? (float)s0.y : 0.f;
+      b[2 * e + 1] = 2 * e + 1 < NB ? (float)s1.y : 0.f;
     } else {
-      a[j] = 0.f;
-      b[j] = 0.f;
+      a[2 * e] = a[2 * e + 1] = 0.f;
+      b[2 * e] = b[2 * e + 1] = 0.f;
     }
   }
   if (KR > 0) {  // rotate.cu:352-370: zero-padded correlation over bins
@@ -331,13 +337,12 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
       float sa = 0.f, sb = 0.f;
 #pragma unroll
       for (int nk = 0; nk < 2 * KR + 1; nk++) {
+        // a padding term fma(0, k, s) leaves s unchanged (s >= +0, k
+        // finite), so it is not formed
         const int j = i - KR + nk;
         if (j >= 0 && j < NB) {
           sa = fmaf(a[j], bk.k[nk], sa);
           sb = fmaf(b[j], bk.k[nk], sb);
-        } else {
-          sa = fmaf(0.0f, bk.k[nk], sa);
-          sb = fmaf(0.0f, bk.k[nk], sb);
         }
       }
       ca[i] = sa;
@@ -349,29 +354,34 @@ __device__ __forceinline__ float chi2_split(const uint32_t (&pk)[8][16], int i0,
       b[i] = cb[i];
     }
   }
+  // slots NB..31 hold +0 and every other slot is >= +0, so the tree sums
+  // skip the additions of those zeros (x + 0 = x exactly) and the chi terms
+  // of the empty slots are not formed
   float t32[32];
 #pragma unroll
   for (int i = 0; i < 32; i++) t32[i] = a[i];
-  float sa = dev::tree32(t32);
+  float sa = dev::tree32z<NB>(t32);
   if (PAIR) sa = sa + __shfl_xor(sa, 1, 64);
   const float na = (sa != 0.f) ? 8.0f / sa : 0.f;
 #pragma unroll
   for (int i = 0; i < 32; i++) t32[i] = b[i];
-  float sb = dev::tree32(t32);
+  float sb = dev::tree32z<NB>(t32);
   if (PAIR) sb = sb + __shfl_xor(sb, 1, 64);
   const float nb = (sb != 0.f) ? 8.0f / sb : 0.f;
 #pragma unroll
   for (int i = 0; i < 32; i++) {
-    const float ai = a[i] * na, bi = b[i] * nb;
-    const float s = ai + bi;
     float chi = 0.f;
-    if (s != 0.f) {
-      const float d = ai - bi;
-      chi = d * d / s;
+    if (i < NB) {
+      const float ai = a[i] * na, bi = b[i] * nb;
+      const float s = ai + bi;
+      if (s != 0.f) {
+        const float d = ai - bi;
+        chi = d * d / s;
+      }
     }
     t32[i] = chi;
   }
-  float c = dev::tree32(t32);
+  float c = dev::tree32z<NB>(t32);
   if (PAIR) c = c + __shfl_xor(c, 1, 64);
   return 0.5f * c;
 }

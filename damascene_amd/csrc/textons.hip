@@ -9,9 +9,10 @@
 // loads).
 //
 // k-means: per iteration one accumulation kernel (int32 wrap-around centroid
-// sums via LDS atomics, exact and order independent), one centroid kernel
-// and one assignment kernel (centroids staged per block, 32 or 64 distances
-// per pixel in the reference order, argmin with strict '<').  A device-side done flag lets
+// sums via LDS atomics, exact and order independent) and one assignment
+// kernel (every block forms the centroids from the sums in LDS while its
+// pixels' responses load, then 32 or 64 distances per pixel in the
+// reference order, argmin with strict '<').  A device-side done flag lets
 // all iterations be enqueued without host round trips.
 #include "dmsc_device.hpp"
 #include "dmsc_internal.hpp"
@@ -81,10 +82,6 @@ struct KmState {
   unsigned int done;        // set when an iteration changed no label
   unsigned int iterations;  // iterations executed
   unsigned int rechecks;    // mixed path: pixels re-labelled by the exact FP32 distances
-  float cent[kNFilters][kMaxK];  // centroids of the current iteration (FP32), [filter][cluster]
-  float cdot[kMaxK];             // |c|^2 in the reference order
-  alignas(16) uint16_t centb[kMaxK][32];   // mixed path: BF16 centroids, filters 0..31, [cluster][filter]
-  alignas(16) uint16_t centbl[kMaxK][32];  // mixed path: BF16 of (centroid - its BF16), the low part
 };
 
 // kmeans.cu:14-35 assignInitialClusters (6x6 grid capped at 31) for K = 32,
@@ -118,7 +115,8 @@ __global__ void k_km_init(int w, int h, int nf, const float* __restrict__ resp, 
   pdot[p] = d;
 }
 
-// One k-means iteration is three kernels: accumulate, centroids, assign.
+// One k-means iteration is two kernels: accumulate, then assign (which
+// forms the centroids from the sums in every block).
 // The stop test (kmeans.cu:712: no label changed) is taken at the start of
 // the next iteration's accumulation, from the change count the previous
 // assignment left; every block reads the same count, so all agree.
@@ -142,8 +140,17 @@ __global__ __launch_bounds__(256) void k_km_accum(int n, int nf, const float* __
   __shared__ uint32_t scount[K];
   for (int i = threadIdx.x; i < nf * K; i += 256) smass[i] = 0;
   if (threadIdx.x < K) scount[threadIdx.x] = 0;
-  __syncthreads();
   const int buf = it & 1;
+  if (blockIdx.x == 0) {
+    // the next iteration's sums (last read by the previous assignment) and
+    // this iteration's change count (last read by this kernel's stop test,
+    // for iteration it - 2 ... it - 1 of the other parity): cleared here,
+    // before the assignment below adds to it
+    for (int i = threadIdx.x; i < kNFilters * K; i += 256) st->mass[buf ^ 1][i] = 0;
+    if (threadIdx.x < K) st->count[buf ^ 1][threadIdx.x] = 0;
+    if (threadIdx.x == 0) st->changes[buf] = 0;
+  }
+  __syncthreads();
   for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
     const int c = labels[p];
     // all 34 responses are loaded before the first atomic, so their loads
@@ -175,47 +182,33 @@ __device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) 
   lo = to_bf16(x - from_bf16(hi));
 }
 
-// kmeans.cu:268-286 finishCentroids(_64) (an empty cluster gives NaN, which
-// never wins a comparison) and makeSelfDots for the centroids; one block.
-// Also clears the buffers the next iteration accumulates into.
+// Centroids and their norms, computed in LDS by every assignment block.
 template <int K>
-__global__ __launch_bounds__(256) void k_km_centroids(int nf, KmState* st, int it) {
-  if (st->done) return;
-  const int buf = it & 1;
+struct KmCent {
+  float cent[kNFilters][K];
+  float cdot[K];
+  float cnorm[K];  // |c| (mixed path)
+};
+
+// kmeans.cu:268-286 finishCentroids(_64) (an empty cluster gives NaN, which
+// never wins a comparison) and makeSelfDots for the centroids, from the
+// iteration's int32 sums: every block of the assignment computes the same
+// values with the same operations, so they agree bit for bit (there is no
+// separate centroid kernel between accumulation and assignment).
+template <int K>
+__device__ __forceinline__ void km_centroids(KmCent<K>& sc, const KmState* st, int nf, int buf) {
   for (int i = threadIdx.x; i < kNFilters * K; i += 256) {
     const int f = i / K, c = i - f * K;
-    st->cent[f][c] = f < nf ? (float)(int)st->mass[buf][i] / (100000.0f * (float)st->count[buf][c]) : 0.0f;
+    sc.cent[f][c] = f < nf ? (float)(int)st->mass[buf][i] / (100000.0f * (float)st->count[buf][c]) : 0.0f;
   }
   __syncthreads();
   if (threadIdx.x < K) {
     const int c = threadIdx.x;
     float d = 0.0f;
-    for (int f = 0; f < nf; f++) d = fmaf(st->cent[f][c], st->cent[f][c], d);
-    st->cdot[c] = d;
+    for (int f = 0; f < nf; f++) d = fmaf(sc.cent[f][c], sc.cent[f][c], d);
+    sc.cdot[c] = d;
+    sc.cnorm[c] = sqrtf(d);
   }
-  for (int i = threadIdx.x; i < K * 32; i += 256) {
-    const int c = i / 32, f = i - c * 32;
-    split_bf16(st->cent[f][c], st->centb[c][f], st->centbl[c][f]);
-  }
-  for (int i = threadIdx.x; i < kNFilters * K; i += 256) st->mass[buf ^ 1][i] = 0;
-  if (threadIdx.x < K) st->count[buf ^ 1][threadIdx.x] = 0;
-  if (threadIdx.x == 0) st->changes[buf] = 0;
-}
-
-// Centroids and their norms staged in LDS by every assignment block.
-template <int K>
-struct KmCent {
-  float cent[kNFilters][K];
-  float cdot[K];
-};
-
-template <int K>
-__device__ __forceinline__ void km_stage(KmCent<K>& sc, const KmState* st) {
-  for (int i = threadIdx.x; i < kNFilters * K; i += 256) {
-    const int f = i / K, c = i - f * K;
-    sc.cent[f][c] = st->cent[f][c];
-  }
-  if (threadIdx.x < K) sc.cdot[threadIdx.x] = st->cdot[threadIdx.x];
 }
 
 // Exact FP32 label of one pixel: kmeans.cu:296-338 makeDots + SGEMM(alpha
@@ -239,23 +232,27 @@ __device__ __forceinline__ int km_exact_label(const float (&x)[kNFilters], float
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_km_assign(int n, const float* __restrict__ resp,
+__global__ __launch_bounds__(256) void k_km_assign(int n, int nf, const float* __restrict__ resp,
                                                    const float* __restrict__ pdot, int* __restrict__ labels,
                                                    KmState* st, int it) {
   if (st->done) return;
   __shared__ KmCent<K> sc;
   __shared__ int schanges;
-  km_stage<K>(sc, st);
-  if (threadIdx.x == 0) schanges = 0;
-  __syncthreads();
   const int p = blockIdx.x * 256 + threadIdx.x;
+  const int pp = p < n ? p : n - 1;
+  // the pixel's responses are in flight while the block forms the centroids
+  float x[kNFilters];
+#pragma unroll
+  for (int f = 0; f < kNFilters; f++) x[f] = resp[(size_t)f * n + pp];
+  const float pd = pdot[pp];
+  const int old = labels[pp];
+  if (threadIdx.x == 0) schanges = 0;
+  km_centroids<K>(sc, st, nf, it & 1);
+  __syncthreads();
   int changed = 0;
   if (p < n) {
-    float x[kNFilters];
-#pragma unroll
-    for (int f = 0; f < kNFilters; f++) x[f] = resp[(size_t)f * n + p];
-    const int bl = km_exact_label<K>(x, pdot[p], sc);
-    changed = bl != labels[p];
+    const int bl = km_exact_label<K>(x, pd, sc);
+    changed = bl != old;
     labels[p] = bl;
   }
   const unsigned long long m = __ballot(changed);
@@ -284,39 +281,26 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // reference-order FP32 dot product at most 2^-19 sum |x_f c_f|, so with
 // sum |x_f c_f| <= |x||c| (Cauchy-Schwarz) and the factor 2 of the
 // distance below 2^-14.7 |x||c|; FP32 rounding of |x|^2 + |c|^2 - 2 x.c in
-// both computations stays inside the second term.  The label of the smallest approximate distance is the
-// exact label when its upper bound lies below every other cluster's lower
-// bound (no tie is possible then); any other pixel is re-labelled with the
-// exact FP32 distances in the reference order, so labels are bit-exact.
+// both computations stays inside the second term.  The label of the
+// smallest approximate distance is the exact label when its upper bound
+// lies below every other cluster's lower bound (no tie is possible then);
+// any other pixel is re-labelled with the exact FP32 distances in the
+// reference order, so labels are bit-exact.
 // Lane layout: A row l%16, B column l%16, K slice 8*(l/16)..+7; result rows
 // 4*(l/16)+i of column l%16.
 template <int K>
-__global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __restrict__ resp,
+__global__ __launch_bounds__(256) void k_km_assign_mfma(int n, int nf, const float* __restrict__ resp,
                                                         const float* __restrict__ pdot, int* __restrict__ labels,
                                                         KmState* st, int it) {
   constexpr int NT = K / 16;  // cluster tiles
   if (st->done) return;
   __shared__ KmCent<K> sc;
-  __shared__ float cnorm[K];  // |c|
+  __shared__ __attribute__((aligned(16))) uint16_t scb[K][32], scbl[K][32];  // BF16 split of filters 0..31
   __shared__ int schanges, srecheck;
-  km_stage<K>(sc, st);
-  if (threadIdx.x < K) cnorm[threadIdx.x] = sqrtf(st->cdot[threadIdx.x]);
-  if (threadIdx.x == 0) schanges = srecheck = 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  // A operands: centroid rows (clusters) 16*t + col, filters 8*grp..+7
-  bf16x8 a[NT], al[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const uint4 v = *reinterpret_cast<const uint4*>(&st->centb[16 * t + col][8 * grp]);
-    __builtin_memcpy(&a[t], &v, 16);
-    const uint4 vl = *reinterpret_cast<const uint4*>(&st->centbl[16 * t + col][8 * grp]);
-    __builtin_memcpy(&al[t], &vl, 16);
-  }
-  int changed = 0, rech = 0;
   // each wave labels 64 pixels, 16 per MFMA column tile; every tile's
-  // operands are loaded before the first tile is computed
+  // operands are in flight while the block forms the centroids
   float xv[4][8];
   float pdv[4], x32v[4];
   int oldl[4];  // the labels being replaced (for the change count)
@@ -330,6 +314,23 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
     x32v[tile] = resp[(size_t)32 * n + pp];
     oldl[tile] = labels[pp];
   }
+  if (threadIdx.x == 0) schanges = srecheck = 0;
+  km_centroids<K>(sc, st, nf, it & 1);
+  for (int i = threadIdx.x; i < K * 32; i += 256) {
+    const int c = i / 32, f = i - c * 32;
+    split_bf16(sc.cent[f][c], scb[c][f], scbl[c][f]);
+  }
+  __syncthreads();
+  // A operands: centroid rows (clusters) 16*t + col, filters 8*grp..+7
+  bf16x8 a[NT], al[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const uint4 v = *reinterpret_cast<const uint4*>(&scb[16 * t + col][8 * grp]);
+    __builtin_memcpy(&a[t], &v, 16);
+    const uint4 vl = *reinterpret_cast<const uint4*>(&scbl[16 * t + col][8 * grp]);
+    __builtin_memcpy(&al[t], &vl, 16);
+  }
+  int changed = 0, rech = 0;
 #pragma unroll
   for (int tile = 0; tile < 4; tile++) {
     const int p = (blockIdx.x * 4 + wave) * 64 + tile * 16 + col;
@@ -375,14 +376,14 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, const float* __re
 #pragma unroll
     for (int i = 0; i < 4 * NT; i++)
       if (ci[i] != bc) {
-        const float e = 6.103515625e-05f * xn * cnorm[ci[i]] + 9.5367431640625e-7f * (pd + sc.cdot[ci[i]]);
+        const float e = 6.103515625e-05f * xn * sc.cnorm[ci[i]] + 9.5367431640625e-7f * (pd + sc.cdot[ci[i]]);
         lo_other = fminf(lo_other, dt[i] - e);
       }
 #pragma unroll
     for (int o = 16; o <= 32; o <<= 1) lo_other = fminf(lo_other, __shfl_xor(lo_other, o, 64));
     bool certain = false;
     if (bc < K) {
-      const float e = 6.103515625e-05f * xn * cnorm[bc] + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
+      const float e = 6.103515625e-05f * xn * sc.cnorm[bc] + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
       certain = best + e < lo_other && best + e < 10000000.0f;
     }
     if (grp == 0 && live) {
@@ -430,11 +431,11 @@ void km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels,
   const int blocks_acc = std::min(div_up(n, 256), 1024);
   for (int it = 0; it < max_iter; it++) {
     hipLaunchKernelGGL(k_km_accum<K>, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
-    hipLaunchKernelGGL(k_km_centroids<K>, dim3(1), dim3(256), 0, s, nf, st, it);
-    if (mixed)
-      hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
-    else
-      hipLaunchKernelGGL(k_km_assign<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, resp, pdot, labels, st, it);
+    if (mixed) {
+      hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, nf, resp, pdot, labels, st,
+                         it);
+    } else
+      hipLaunchKernelGGL(k_km_assign<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, nf, resp, pdot, labels, st, it);
   }
 }
 

@@ -128,7 +128,7 @@ int dmsc_kmeans_rechecks(dmsc_ctx* ctx, int* rechecks);
  * dmsc_kmeans; its scratch (the k-means state and |x|^2 per pixel) lives in
  * a fixed device array of DMSC_KMEANS_NOALLOC_SCRATCH_BYTES per GPU, held
  * from library load to process exit, so the call allocates nothing.  Inputs
- * of more than dmsc_kmeans_noalloc_max_pixels() pixels (2,088,320: a
+ * of more than dmsc_kmeans_noalloc_max_pixels() pixels (2,092,608: a
  * 1920x1080 image fits) return DMSC_ERR_ARG.  With changes and iterations
  * both NULL the call only enqueues work on the context's stream (no
  * read-back, no synchronisation).  Every call on one GPU shares that GPU's
@@ -292,13 +292,19 @@ int dmsc_lanczos_time_steps(dmsc_ctx* ctx, int w, int h, float* d_matrix, int ma
                             int steps, float* ms_per_step);
 /* No reference counterpart: times `products` filter steps of the block
  * eigensolver (eigensolver = 1) on a converted matrix (L_sym, as
- * dmsc_convert_matrix leaves it) with the current chebyshev_shape tile, a
+ * dmsc_convert_matrix leaves it) with the tile dmsc_chebyshev_tile names, a
  * block of 32 vectors, after 5 untimed steps, with HIP events on the
  * context's stream; returns the average milliseconds per step. */
 int dmsc_chebyshev_time_products(dmsc_ctx* ctx, int w, int h, const float* d_matrix, int matrix_pitch,
                                  int products, float* ms_per_product);
+/* No reference counterpart: the filter tile (a chebyshev_shape value) that a
+ * block solve at w x h uses on this context's device: the autotuner's pick
+ * for that size, else the tuning key, else (key 0, the default) 2337 where
+ * its grid fits in one round of the device's block slots and 2328's does
+ * not, else 2328. */
+int dmsc_chebyshev_tile(dmsc_ctx* ctx, int w, int h, int* shape);
 /* No reference counterpart: times every filter tile of the block eigensolver
- * (the 13 chebyshev_shape values) at w x h on a converted matrix, `products`
+ * (the 18 chebyshev_shape values) at w x h on a converted matrix, `products`
  * steps each, and makes the fastest the tile of later block solves at that
  * size (tiles give bit-identical results).  shapes_tried/us_per_product
  * receive up to max_tried (tile, microseconds per step) pairs;

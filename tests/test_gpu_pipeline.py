@@ -269,7 +269,7 @@ def test_block_solver_large_image(ctx, w, h, seed):
 
 
 @pytest.mark.parametrize("w,h,seed", [(160, 120, 20261202), (61, 37, 7)])
-@pytest.mark.parametrize("shape", [22, 28, 42, 324, 344, 342, 328, 348, 1324, 1328, 1336, 1337, 1344, 1348])
+@pytest.mark.parametrize("shape", [22, 28, 42, 324, 344, 342, 328, 348, 1324, 1328, 1336, 1337, 1344, 1348, 2324, 2328, 2337])
 def test_block_solver_shapes(ctx, w, h, seed, shape):
     """Every filter-kernel tile of the block solver ("chebyshev_shape": rows
     per lane x 10 + float4 planes per lane) gives the default tile's (24)

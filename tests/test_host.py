@@ -363,14 +363,17 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     assert L.dmsc_kmeans_rechecks(None, None) == 1
     assert L.dmsc_chebyshev_time_products(None, 64, 64, None, 4096, 10, None) == 1
     assert L.dmsc_autotune_chebyshev(None, 64, 64, None, 4096, 10, None, None, None, 0, None) == 1
+    assert L.dmsc_chebyshev_tile(None, 64, 64, None) == 1
     for key, bad in ((b"lanczos_tile", 6405), (b"lanczos_max_iter", 10), (b"lanczos_mixed_stop", 9),
                      (b"lanczos_check_max", 30), (b"batch_lanes", 0), (b"eigensolver", 2),
-                     (b"chebyshev_shape", 44), (b"chebyshev_shape", 346), (b"integral_pass", 2)):
+                     (b"chebyshev_shape", 44), (b"chebyshev_shape", 346), (b"integral_pass", 2),
+                     (b"chebyshev_start", 2), (b"chebyshev_first_degree", 1), (b"chebyshev_first_degree", 301)):
         assert L.dmsc_tune_set(key, bad) == 1, key
     v = ctypes.c_int()
     for key, want in ((b"lanczos_tile", 3208), (b"lanczos_max_iter", 6000), (b"lanczos_mixed_stop", 4),
                       (b"lanczos_check_max", 0), (b"batch_lanes", 4), (b"eigensolver", 1),
-                      (b"chebyshev_shape", 1328), (b"integral_pass", 1), (b"chebyshev_reduction", 10)):
+                      (b"chebyshev_shape", 0), (b"integral_pass", 1), (b"chebyshev_reduction", 10),
+                      (b"chebyshev_start", 1), (b"chebyshev_first_degree", 50)):
         assert L.dmsc_tune_get(key, ctypes.byref(v)) == 0 and v.value == want, key
     assert L.dmsc_kmeans_noalloc(None, 64, 64, 32, 34, None, None, 15, 0, 0, None, None) == 1
 

@@ -662,66 +662,6 @@ __global__ __launch_bounds__(256) void k_cheb_rotate(int n, size_t vs, const flo
   }
 }
 
-// ---------------------------------------------------------------- host FP64
-// Cholesky G = L L' (lower, in place); false if G is not positive definite.
-bool cholesky(std::vector<double>& L, int m) {
-  for (int j = 0; j < m; j++) {
-    double d = L[j * m + j];
-    for (int k = 0; k < j; k++) d -= L[j * m + k] * L[j * m + k];
-    if (!(d > 0.0)) return false;
-    d = std::sqrt(d);
-    L[j * m + j] = d;
-    for (int i = j + 1; i < m; i++) {
-      double s = L[i * m + j];
-      for (int k = 0; k < j; k++) s -= L[i * m + k] * L[j * m + k];
-      L[i * m + j] = s / d;
-    }
-    for (int k = j + 1; k < m; k++) L[j * m + k] = 0.0;
-  }
-  return true;
-}
-
-// H s = theta G s: S (row-major kB x kB, column j = s_j, ascending theta)
-// with S' G S = I.  False if G is not positive definite.
-bool rayleigh_ritz(const double* G, const double* H, std::vector<double>& theta, std::vector<double>& S) {
-  const int m = kB;
-  std::vector<double> L(G, G + m * m);
-  for (int i = 0; i < m; i++)
-    for (int j = 0; j < i; j++) L[i * m + j] = L[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
-  if (!cholesky(L, m)) return false;
-  // M = L^-1 H L^-T, with H symmetrised
-  std::vector<double> T((size_t)m * m), M((size_t)m * m);
-  for (int c = 0; c < m; c++)  // T = L^-1 H (forward substitution per column)
-    for (int i = 0; i < m; i++) {
-      double s = 0.5 * (H[i * m + c] + H[c * m + i]);
-      for (int k = 0; k < i; k++) s -= L[i * m + k] * T[k * m + c];
-      T[i * m + c] = s / L[i * m + i];
-    }
-  for (int r = 0; r < m; r++)  // M' = L^-1 T' (M is symmetric)
-    for (int i = 0; i < m; i++) {
-      double s = T[r * m + i];
-      for (int k = 0; k < i; k++) s -= L[i * m + k] * M[r * m + k];
-      M[r * m + i] = s / L[i * m + i];
-    }
-  for (int i = 0; i < m; i++)
-    for (int j = 0; j < i; j++) M[i * m + j] = M[j * m + i] = 0.5 * (M[i * m + j] + M[j * m + i]);
-  std::vector<double> ev(m), Z((size_t)m * m);
-  if (!sym_eigen(M.data(), m, ev.data(), Z.data())) return false;
-  std::vector<int> ord(m);
-  for (int i = 0; i < m; i++) ord[i] = i;
-  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
-  theta.resize(m);
-  S.assign((size_t)m * m, 0.0);
-  for (int j = 0; j < m; j++) {
-    theta[j] = ev[ord[j]];
-    for (int i = m - 1; i >= 0; i--) {  // s_j = L^-T z_j (back substitution)
-      double s = Z[i * m + ord[j]];
-      for (int k = i + 1; k < m; k++) s -= L[k * m + i] * S[k * m + j];
-      S[i * m + j] = s / L[i * m + i];
-    }
-  }
-  return true;
-}
 
 template <int R, int VP, bool F64 = false>
 int launch_step(hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x,
@@ -941,7 +881,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     // block and the residuals of the previous Ritz pairs (enqueued last
     // iteration, read here)
     DMSC_HIP(hipStreamSynchronize(s));
-    if (!rayleigh_ritz(h_gram, h_gram + kB * kB, theta, S))
+    if (!rayleigh_ritz(h_gram, h_gram + kB * kB, kB, theta, S))
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: filtered block lost rank at outer iteration %d", outer);
     double resid_prev = HUGE_VAL;  // the previous outer iteration's Ritz pairs
     if (outer > 0) {

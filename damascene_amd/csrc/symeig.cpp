@@ -1,12 +1,13 @@
-// symeig.cpp -- dense symmetric eigendecomposition on the host (FP64), for
-// the block eigensolver's Rayleigh-Ritz step (a 32 x 32 problem per outer
-// iteration).
+// symeig.cpp -- the block eigensolver's Rayleigh-Ritz step on the host
+// (FP64): Cholesky of the Gram matrix and a dense symmetric
+// eigendecomposition (a 32 x 32 problem per outer iteration).
 //
 // Householder reduction to tridiagonal form T = Q' A Q, then implicit
 // symmetric QR steps with Wilkinson shifts on T, the rotations accumulated
 // into Q (Golub and Van Loan, Matrix Computations, 4th ed., Algorithms 5.1.1,
 // 8.3.1 and 8.3.3).  About 9 n^3 flops in all, against roughly 6 n^3 per
 // sweep for cyclic Jacobi.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -36,7 +37,12 @@ void house(const double* x, int m, double* v, double* beta) {
 
 }  // namespace
 
-bool sym_eigen(const double* A_in, int n, double* ev, double* Z) {
+// Built at -O3 with floating-point contraction, in two clones picked at load
+// time by the CPU: AVX2 + FMA (every host this library runs on) and
+// baseline x86-64.  A host always runs the same clone, so its results are
+// reproducible there.
+__attribute__((target_clones("arch=haswell", "default"))) bool sym_eigen(const double* A_in, int n, double* ev,
+                                                                         double* Z) {
   if (n <= 0) return false;
   // work on A / 2^ceil(log2 max|a_ij|) (undone on the eigenvalues), so the squares in the
   // Givens rotations neither overflow nor underflow
@@ -135,6 +141,72 @@ bool sym_eigen(const double* A_in, int n, double* ev, double* Z) {
     }
   }
   return false;
+}
+
+
+namespace {
+
+// Cholesky G = L L' (lower, in place); false if G is not positive definite.
+bool cholesky(std::vector<double>& L, int m) {
+  for (int j = 0; j < m; j++) {
+    double d = L[j * m + j];
+    for (int k = 0; k < j; k++) d -= L[j * m + k] * L[j * m + k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    L[j * m + j] = d;
+    for (int i = j + 1; i < m; i++) {
+      double s = L[i * m + j];
+      for (int k = 0; k < j; k++) s -= L[i * m + k] * L[j * m + k];
+      L[i * m + j] = s / d;
+    }
+    for (int k = j + 1; k < m; k++) L[j * m + k] = 0.0;
+  }
+  return true;
+}
+
+}  // namespace
+
+// H s = theta G s: S (row-major m x m, column j = s_j, ascending theta)
+// with S' G S = I.  False if G is not positive definite.
+__attribute__((target_clones("arch=haswell", "default"))) bool rayleigh_ritz(const double* G, const double* H, int m,
+                                                                             std::vector<double>& theta,
+                                                                             std::vector<double>& S) {
+  std::vector<double> L(G, G + m * m);
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < i; j++) L[i * m + j] = L[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
+  if (!cholesky(L, m)) return false;
+  // M = L^-1 H L^-T, with H symmetrised
+  std::vector<double> T((size_t)m * m), M((size_t)m * m);
+  for (int c = 0; c < m; c++)  // T = L^-1 H (forward substitution per column)
+    for (int i = 0; i < m; i++) {
+      double s = 0.5 * (H[i * m + c] + H[c * m + i]);
+      for (int k = 0; k < i; k++) s -= L[i * m + k] * T[k * m + c];
+      T[i * m + c] = s / L[i * m + i];
+    }
+  for (int r = 0; r < m; r++)  // M' = L^-1 T' (M is symmetric)
+    for (int i = 0; i < m; i++) {
+      double s = T[r * m + i];
+      for (int k = 0; k < i; k++) s -= L[i * m + k] * M[r * m + k];
+      M[r * m + i] = s / L[i * m + i];
+    }
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < i; j++) M[i * m + j] = M[j * m + i] = 0.5 * (M[i * m + j] + M[j * m + i]);
+  std::vector<double> ev(m), Z((size_t)m * m);
+  if (!sym_eigen(M.data(), m, ev.data(), Z.data())) return false;
+  std::vector<int> ord(m);
+  for (int i = 0; i < m; i++) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  theta.resize(m);
+  S.assign((size_t)m * m, 0.0);
+  for (int j = 0; j < m; j++) {
+    theta[j] = ev[ord[j]];
+    for (int i = m - 1; i >= 0; i--) {  // s_j = L^-T z_j (back substitution)
+      double s = Z[i * m + ord[j]];
+      for (int k = i + 1; k < m; k++) s -= L[k * m + i] * S[k * m + j];
+      S[i * m + j] = s / L[i * m + i];
+    }
+  }
+  return true;
 }
 
 }  // namespace dmsc

@@ -43,6 +43,10 @@ constexpr int kMinSide = 32;      // mirror border 30 needs >= 30 px per side
 // symmetric n x n FP64 matrix: Householder tridiagonalisation + implicit
 // symmetric QR (symeig.cpp).  False if the iteration did not converge.
 bool sym_eigen(const double* A, int n, double* ev, double* Z);
+// The m x m generalized symmetric eigenproblem H s = theta G s of the block
+// eigensolver's Rayleigh-Ritz step (symeig.cpp); false if G is not
+// positive definite or the QR iteration does not converge.
+bool rayleigh_ritz(const double* G, const double* H, int m, std::vector<double>& theta, std::vector<double>& S);
 inline int pitch_floats(long n) { return (int)(((n + 63) / 64) * 64); }
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 

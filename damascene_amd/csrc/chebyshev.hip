@@ -407,13 +407,26 @@ __global__ __launch_bounds__(64 * NW) void k_cheb_lds(int w, int h, int ntx, con
 }
 
 // k_cheb_lds (FP32 sums, pipelined matrix loads) with the LDS rows read
-// three rows ahead of their FMAs: bit-identical results.
+// ahead of their FMAs, four rows for R = 2 and three for R = 3 (the depths
+// a sweep of 2 to 6 rows found fastest; deeper costs occupancy at R = 3):
+// bit-identical results.
 template <int R, int NW>
 __global__ __launch_bounds__(64 * NW) void k_cheb_ldp(int w, int h, int ntx, const float* __restrict__ A,
                                                       size_t pitch, const float4* __restrict__ x,
                                                       const float4* __restrict__ z, float4* __restrict__ y,
                                                       size_t vs, float alpha, float c, float beta) {
-  cheb_lds_body<R, NW, false, true, 3>(w, h, ntx, A, pitch, x, z, y, vs, alpha, c, beta);
+  cheb_lds_body<R, NW, false, true, R == 2 ? 4 : 3>(w, h, ntx, A, pitch, x, z, y, vs, alpha, c, beta);
+}
+
+// The FP64-sum product W = AY (alpha 1, c and beta 0) of the read-ahead
+// tiles, with pipelined matrix loads: bit-identical to k_cheb_lds<R, NW,
+// true>.
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_cheb_ldp64(int w, int h, int ntx, const float* __restrict__ A,
+                                                        size_t pitch, const float4* __restrict__ x,
+                                                        const float4* __restrict__ z, float4* __restrict__ y,
+                                                        size_t vs, float alpha, float c, float beta) {
+  cheb_lds_body<R, NW, true, true, 3>(w, h, ntx, A, pitch, x, z, y, vs, alpha, c, beta);
 }
 
 // Uniform value in [-1, 1) from an integer hash of element index i.
@@ -476,67 +489,82 @@ CosModes cos_modes(int w, int h) {
   return m;
 }
 
-// Partial Gram matrices over one chunk of pixels, FP64: blockIdx.y = 0
-// G = Y'Y, 1 H = Y'W, 2 C = X0'Y (entry [i][j] = U_i . V_j).  Thread t
-// owns row i = t / 8 and columns 4 (t % 8) .. +3.
+// Partial Gram matrices over one chunk of pixels, FP64: G = Y'Y, H = Y'W and
+// C = X0'Y (entry [i][j] = U_i . V_j), one block per chunk.  The chunk's
+// Y, W and X0 are staged in LDS once per round of kGramPx pixels and feed
+// all three products.  Thread t owns row i = t / 8 and columns 4 (t % 8) ..
+// +3 of each; every entry sums its pixels in order with one fused
+// multiply-add each.  (Splitting a round's pixels over two threads per
+// entry runs 10 % faster but changes the sums' rounding, and with it the
+// iteration at which the stop test passes on some images.)
 __global__ __launch_bounds__(256) void k_cheb_gram(int n, size_t vs, int chunk, const float4* __restrict__ Y,
                                                    const float4* __restrict__ Wb, const float4* __restrict__ X0,
                                                    double* __restrict__ part) {
-  __shared__ float su[kGramPx][kB + 1];
-  __shared__ float4 sv[kGramPx][kPlanes];
-  const int which = blockIdx.y;
-  const float4* U = which == 2 ? X0 : Y;
-  const float4* V = which == 1 ? Wb : Y;
+  __shared__ float4 sy[kGramPx][kPlanes], sw[kGramPx][kPlanes], sx[kGramPx][kPlanes];
   const int i = threadIdx.x >> 3, j4 = threadIdx.x & 7;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double g[4] = {0.0, 0.0, 0.0, 0.0}, hh[4] = {0.0, 0.0, 0.0, 0.0}, cc[4] = {0.0, 0.0, 0.0, 0.0};
   const int p0 = blockIdx.x * chunk, p1 = min(n, p0 + chunk);
   for (int base = p0; base < p1; base += kGramPx) {
     for (int s = threadIdx.x; s < kGramPx * kPlanes; s += 256) {
       const int px = s % kGramPx, pl = s / kGramPx, p = base + px;
       const bool in = p < p1;
-      const float4 u = in ? U[(size_t)pl * vs + p] : make_float4(0.f, 0.f, 0.f, 0.f);
-      su[px][pl * 4 + 0] = u.x;
-      su[px][pl * 4 + 1] = u.y;
-      su[px][pl * 4 + 2] = u.z;
-      su[px][pl * 4 + 3] = u.w;
-      sv[px][pl] = in ? V[(size_t)pl * vs + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const size_t at = (size_t)pl * vs + p;
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      sy[px][pl] = in ? Y[at] : zero;
+      sw[px][pl] = in ? Wb[at] : zero;
+      sx[px][pl] = in ? X0[at] : zero;
     }
     __syncthreads();
-#pragma unroll 8
+    const float* fy = &sy[0][0].x;
+    const float* fx = &sx[0][0].x;
+#pragma unroll 4
     for (int px = 0; px < kGramPx; px++) {
-      const double a = su[px][i];
-      const float4 b = sv[px][j4];
-      acc[0] = fma(a, (double)b.x, acc[0]);
-      acc[1] = fma(a, (double)b.y, acc[1]);
-      acc[2] = fma(a, (double)b.z, acc[2]);
-      acc[3] = fma(a, (double)b.w, acc[3]);
+      const double yi = fy[px * kB + i], xi = fx[px * kB + i];
+      const float4 yv = sy[px][j4], wv = sw[px][j4];
+      g[0] = fma(yi, (double)yv.x, g[0]);
+      g[1] = fma(yi, (double)yv.y, g[1]);
+      g[2] = fma(yi, (double)yv.z, g[2]);
+      g[3] = fma(yi, (double)yv.w, g[3]);
+      hh[0] = fma(yi, (double)wv.x, hh[0]);
+      hh[1] = fma(yi, (double)wv.y, hh[1]);
+      hh[2] = fma(yi, (double)wv.z, hh[2]);
+      hh[3] = fma(yi, (double)wv.w, hh[3]);
+      cc[0] = fma(xi, (double)yv.x, cc[0]);
+      cc[1] = fma(xi, (double)yv.y, cc[1]);
+      cc[2] = fma(xi, (double)yv.z, cc[2]);
+      cc[3] = fma(xi, (double)yv.w, cc[3]);
     }
     __syncthreads();
   }
-  double* o = part + ((size_t)which * gridDim.x + blockIdx.x) * (kB * kB) + i * kB + j4 * 4;
+  const size_t at = (size_t)blockIdx.x * (kB * kB) + i * kB + j4 * 4, stride = (size_t)gridDim.x * (kB * kB);
 #pragma unroll
-  for (int k = 0; k < 4; k++) o[k] = acc[k];
+  for (int k = 0; k < 4; k++) {
+    part[at + k] = g[k];
+    part[stride + at + k] = hh[k];
+    part[2 * stride + at + k] = cc[k];
+  }
 }
 
 // Residuals of the wanted Ritz pairs, formed directly: r_j = W s_j -
 // theta_j Y s_j per pixel in FP64 (S from the rotation buffer, row i =
-// input vector i), |r_j|^2 summed per block of 256 pixels into part[blk][j].
+// input vector i), |r_j|^2 summed per block of 256 pixels into part[blk][j];
+// the same pass writes the rotated block Y S (nout < 0: all kB vectors to
+// `planes`, the next filter input; nout > 0: vectors 0..nout-1 to out[j * n
+// + p], the solver's output), FP64 sums over the input vectors in order.
 // The Gram form s'K s - theta^2 cancels: W is stored in FP32, so K carries
 // a relative rounding of about 2^-23 and the difference keeps an error of
 // about theta * 2^-11.5, which at 61x37 (theta near 1e-2) exceeds the
 // residual itself; the direct form errs by about 2^-24 theta |y|.
 constexpr int kResidMax = kB / 2;
-__global__ __launch_bounds__(256) void k_cheb_resid(int n, size_t vs, const float4* __restrict__ Y,
-                                                    const float4* __restrict__ Wb, const double* __restrict__ S,
-                                                    const double* __restrict__ theta, int nvec,
-                                                    double* __restrict__ part) {
-  __shared__ double ss[kB * kResidMax];
+__global__ __launch_bounds__(256) void k_cheb_resid_rotate(int n, size_t vs, const float4* __restrict__ Y,
+                                                           const float4* __restrict__ Wb, const double* __restrict__ S,
+                                                           const double* __restrict__ theta, int nvec,
+                                                           double* __restrict__ part, int nout,
+                                                           float4* __restrict__ planes, float* __restrict__ out) {
+  __shared__ double ss[kB * kB];
   __shared__ double th[kResidMax];
   __shared__ double red[kResidMax][4];
-  for (int k = threadIdx.x; k < kB * kResidMax; k += 256) {
-    const int i = k / kResidMax, j = k - i * kResidMax;
-    ss[k] = j < nvec ? S[i * kB + j] : 0.0;
-  }
+  for (int k = threadIdx.x; k < kB * kB; k += 256) ss[k] = S[k];
   if (threadIdx.x < kResidMax) th[threadIdx.x] = threadIdx.x < nvec ? theta[threadIdx.x] : 0.0;
   __syncthreads();
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -544,36 +572,53 @@ __global__ __launch_bounds__(256) void k_cheb_resid(int n, size_t vs, const floa
 #pragma unroll
   for (int j = 0; j < kResidMax; j++) r2[j] = 0.0;
   if (p < n) {
-    double ys[kResidMax], ws[kResidMax];
+    float yv[kB], wv[kB];
 #pragma unroll
-    for (int j = 0; j < kResidMax; j++) ys[j] = ws[j] = 0.0;
-#pragma unroll 2
     for (int pl = 0; pl < kPlanes; pl++) {
-      const float4 yv = Y[(size_t)pl * vs + p], wv = Wb[(size_t)pl * vs + p];
-      const float yk[4] = {yv.x, yv.y, yv.z, yv.w}, wk[4] = {wv.x, wv.y, wv.z, wv.w};
+      const float4 a = Y[(size_t)pl * vs + p], b = Wb[(size_t)pl * vs + p];
+      yv[pl * 4 + 0] = a.x, yv[pl * 4 + 1] = a.y, yv[pl * 4 + 2] = a.z, yv[pl * 4 + 3] = a.w;
+      wv[pl * 4 + 0] = b.x, wv[pl * 4 + 1] = b.y, wv[pl * 4 + 2] = b.z, wv[pl * 4 + 3] = b.w;
+    }
+    // Y s_j for every j (the rotated block), W s_j for the wanted ones;
+    // each sum runs over the input vectors in order
+    double ys[kResidMax];
+    const int nj = nout < 0 ? kB : nout;
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const double* row = ss + (pl * 4 + k) * kResidMax;
+    for (int j0 = 0; j0 < kB; j0 += 4) {
+      if (j0 >= nj && j0 >= nvec) break;
+      double o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+      for (int i = 0; i < kB; i++) {
 #pragma unroll
-        for (int j = 0; j < kResidMax; j++) {
-          ys[j] = fma((double)yk[k], row[j], ys[j]);
-          ws[j] = fma((double)wk[k], row[j], ws[j]);
-        }
+        for (int k = 0; k < 4; k++) o[k] = fma((double)yv[i], ss[i * kB + j0 + k], o[k]);
+      }
+      if (j0 < kResidMax) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) ys[j0 + k] = o[k];
+      }
+      if (nout < 0) {
+        planes[(size_t)(j0 / 4) * vs + p] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+      } else {
+        for (int k = 0; k < 4 && j0 + k < nout; k++) out[(size_t)(j0 + k) * n + p] = (float)o[k];
       }
     }
 #pragma unroll
     for (int j = 0; j < kResidMax; j++) {
-      const double r = fma(-th[j], ys[j], ws[j]);
+      if (j >= nvec) break;
+      double ws = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < kB; i++) ws = fma((double)wv[i], ss[i * kB + j], ws);
+      const double r = fma(-th[j], ys[j], ws);
       r2[j] = r * r;
     }
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wvi = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < kResidMax; j++) {
     double v = r2[j];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[j][wv] = v;
+    if (lane == 0) red[j][wvi] = v;
   }
   __syncthreads();
   if (threadIdx.x < kResidMax)
@@ -581,7 +626,7 @@ __global__ __launch_bounds__(256) void k_cheb_resid(int n, size_t vs, const floa
         ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
 }
 
-// Residual norms from k_cheb_resid's partials: block j sums column j, each
+// Residual norms from k_cheb_resid_rotate's partials: block j sums column j, each
 // thread a strided subset, then the threads' sums in a fixed tree order.
 __global__ __launch_bounds__(256) void k_cheb_resid_sum(int nblk, const double* __restrict__ part,
                                                         double* __restrict__ out) {
@@ -626,43 +671,6 @@ __global__ __launch_bounds__(1024) void k_cheb_gram_sum(int nblk, const double* 
   }
 }
 
-// Rotation by S (kB x kB, FP64, row i = input vector i), FP64 sums: nout < 0
-// writes the whole block to `planes`; nout > 0 writes vectors 0..nout-1 to
-// out[j * n + p] (the solver's output layout).
-__global__ __launch_bounds__(256) void k_cheb_rotate(int n, size_t vs, const float4* __restrict__ Y,
-                                                     const double* __restrict__ S, int nout,
-                                                     float4* __restrict__ planes, float* __restrict__ out) {
-  __shared__ double ss[kB * kB];
-  for (int k = threadIdx.x; k < kB * kB; k += blockDim.x) ss[k] = S[k];
-  __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  float yv[kB];
-#pragma unroll
-  for (int pl = 0; pl < kPlanes; pl++) {
-    const float4 v = Y[(size_t)pl * vs + p];
-    yv[pl * 4 + 0] = v.x;
-    yv[pl * 4 + 1] = v.y;
-    yv[pl * 4 + 2] = v.z;
-    yv[pl * 4 + 3] = v.w;
-  }
-  const int nj = nout < 0 ? kB : nout;
-  for (int j0 = 0; j0 < nj; j0 += 4) {
-    double o[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-    for (int i = 0; i < kB; i++) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) o[k] = fma((double)yv[i], ss[i * kB + j0 + k], o[k]);
-    }
-    if (nout < 0) {
-      planes[(size_t)(j0 / 4) * vs + p] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
-    } else {
-      for (int k = 0; k < 4 && j0 + k < nout; k++) out[(size_t)(j0 + k) * n + p] = (float)o[k];
-    }
-  }
-}
-
-
 template <int R, int VP, bool F64 = false>
 int launch_step(hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x,
                 const float4* z, float4* y, size_t vs, double alpha, double c, double beta) {
@@ -693,6 +701,16 @@ int launch_ldp(hipStream_t s, int w, int h, const float* A, size_t pitch, const 
   const int ntx = div_up(w, 64), nty = div_up(h, R * NW);
   hipLaunchKernelGGL((k_cheb_ldp<R, NW>), dim3(ntx * nty * 4), dim3(64 * NW), 0, s, w, h, ntx, A, pitch, x, z, y, vs,
                      (float)alpha, (float)c, (float)beta);
+  return DMSC_OK;
+}
+
+template <int R, int NW>
+int launch_ldp64(hipStream_t s, int w, int h, const float* A, size_t pitch, const float4* x, float4* y, size_t vs) {
+  if ((size_t)kStencilArea * pitch * sizeof(float) >= ((size_t)1 << 31))  // one buffer descriptor's range
+    return launch_lds<R, NW, true, false>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  const int ntx = div_up(w, 64), nty = div_up(h, R * NW);
+  hipLaunchKernelGGL((k_cheb_ldp64<R, NW>), dim3(ntx * nty * 4), dim3(64 * NW), 0, s, w, h, ntx, A, pitch, x,
+                     nullptr, y, vs, 1.0f, 0.0f, 0.0f);
   return DMSC_OK;
 }
 
@@ -730,9 +748,10 @@ int product_f64(int shape, hipStream_t s, int w, int h, const float* A, size_t p
   if (shape == 1328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1344) return launch_lds<4, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
   if (shape == 1336) return launch_lds<3, 6, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
-  if (shape == 1337 || shape == 2337) return launch_lds<3, 7, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
-  if (shape == 2324) return launch_lds<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
-  if (shape == 2328) return launch_lds<2, 8, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 1337) return launch_lds<3, 7, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
+  if (shape == 2337) return launch_ldp64<3, 7>(s, w, h, A, pitch, x, y, vs);
+  if (shape == 2324) return launch_ldp64<2, 4>(s, w, h, A, pitch, x, y, vs);
+  if (shape == 2328) return launch_ldp64<2, 8>(s, w, h, A, pitch, x, y, vs);
   return launch_step<2, 4, true>(s, w, h, A, pitch, x, nullptr, y, vs, 1.0, 0.0, 0.0);
 }
 
@@ -811,7 +830,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
   const int chunk = std::max(kGramPx, div_up(div_up(n, 512), kGramPx) * kGramPx);
   const int nblk = div_up(n, chunk);
   const size_t blk = (size_t)kPlanes * vs * sizeof(float4);
-  const int nrb = div_up(n, 256);  // k_cheb_resid blocks
+  const int nrb = div_up(n, 256);  // k_cheb_resid_rotate blocks
   const size_t o_part = 4 * blk, o_gram = o_part + (size_t)kGrams * nblk * kB * kB * sizeof(double),
                o_rot = o_gram + (size_t)kGrams * kB * kB * sizeof(double),
                o_theta = o_rot + (size_t)kB * kB * sizeof(double),
@@ -874,7 +893,7 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     float4* Y = ring[cur];
     float4* Wb = ring[(cur + 1) % 3];
     DMSC_TRY(product_f64(shape, s, w, h, A, pitchz, Y, Wb, vs));
-    hipLaunchKernelGGL(k_cheb_gram, dim3(nblk, kGrams), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
+    hipLaunchKernelGGL(k_cheb_gram, dim3(nblk), dim3(256), 0, s, n, vs, chunk, Y, Wb, X0, part);
     hipLaunchKernelGGL(k_cheb_gram_sum, dim3(div_up(kGrams * kB * kB, 64)), dim3(1024), 0, s, nblk, part, gram);
     DMSC_HIP(hipMemcpyAsync(h_gram, gram, kGrams * kB * kB * sizeof(double), hipMemcpyDeviceToHost, s));
     // one host round trip per outer iteration: the Gram matrices of this
@@ -930,13 +949,14 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     DMSC_HIP(hipMemcpyAsync(rot, h_rot, sizeof(double) * kB * kB, hipMemcpyHostToDevice, s));
     DMSC_HIP(hipMemcpyAsync(d_theta, h_theta, sizeof(double) * nvec, hipMemcpyHostToDevice, s));
     // residual |A y_j - theta_j y_j| of each wanted Ritz vector y_j = Y s_j,
-    // formed per pixel on the device (k_cheb_resid); read back at the next
-    // round trip (or below, when the solve stops here)
-    hipLaunchKernelGGL(k_cheb_resid, dim3(nrb), dim3(256), 0, s, n, vs, Y, Wb, rot, d_theta, nvec, rpart);
+    // formed per pixel on the device by the pass that writes the rotated
+    // block (the next filter input, or the output when the solve stops
+    // here); read back at the next round trip (or below, at a stop)
+    hipLaunchKernelGGL(k_cheb_resid_rotate, dim3(nrb), dim3(256), 0, s, n, vs, Y, Wb, rot, d_theta, nvec, rpart,
+                       rule ? nvec : -1, rule ? nullptr : X0, rule ? d_out : nullptr);
     hipLaunchKernelGGL(k_cheb_resid_sum, dim3(nvec), dim3(256), 0, s, nrb, rpart, d_res);
     DMSC_HIP(hipMemcpyAsync(h_res, d_res, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
     if (rule) {
-      hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, nvec, nullptr, d_out);
       DMSC_HIP(hipGetLastError());
       DMSC_HIP(hipStreamSynchronize(s));
       double resid = 0.0;
@@ -947,7 +967,6 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       ctx->eig_residual = resid;
       return DMSC_OK;
     }
-    hipLaunchKernelGGL(k_cheb_rotate, dim3(div_up(n, 256)), dim3(256), 0, s, n, vs, Y, rot, -1, X0, nullptr);
     DMSC_HIP(hipGetLastError());
     total += deg + 1;
     cut = std::max(theta[kB - 1], 1e-6);

@@ -32,6 +32,7 @@
 // of pixel p is the linear index p + dy*w + dx, 0 outside [0, n)
 // (stencilMVM.cu:127-180), as in the Lanczos steps.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -900,8 +901,10 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
     // block and the residuals of the previous Ritz pairs (enqueued last
     // iteration, read here)
     DMSC_HIP(hipStreamSynchronize(s));
+    const auto t_rr = std::chrono::steady_clock::now();
     if (!rayleigh_ritz(h_gram, h_gram + kB * kB, kB, theta, S))
       return fail(DMSC_ERR_INTERNAL, "block eigensolver: filtered block lost rank at outer iteration %d", outer);
+    const double rr_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t_rr).count();
     double resid_prev = HUGE_VAL;  // the previous outer iteration's Ritz pairs
     if (outer > 0) {
       resid_prev = 0.0;
@@ -917,8 +920,10 @@ int chebyshev_solve(Ctx* ctx, int w, int h, const float* A, int pitch, int nvec,
       }
     }
     if (lanczos_trace)
-      std::fprintf(stderr, "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e previous residual %.3e\n",
-                   outer, deg, cut, theta[0], theta[nvec - 1], move, resid_prev);
+      std::fprintf(stderr,
+                   "[chebyshev] outer %d degree %d cut %.4e theta %.6e .. %.6e move %.3e previous residual %.3e "
+                   "(Rayleigh-Ritz on the host %.0f us)\n",
+                   outer, deg, cut, theta[0], theta[nvec - 1], move, resid_prev, rr_us);
     // done when the wanted vectors stopped moving, or (a degenerate
     // eigenvalue leaves its vectors free to turn) when every residual of the
     // previous Ritz pairs met the mixed Lanczos bound, tol *

@@ -35,6 +35,55 @@ void house(const double* x, int m, double* v, double* beta) {
   for (int i = 1; i < m; i++) v[i] /= v0;
 }
 
+// Implicit symmetric QR with Wilkinson shifts (Algorithm 8.3.3) on the
+// tridiagonal (d, e), |entries| = O(1): d becomes the eigenvalues and the
+// rotations are applied to T (n x n, row k = column k of the current Q).
+// The rotations of a sweep form a dependent chain; their updates of T are
+// vector work that runs beside it.  (Eigenvalues alone, then inverse
+// iteration for the vectors, measured slower at n = 32: the value-only
+// sweeps keep the same chain, and the solves add chains of their own.)
+// False if 40 n sweeps do not converge.
+bool tridiag_qr(double* d, double* e, int n, double* T) {
+  const double eps = 2.220446049250313e-16;
+  for (int iter = 0; iter < 40 * n; iter++) {
+    for (int i = 0; i + 1 < n; i++)
+      if (std::fabs(e[i]) <= eps * (std::fabs(d[i]) + std::fabs(d[i + 1]))) e[i] = 0.0;
+    int hi = n - 1;
+    while (hi > 0 && e[hi - 1] == 0.0) hi--;
+    if (hi == 0) return true;
+    int lo = hi - 1;
+    while (lo > 0 && e[lo - 1] != 0.0) lo--;
+    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer d[hi]
+    const double dd = 0.5 * (d[hi - 1] - d[hi]), bb = e[hi - 1];
+    const double den = dd + std::copysign(std::hypot(dd, bb), dd == 0.0 ? 1.0 : dd);
+    const double mu = d[hi] - bb * bb / den;
+    double x = d[lo] - mu, z = e[lo], bulge = 0.0;
+    for (int k = lo; k < hi; k++) {
+      const double r = std::sqrt(x * x + z * z);  // |x|, |z| = O(1) after the scaling
+      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : -z / r;
+      if (k > lo) e[k - 1] = c * e[k - 1] - s * bulge;
+      const double a = d[k], b = e[k], ap = d[k + 1];
+      d[k] = c * c * a - 2.0 * c * s * b + s * s * ap;
+      d[k + 1] = s * s * a + 2.0 * c * s * b + c * c * ap;
+      e[k] = c * s * (a - ap) + (c * c - s * s) * b;
+      if (k + 1 < hi) {
+        bulge = -s * e[k + 1];
+        e[k + 1] = c * e[k + 1];
+        x = e[k];
+        z = bulge;
+      }
+      double* tk = &T[(size_t)k * n];
+      double* tk1 = &T[(size_t)(k + 1) * n];
+      for (int col = 0; col < n; col++) {
+        const double zk = tk[col], zk1 = tk1[col];
+        tk[col] = c * zk - s * zk1;
+        tk1[col] = s * zk + c * zk1;
+      }
+    }
+  }
+  return false;
+}
+
 }  // namespace
 
 // Built at -O3 with floating-point contraction, in two clones picked at load
@@ -97,50 +146,12 @@ __attribute__((target_clones("arch=haswell", "default"))) bool sym_eigen(const d
   std::vector<double> d(n), e(n, 0.0);
   for (int i = 0; i < n; i++) d[i] = A[i * n + i];
   for (int i = 0; i + 1 < n; i++) e[i] = A[(i + 1) * n + i];
-  // 2. implicit symmetric QR with Wilkinson shifts on (d, e)
-  const double eps = 2.220446049250313e-16;
-  for (int iter = 0; iter < 40 * n; iter++) {
-    for (int i = 0; i + 1 < n; i++)
-      if (std::fabs(e[i]) <= eps * (std::fabs(d[i]) + std::fabs(d[i + 1]))) e[i] = 0.0;
-    int hi = n - 1;
-    while (hi > 0 && e[hi - 1] == 0.0) hi--;
-    if (hi == 0) {
-      for (int i = 0; i < n; i++) ev[i] = d[i] * scale;
-      for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) Z[i * n + j] = T[(size_t)j * n + i];
-      return true;
-    }
-    int lo = hi - 1;
-    while (lo > 0 && e[lo - 1] != 0.0) lo--;
-    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer d[hi]
-    const double dd = 0.5 * (d[hi - 1] - d[hi]), bb = e[hi - 1];
-    const double den = dd + std::copysign(std::hypot(dd, bb), dd == 0.0 ? 1.0 : dd);
-    const double mu = d[hi] - bb * bb / den;
-    double x = d[lo] - mu, z = e[lo], bulge = 0.0;
-    for (int k = lo; k < hi; k++) {
-      const double r = std::sqrt(x * x + z * z);  // |x|, |z| = O(1) after the scaling
-      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : -z / r;
-      if (k > lo) e[k - 1] = c * e[k - 1] - s * bulge;
-      const double a = d[k], b = e[k], ap = d[k + 1];
-      d[k] = c * c * a - 2.0 * c * s * b + s * s * ap;
-      d[k + 1] = s * s * a + 2.0 * c * s * b + c * c * ap;
-      e[k] = c * s * (a - ap) + (c * c - s * s) * b;
-      if (k + 1 < hi) {
-        bulge = -s * e[k + 1];
-        e[k + 1] = c * e[k + 1];
-        x = e[k];
-        z = bulge;
-      }
-      double* tk = &T[(size_t)k * n];
-      double* tk1 = &T[(size_t)(k + 1) * n];
-      for (int col = 0; col < n; col++) {
-        const double zk = tk[col], zk1 = tk1[col];
-        tk[col] = c * zk - s * zk1;
-        tk1[col] = s * zk + c * zk1;
-      }
-    }
-  }
-  return false;
+  // 2. implicit symmetric QR on (d, e), the rotations accumulated into Q
+  if (!tridiag_qr(d.data(), e.data(), n, T.data())) return false;
+  for (int i = 0; i < n; i++) ev[i] = d[i] * scale;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) Z[i * n + j] = T[(size_t)j * n + i];
+  return true;
 }
 
 

@@ -295,7 +295,7 @@ def test_block_solver_finds_eigenvalue_uniform_start_lanczos_misses(ctx, idx):
     1/sqrt(N) (lanczos.cu:64-72), returns nine eigenpairs that are each
     correct (FP64 residuals at the floor) but skips one of the nine smallest
     eigenvalues, whose eigenvector is nearly orthogonal to the start; the
-    block solver (random start, the mixed default) returns the true nine.
+    block solver (the mixed default) returns the true nine.
     Checked without a stored truth: both sets are eigenpairs, the block
     set is element-wise no larger, and it holds an eigenvalue below the
     Lanczos set's largest that the Lanczos set lacks."""
@@ -317,3 +317,39 @@ def test_block_solver_finds_eigenvalue_uniform_start_lanczos_misses(ctx, idx):
     assert rl["residual_max"] < 1e-6 and rb["residual_max"] < 1e-6
     assert np.all(eb <= el * (1 + 1e-5))
     assert missing and min(missing) < el.max()
+
+
+@pytest.mark.parametrize("start,first", [(0, 50), (1, 30)])
+def test_block_solver_start_and_first_degree(ctx, start, first):
+    """The block solver's other starting block (chebyshev_start 0: hashed
+    uniform values) and another first filter degree reach the same
+    eigenpairs as the defaults (cosine start, degree 50) on the 160x120
+    image: eigenvalues to 1e-5 relative, FP64 residuals below 1e-6, gPb to
+    1e-3."""
+    import bench as B
+
+    w, h = 160, 120
+    print(f"case: block solver start {start} first degree {first}, {w}x{h}", flush=True)
+    img = ctx.upload(dm.generate_image(w, h, 12, 20261202))
+    with dm.tuned(eigensolver=1):
+        ref = ctx.gpb(img, precision=dm.PREC_MIXED)
+        with dm.tuned(chebyshev_start=start, chebyshev_first_degree=first):
+            out = ctx.gpb(img, precision=dm.PREC_MIXED)
+    ev, er = out["eigvals"][1:].astype(np.float64), ref["eigvals"][1:].astype(np.float64)
+    assert np.max(np.abs(ev - er) / er) < 1e-5
+    assert B.fp64_residuals(ctx, out, w, h)["residual_max"] < 1e-6
+    assert np.abs(host(out["gpb"]) - host(ref["gpb"])).max() < 1e-3
+
+
+def test_chebyshev_tile_choice(ctx):
+    """dmsc_chebyshev_tile: with the tuning key at 0 the 64 x 21 read-ahead
+    tile (2337) where its grid fits in one round of the device's block slots
+    and the 64 x 16 one's does not (481x321 on a 256-CU MI355X: 512 against
+    672 blocks), else 2328; a set key wins over the size rule."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    tile = ctx.chebyshev_tile(481, 321)
+    assert tile == (2337 if cus == 256 else tile) and tile in (2337, 2328)
+    assert ctx.chebyshev_tile(1024, 768) == 2328
+    assert ctx.chebyshev_tile(321, 481) == 2328
+    with dm.tuned(chebyshev_shape=1324):
+        assert ctx.chebyshev_tile(481, 321) == 1324

@@ -38,11 +38,13 @@ void house(const double* x, int m, double* v, double* beta) {
 // Implicit symmetric QR with Wilkinson shifts (Algorithm 8.3.3) on the
 // tridiagonal (d, e), |entries| = O(1): d becomes the eigenvalues and the
 // rotations are applied to T (n x n, row k = column k of the current Q).
-// The rotations of a sweep form a dependent chain; their updates of T are
-// vector work that runs beside it.  (Eigenvalues alone, then inverse
-// iteration for the vectors, measured slower at n = 32: the value-only
-// sweeps keep the same chain, and the solves add chains of their own.)
-// False if 40 n sweeps do not converge.
+// The rotations of a sweep form a dependent chain (about 100 cycles per
+// rotation at n = 32, where the square root and the divisions sit); their
+// updates of T are vector work that runs beside it.  (Eigenvalues alone,
+// then inverse iteration for the vectors, measured slower at n = 32: the
+// value-only sweeps keep the same chain, and the solves add chains of their
+// own.  Applying a sweep's rotations to T afterwards, in registers, did not
+// shorten it either.)  False if 40 n sweeps do not converge.
 bool tridiag_qr(double* d, double* e, int n, double* T) {
   const double eps = 2.220446049250313e-16;
   for (int iter = 0; iter < 40 * n; iter++) {
@@ -115,7 +117,23 @@ __attribute__((target_clones("arch=haswell", "default"))) bool sym_eigen(const d
     house(x.data(), m, v.data(), &beta);
     if (beta == 0.0) continue;
     // p = beta A22 v, w = p - (beta p'v / 2) v, A22 <- A22 - v w' - w v'
-    for (int i = 0; i < m; i++) {
+    // (four rows' sums at a time: independent chains, each in column order)
+    int i4 = 0;
+    for (; i4 + 4 <= m; i4 += 4) {
+      const double* a0 = &A[(k + 1 + i4) * n + k + 1];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int j = 0; j < m; j++) {
+        s0 += a0[j] * v[j];
+        s1 += a0[n + j] * v[j];
+        s2 += a0[2 * n + j] * v[j];
+        s3 += a0[3 * n + j] * v[j];
+      }
+      p[i4] = beta * s0;
+      p[i4 + 1] = beta * s1;
+      p[i4 + 2] = beta * s2;
+      p[i4 + 3] = beta * s3;
+    }
+    for (int i = i4; i < m; i++) {
       double s = 0.0;
       for (int j = 0; j < m; j++) s += A[(k + 1 + i) * n + (k + 1 + j)] * v[j];
       p[i] = beta * s;
@@ -165,7 +183,24 @@ bool cholesky(std::vector<double>& L, int m) {
     if (!(d > 0.0)) return false;
     d = std::sqrt(d);
     L[j * m + j] = d;
-    for (int i = j + 1; i < m; i++) {
+    // the rows below, four at a time (independent chains, each in k order)
+    const double* lj = &L[j * m];
+    int i = j + 1;
+    for (; i + 4 <= m; i += 4) {
+      double* l0 = &L[i * m];
+      double s0 = l0[j], s1 = l0[m + j], s2 = l0[2 * m + j], s3 = l0[3 * m + j];
+      for (int k = 0; k < j; k++) {
+        s0 -= l0[k] * lj[k];
+        s1 -= l0[m + k] * lj[k];
+        s2 -= l0[2 * m + k] * lj[k];
+        s3 -= l0[3 * m + k] * lj[k];
+      }
+      l0[j] = s0 / d;
+      l0[m + j] = s1 / d;
+      l0[2 * m + j] = s2 / d;
+      l0[3 * m + j] = s3 / d;
+    }
+    for (; i < m; i++) {
       double s = L[i * m + j];
       for (int k = 0; k < j; k++) s -= L[i * m + k] * L[j * m + k];
       L[i * m + j] = s / d;
@@ -186,22 +221,29 @@ __attribute__((target_clones("arch=haswell", "default"))) bool rayleigh_ritz(con
   for (int i = 0; i < m; i++)
     for (int j = 0; j < i; j++) L[i * m + j] = L[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
   if (!cholesky(L, m)) return false;
-  // M = L^-1 H L^-T, with H symmetrised
-  std::vector<double> T((size_t)m * m), M((size_t)m * m);
-  for (int c = 0; c < m; c++)  // T = L^-1 H (forward substitution per column)
-    for (int i = 0; i < m; i++) {
-      double s = 0.5 * (H[i * m + c] + H[c * m + i]);
-      for (int k = 0; k < i; k++) s -= L[i * m + k] * T[k * m + c];
-      T[i * m + c] = s / L[i * m + i];
+  // M = L^-1 H L^-T, with H symmetrised.  The triangular solves run row by
+  // row with the columns side by side (each entry's sum still in k order).
+  std::vector<double> T((size_t)m * m), U((size_t)m * m), M((size_t)m * m), acc(m);
+  for (int i = 0; i < m; i++) {  // T = L^-1 H
+    for (int c = 0; c < m; c++) acc[c] = 0.5 * (H[i * m + c] + H[c * m + i]);
+    for (int k = 0; k < i; k++) {
+      const double l = L[i * m + k];
+      for (int c = 0; c < m; c++) acc[c] -= l * T[k * m + c];
     }
-  for (int r = 0; r < m; r++)  // M' = L^-1 T' (M is symmetric)
-    for (int i = 0; i < m; i++) {
-      double s = T[r * m + i];
-      for (int k = 0; k < i; k++) s -= L[i * m + k] * M[r * m + k];
-      M[r * m + i] = s / L[i * m + i];
+    for (int c = 0; c < m; c++) T[i * m + c] = acc[c] / L[i * m + i];
+  }
+  for (int i = 0; i < m; i++) {  // U = L^-1 T' = M' (M is symmetric)
+    for (int r = 0; r < m; r++) acc[r] = T[r * m + i];
+    for (int k = 0; k < i; k++) {
+      const double l = L[i * m + k];
+      for (int r = 0; r < m; r++) acc[r] -= l * U[k * m + r];
     }
-  for (int i = 0; i < m; i++)
-    for (int j = 0; j < i; j++) M[i * m + j] = M[j * m + i] = 0.5 * (M[i * m + j] + M[j * m + i]);
+    for (int r = 0; r < m; r++) U[i * m + r] = acc[r] / L[i * m + i];
+  }
+  for (int i = 0; i < m; i++) {
+    M[i * m + i] = U[i * m + i];
+    for (int j = 0; j < i; j++) M[i * m + j] = M[j * m + i] = 0.5 * (U[j * m + i] + U[i * m + j]);
+  }
   std::vector<double> ev(m), Z((size_t)m * m);
   if (!sym_eigen(M.data(), m, ev.data(), Z.data())) return false;
   std::vector<int> ord(m);
@@ -209,13 +251,14 @@ __attribute__((target_clones("arch=haswell", "default"))) bool rayleigh_ritz(con
   std::sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
   theta.resize(m);
   S.assign((size_t)m * m, 0.0);
-  for (int j = 0; j < m; j++) {
-    theta[j] = ev[ord[j]];
-    for (int i = m - 1; i >= 0; i--) {  // s_j = L^-T z_j (back substitution)
-      double s = Z[i * m + ord[j]];
-      for (int k = i + 1; k < m; k++) s -= L[k * m + i] * S[k * m + j];
-      S[i * m + j] = s / L[i * m + i];
+  for (int j = 0; j < m; j++) theta[j] = ev[ord[j]];
+  for (int i = m - 1; i >= 0; i--) {  // s_j = L^-T z_j (back substitution, all j side by side)
+    for (int j = 0; j < m; j++) acc[j] = Z[i * m + ord[j]];
+    for (int k = i + 1; k < m; k++) {
+      const double l = L[k * m + i];
+      for (int j = 0; j < m; j++) acc[j] -= l * S[k * m + j];
     }
+    for (int j = 0; j < m; j++) S[i * m + j] = acc[j] / L[i * m + i];
   }
   return true;
 }

@@ -115,10 +115,14 @@ __global__ void k_km_init(int w, int h, int nf, const float* __restrict__ resp, 
   pdot[p] = d;
 }
 
-// One k-means iteration is two kernels: accumulate, then assign (which
-// forms the centroids from the sums in every block).
+// A k-means iteration accumulates the int32 sums of the current labels,
+// then assigns: every assignment block forms the centroids from the sums.
+// The mixed path accumulates only in its first iteration: its assignment
+// also adds each pixel's responses, by the new label, to the next
+// iteration's sums, so each later iteration is a small clear kernel and the
+// assignment, and reads the responses once.
 // The stop test (kmeans.cu:712: no label changed) is taken at the start of
-// the next iteration's accumulation, from the change count the previous
+// the next iteration's kernels, from the change count the previous
 // assignment left; every block reads the same count, so all agree.
 __device__ __forceinline__ bool km_stopped(KmState* st, int it) {
   if (st->done) return true;
@@ -167,6 +171,33 @@ __global__ __launch_bounds__(256) void k_km_accum(int n, int nf, const float* __
   for (int i = threadIdx.x; i < nf * K; i += 256)
     if (smass[i]) atomicAdd(&st->mass[buf][i], smass[i]);
   if (threadIdx.x < K && scount[threadIdx.x]) atomicAdd(&st->count[buf][threadIdx.x], scount[threadIdx.x]);
+}
+
+// Iterations after the first: the next iteration's sums (read last as
+// centroids by the previous assignment) and this iteration's change count
+// (read last by the previous assignment's stop test) cleared before this
+// iteration's assignment adds to them.
+template <int K>
+__global__ __launch_bounds__(256) void k_km_clear(KmState* st, int it) {
+  if (st->done) return;
+  const int nb = (it + 1) & 1;
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) st->mass[nb][i] = 0;
+  if (threadIdx.x < K) st->count[nb][threadIdx.x] = 0;
+  if (threadIdx.x == 0) st->changes[it & 1] = 0;
+}
+
+// One pixel's int32 contributions (kmeans.cu findCentroids: (int)(1e5 r))
+// to the block's sums of cluster c, and the block's sums added to the next
+// iteration's buffer.
+__device__ __forceinline__ void km_add(uint32_t* smass, int K, int f, int c, float r) {
+  atomicAdd(&smass[f * K + c], (uint32_t)(int)(100000.0f * r));
+}
+template <int K>
+__device__ __forceinline__ void km_flush(const uint32_t* smass, const uint32_t* scount, KmState* st, int nf, int it) {
+  const int nb = (it + 1) & 1;
+  for (int i = threadIdx.x; i < nf * K; i += 256)
+    if (smass[i]) atomicAdd(&st->mass[nb][i], smass[i]);
+  if (threadIdx.x < K && scount[threadIdx.x]) atomicAdd(&st->count[nb][threadIdx.x], scount[threadIdx.x]);
 }
 
 __device__ __forceinline__ uint16_t to_bf16(float x) {  // round to nearest even
@@ -231,12 +262,17 @@ __device__ __forceinline__ int km_exact_label(const float (&x)[kNFilters], float
   return bl;
 }
 
-template <int K>
+// FUSE: also add each pixel's responses to the next iteration's sums (the
+// mixed path's scheme); the plain path accumulates in k_km_accum, whose
+// many blocks of one pixel per thread measured faster than the atomics
+// added to this kernel's tail.
+template <int K, bool FUSE>
 __global__ __launch_bounds__(256) void k_km_assign(int n, int nf, const float* __restrict__ resp,
                                                    const float* __restrict__ pdot, int* __restrict__ labels,
                                                    KmState* st, int it) {
-  if (st->done) return;
+  if (km_stopped(st, it)) return;
   __shared__ KmCent<K> sc;
+  __shared__ uint32_t smass[kNFilters * K], scount[K];
   __shared__ int schanges;
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int pp = p < n ? p : n - 1;
@@ -247,6 +283,8 @@ __global__ __launch_bounds__(256) void k_km_assign(int n, int nf, const float* _
   const float pd = pdot[pp];
   const int old = labels[pp];
   if (threadIdx.x == 0) schanges = 0;
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) smass[i] = 0;
+  if (threadIdx.x < K) scount[threadIdx.x] = 0;
   km_centroids<K>(sc, st, nf, it & 1);
   __syncthreads();
   int changed = 0;
@@ -254,10 +292,16 @@ __global__ __launch_bounds__(256) void k_km_assign(int n, int nf, const float* _
     const int bl = km_exact_label<K>(x, pd, sc);
     changed = bl != old;
     labels[p] = bl;
+    if (FUSE && bl >= 0) {  // -1: no finite distance, in no cluster
+#pragma unroll
+      for (int f = 0; f < kNFilters; f++) km_add(smass, K, f, bl, x[f]);
+      atomicAdd(&scount[bl], 1u);
+    }
   }
   const unsigned long long m = __ballot(changed);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&schanges, __popcll(m));
   __syncthreads();
+  if (FUSE) km_flush<K>(smass, scount, st, nf, it);
   if (threadIdx.x == 0) {
     if (schanges) atomicAdd(&st->changes[it & 1], schanges);
     if (blockIdx.x == 0) st->iterations = it + 1;
@@ -293,8 +337,10 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, int nf, const flo
                                                         const float* __restrict__ pdot, int* __restrict__ labels,
                                                         KmState* st, int it) {
   constexpr int NT = K / 16;  // cluster tiles
-  if (st->done) return;
+  static_assert(kNFilters == 34, "filters 0..31 on the four lane groups, 32 and 33 on groups 0 and 1");
+  if (km_stopped(st, it)) return;
   __shared__ KmCent<K> sc;
+  __shared__ uint32_t smass[kNFilters * K], scount[K];
   __shared__ __attribute__((aligned(16))) uint16_t scb[K][32], scbl[K][32];  // BF16 split of filters 0..31
   __shared__ int schanges, srecheck;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -315,6 +361,8 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, int nf, const flo
     oldl[tile] = labels[pp];
   }
   if (threadIdx.x == 0) schanges = srecheck = 0;
+  for (int i = threadIdx.x; i < kNFilters * K; i += 256) smass[i] = 0;
+  if (threadIdx.x < K) scount[threadIdx.x] = 0;
   km_centroids<K>(sc, st, nf, it & 1);
   for (int i = threadIdx.x; i < K * 32; i += 256) {
     const int c = i / 32, f = i - c * 32;
@@ -386,8 +434,9 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, int nf, const flo
       const float e = 6.103515625e-05f * xn * sc.cnorm[bc] + 9.5367431640625e-7f * (pd + sc.cdot[bc]);
       certain = best + e < lo_other && best + e < 10000000.0f;
     }
+    int fl = -1;
     if (grp == 0 && live) {
-      int fl = bc;
+      fl = bc;
       if (!certain) {  // near tie (or no finite distance): exact FP32 distances
         float x[kNFilters];
 #pragma unroll
@@ -398,10 +447,24 @@ __global__ __launch_bounds__(256) void k_km_assign_mfma(int n, int nf, const flo
       changed += fl != oldl[tile];
       labels[p] = fl;
     }
+    // the pixel's sums by its new label: filters 8 grp .. 8 grp + 7 on each
+    // of its four lanes, filter 32 (and the count) on group 0, 33 on group 1
+    const int lab = __shfl(fl, col, 64);
+    if (live && lab >= 0) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) km_add(smass, K, 8 * grp + j, lab, xv[tile][j]);
+      if (grp == 0) {
+        km_add(smass, K, 32, lab, x32);
+        atomicAdd(&scount[lab], 1u);
+      } else if (grp == 1) {
+        km_add(smass, K, 33, lab, resp[(size_t)33 * n + p]);
+      }
+    }
   }
   if (changed) atomicAdd(&schanges, changed);
   if (rech) atomicAdd(&srecheck, rech);
   __syncthreads();
+  km_flush<K>(smass, scount, st, nf, it);
   if (threadIdx.x == 0) {
     if (schanges) atomicAdd(&st->changes[it & 1], schanges);
     if (srecheck) atomicAdd(&st->rechecks, (unsigned)srecheck);
@@ -430,12 +493,16 @@ void km_iterations(hipStream_t s, int n, int nf, const float* resp, int* labels,
                    int max_iter, bool mixed) {
   const int blocks_acc = std::min(div_up(n, 256), 1024);
   for (int it = 0; it < max_iter; it++) {
-    hipLaunchKernelGGL(k_km_accum<K>, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
+    if (it == 0 || !mixed)
+      hipLaunchKernelGGL(k_km_accum<K>, dim3(blocks_acc), dim3(256), 0, s, n, nf, resp, labels, st, it);
+    else
+      hipLaunchKernelGGL(k_km_clear<K>, dim3(1), dim3(256), 0, s, st, it);
     if (mixed) {
       hipLaunchKernelGGL(k_km_assign_mfma<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, nf, resp, pdot, labels, st,
                          it);
     } else
-      hipLaunchKernelGGL(k_km_assign<K>, dim3(div_up(n, 256)), dim3(256), 0, s, n, nf, resp, pdot, labels, st, it);
+      hipLaunchKernelGGL((k_km_assign<K, false>), dim3(div_up(n, 256)), dim3(256), 0, s, n, nf, resp, pdot, labels, st,
+                         it);
   }
 }
 

@@ -562,10 +562,10 @@ __global__ __launch_bounds__(256) void k_cheb_resid_rotate(int n, size_t vs, con
                                                            const double* __restrict__ theta, int nvec,
                                                            double* __restrict__ part, int nout,
                                                            float4* __restrict__ planes, float* __restrict__ out) {
-  __shared__ double ss[kB * kB];
+  // S is the same for every pixel: its entries are read with scalar loads
+  // (wave-uniform addresses) straight into the FMAs, not staged in LDS
   __shared__ double th[kResidMax];
   __shared__ double red[kResidMax][4];
-  for (int k = threadIdx.x; k < kB * kB; k += 256) ss[k] = S[k];
   if (threadIdx.x < kResidMax) th[threadIdx.x] = threadIdx.x < nvec ? theta[threadIdx.x] : 0.0;
   __syncthreads();
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void k_cheb_resid_rotate(int n, size_t vs, con
 #pragma unroll 8
       for (int i = 0; i < kB; i++) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) o[k] = fma((double)yv[i], ss[i * kB + j0 + k], o[k]);
+        for (int k = 0; k < 4; k++) o[k] = fma((double)yv[i], S[i * kB + j0 + k], o[k]);
       }
       if (j0 < kResidMax) {
 #pragma unroll
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void k_cheb_resid_rotate(int n, size_t vs, con
       if (j >= nvec) break;
       double ws = 0.0;
 #pragma unroll 8
-      for (int i = 0; i < kB; i++) ws = fma((double)wv[i], ss[i * kB + j], ws);
+      for (int i = 0; i < kB; i++) ws = fma((double)wv[i], S[i * kB + j], ws);
       const double r = fma(-th[j], ys[j], ws);
       r2[j] = r * r;
     }
